@@ -1,0 +1,195 @@
+/*
+ * sjf.h -- C ABI of the MI355X FDTD engine (libsjf.so).
+ *
+ * The engine replaces the meep objects that sim_juncs' driver `bound_geom` drives
+ * (reference src/disp.hpp:142-200, src/disp.cpp:482-749).  Every entry point below names
+ * the reference call it stands in for.  No C++ or HIP types cross this boundary: opaque
+ * handle, plain pointers and sizes, `int` status codes (0 = success), never an exception.
+ *
+ * Units are meep units (c = 1, length unit a = 1/um_scale um; reference Appendix of
+ * src/disp.cpp:5-22, src/disp.hpp:157-158).  Grid conventions:
+ *   cell (i,j,k), i < nx, j < ny, k < nz; linear index (i*ny + j)*nz + k (z fastest);
+ *   Ex at ((i+1/2)dx, j dx, k dx), Ey at (i dx, (j+1/2)dx, k dx), Ez at (i dx, j dx, (k+1/2)dx),
+ *   Hx at (i dx, (j+1/2)dx, (k+1/2)dx), Hy at ((i+1/2)dx, j dx, (k+1/2)dx),
+ *   Hz at ((i+1/2)dx, (j+1/2)dx, k dx).
+ *   After n steps E is at time n*dt and H at (n - 1/2)*dt, dt = courant*dx.
+ *   Fields outside the grid are zero (metallic outer edge) unless an axis is periodic.
+ *
+ * Threading: one engine owns one HIP stream and is not thread-safe; separate engines may
+ * be driven from separate threads.  Entry points enqueue work on the engine's stream and
+ * return; the read functions synchronise that stream before copying.
+ */
+#ifndef SJF_H
+#define SJF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sjf_engine sjf_engine;
+
+/* status codes (mirror the reference's parse_ercode style: src/cgs_read.hpp:32) */
+enum {
+    SJF_OK = 0,
+    SJF_E_ARG = -1,         /* invalid argument */
+    SJF_E_HIP = -2,         /* HIP runtime error (message in sjf_last_error) */
+    SJF_E_STATE = -3,       /* call not valid in the engine's current state */
+    SJF_E_UNSUPPORTED = -4, /* valid request outside what this build supports */
+    SJF_E_NOMEM = -5        /* device or host allocation failed */
+};
+
+/* field components (meep::Ex..meep::Hz order used by the reference, src/disp.cpp:335-341) */
+enum { SJF_EX = 0, SJF_EY = 1, SJF_EZ = 2, SJF_HX = 3, SJF_HY = 4, SJF_HZ = 5 };
+
+enum { SJF_FP32 = 4, SJF_FP64 = 8 };
+enum { SJF_SRC_GAUSSIAN = 0, SJF_SRC_CONTINUOUS = 1 };
+
+#define SJF_MAX_POLES 4
+#define SJF_MAX_MATERIALS 255 /* material id 255 is reserved for design voxels */
+#define SJF_MAX_DFT_FREQS 64
+
+typedef struct {
+    int nx, ny, nz;  /* cells per axis; any parity (nz <= 1024) */
+    double dx;       /* cell size in meep length units (1/resolution) */
+    double courant;  /* dt = courant * dx; 0.5 is meep's default and what the reference runs
+                        (the reference parses `courant` but never passes it: src/argparse.h:142,
+                        src/disp.cpp:527) */
+    int precision;   /* SJF_FP32 or SJF_FP64 field storage and arithmetic */
+    int periodic[3]; /* per axis: 0 metallic outer edge, 1 periodic */
+    int part;        /* 0: evolve the real part of meep's complex fields, 1: the imaginary
+                        part (the reference never calls use_real_fields(), so meep runs
+                        complex fields; with real materials the parts are independent) */
+    int device;      /* HIP device ordinal */
+} sjf_grid;
+
+/* One material: eps_inf plus Lorentz/Drude poles, meep units (f0 and gamma already divided
+ * by um_scale, sigma already divided by the 2-D thickness: src/disp.cpp:537-546).
+ * chi(omega) = sum sigma w0^2 / (L w0^2 - w^2 - i w gamma'), w0 = 2 pi f0, gamma' = 2 pi gamma,
+ * L = 0 for Drude ("drude" poles: lorentzian_susceptibility(..., no_omega_0_denominator)). */
+typedef struct {
+    double eps_inf;
+    int npoles;
+    double f0[SJF_MAX_POLES];
+    double gamma[SJF_MAX_POLES];
+    double sigma[SJF_MAX_POLES];
+    int is_drude[SJF_MAX_POLES];
+} sjf_material;
+
+/* Source time function.  SJF_SRC_GAUSSIAN restates gaussian_src_time_phase
+ * (src/disp.cpp:378-400): omega = 2 pi freq, phi' = phase + pi, peak = (t_start+t_end)/2,
+ * cutoff = (t_end - t_start)/2 shrunk while exp(-cutoff^2/2w^2) < 1e-100 then rounded to
+ * float, dipole(t) = exp(-tt^2/2w^2) exp(-i(omega tt + phi')) / (-i omega) for
+ * float(|tt|) <= cutoff.  SJF_SRC_CONTINUOUS restates meep::continuous_src_time
+ * (src/disp.cpp:618): exp(-i omega t)/(-i omega) with a tanh turn-on of `width` and
+ * `slowness`.  The injected current is (dipole(t_{n+1}) - dipole(t_n)) / dt. */
+typedef struct {
+    int kind;
+    double freq;     /* meep frequency = 1/(wavelength_um * um_scale) (src/disp.cpp:606) */
+    double width;    /* meep time units (src/disp.cpp:607) */
+    double phase;    /* CEP phase, radians */
+    double t_start;  /* meep time units (src/disp.cpp:608) */
+    double t_end;    /* meep time units (src/disp.cpp:609) */
+    double slowness; /* continuous only; meep's default is 3 */
+} sjf_pulse;
+
+/* ---- lifetime ------------------------------------------------------------------------ */
+
+/* Replaces meep::vol3d(...) + new meep::structure(...) + meep::fields(strct)
+ * (src/disp.cpp:509,527,560).  Allocates all field arrays on `g->device`. */
+int sjf_create(const sjf_grid* g, sjf_engine** out);
+/* Replaces ~bound_geom's delete strct (src/disp.cpp:647-649). */
+int sjf_destroy(sjf_engine* e);
+/* Message of the last failure on `e` (or of the last failed sjf_create when e == NULL). */
+const char* sjf_last_error(const sjf_engine* e);
+
+/* ---- structure ----------------------------------------------------------------------- */
+
+/* Replaces the structure's chi1p1 sampling and add_susceptibility calls
+ * (src/disp.cpp:515-548, in_bound at src/disp.cpp:264-283).  mat_word holds one uint32 per
+ * cell: bits 0-7, 8-15, 16-23 = material id of the cell's Ex, Ey, Ez Yee point (ids index
+ * `lut`, n_mat <= 255).  Resets all polarisation state. */
+int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_material* lut,
+                      int n_mat);
+/* Per-voxel permittivity (no poles) over the cell box [lo, hi).  New; used for the design
+ * region of the adjoint workloads.  eps is (hi-lo) sized, z fastest. */
+int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const double* eps);
+/* Replaces meep::pml(thickness) (src/disp.cpp:527).  cells = {x_lo, x_hi, y_lo, y_hi, z_lo,
+ * z_hi} PML thicknesses in cells; R = asymptotic reflection (meep default 1e-15); power =
+ * profile exponent (meep default 2). */
+int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power);
+
+/* ---- sources, monitors, DFT ------------------------------------------------------------ */
+
+/* Replaces fields::add_volume_source(c, src, volume, amp) (src/disp.cpp:603,614,619,680) and
+ * add_point_source (src/disp.cpp:659, zero-extent box).  comp in Ex..Ez.  box = {x0,y0,z0,
+ * x1,y1,z1} in meep units; an axis of zero extent is a delta function (linear weights / dx),
+ * otherwise cells weigh by their overlap with the box. */
+int sjf_add_source(sjf_engine* e, int comp, const double box[6], const sjf_pulse* p,
+                   double amp_re, double amp_im);
+/* Replaces fields::last_source_time() (src/disp.cpp:625). */
+int sjf_last_source_time(const sjf_engine* e, double* t);
+/* Replaces fields.dt and fields.time() (src/disp.cpp:704,733). */
+int sjf_dt(const sjf_engine* e, double* dt);
+int sjf_time(const sjf_engine* e, double* t);
+int sjf_step_count(const sjf_engine* e, long* n);
+
+/* Replaces the per-span fields.get_field(c, loc) loop of bound_geom::run
+ * (src/disp.cpp:721-730): registers points sampled on the device, by trilinear interpolation
+ * on the component's lattice, at the start of every step whose index is a multiple of the
+ * save span (sample, then step, as src/disp.cpp:721-740). */
+int sjf_add_monitors(sjf_engine* e, int comp, const double* xyz, int n);
+/* save_span of bound_geom (src/disp.hpp:192, src/disp.cpp:699: 0 means 1). */
+int sjf_set_save_span(sjf_engine* e, int span);
+/* out[s][p][2] = (value, 0) for sample s, point p.  *n_samples = samples taken so far; at
+ * most cap doubles are written. */
+int sjf_read_monitors(sjf_engine* e, double* out, size_t cap, long* n_samples);
+/* Replaces fields::get_field(c, vec) (src/disp.cpp:724) for a one-off read. */
+int sjf_get_field(sjf_engine* e, int comp, const double xyz[3], double out[2]);
+
+/* DFT accumulation over the cell box [lo, hi) (new: the reference only FFTs monitor series
+ * after the run, src/disp.cpp:796-806).  After each step adds F(t) exp(+i 2 pi f t) dt with t
+ * = the component's own time, in FP64 with Neumaier-compensated sums.  comp_mask bit c
+ * selects component c.  nf <= SJF_MAX_DFT_FREQS. */
+int sjf_add_dft(sjf_engine* e, const int lo[3], const int hi[3], int comp_mask,
+                const double* freqs, int nf, int* id);
+/* out[c][f][cell][2] for the masked components in ascending order. */
+int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap);
+
+/* ---- time stepping ---------------------------------------------------------------------- */
+
+/* Replaces fields.step() (src/disp.cpp:740).  n = 1 is the exact drop-in; n > 1 runs n steps
+ * back to back (with the monitor sampling rule above). */
+int sjf_step(sjf_engine* e, long n);
+/* Waits for all work enqueued on the engine's stream. */
+int sjf_synchronize(sjf_engine* e);
+/* Runs n steps and reports, from HIP events on the engine's stream, the wall time of the
+ * whole region and the mean duration of the fused update kernel (ms). */
+int sjf_timed_steps(sjf_engine* e, long n, double* ms_total, double* ms_kernel_mean);
+
+/* ---- field I/O --------------------------------------------------------------------------- */
+
+/* Replaces output_hdf5 (src/disp.cpp:696,735) as the data source: copies a whole component
+ * (nx*ny*nz values, converted to FP64) to host memory. */
+int sjf_read_field(sjf_engine* e, int comp, double* out);
+/* Sets a whole component from host FP64 values (initial conditions, tests). */
+int sjf_write_field(sjf_engine* e, int comp, const double* in);
+
+/* ---- accounting -------------------------------------------------------------------------- */
+
+/* Algorithmic HBM bytes one launch of the fused update kernel must move (E/H read+write,
+ * material word, polarisation state, PML auxiliaries), computed from the actual masks. */
+int sjf_step_bytes(const sjf_engine* e, double* bytes);
+/* Launch configuration of the fused update kernel: rows per thread, x-chunk length,
+ * threads per workgroup, workgroups. */
+int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int* groups);
+/* Override the autotuned tile shape (ty in {1,2,4,8}, xchunk >= 1). */
+int sjf_set_tile(sjf_engine* e, int ty, int xchunk);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SJF_H */

@@ -1,0 +1,417 @@
+// sjf_kernels.h -- gfx950 kernels of the FDTD engine (device code only).
+//
+// Hot path: one fused launch per time step (k_step) that performs, for every cell,
+//   H^{n+1/2} = H^{n-1/2} - (dt/dx) (dE + psi_H)                       (Yee curl, CPML)
+//   Q_k^{n+1} = alpha_k Q_k^n + beta_k P_k^n + c3_k E^n, P_k^{n+1} = P_k^n + Q_k^{n+1}
+//   E^{n+1}   = E^n + inv_eps ((dt/dx) (dH + psi_E) - sum_k Q_k^{n+1})
+// This is the E-direct form of meep's D/E split (src/disp.cpp:740 fields.step(): B, H, D,
+// polarisation, E): with D = eps_inf E + sum P, D^{n+1} - D^n = dt curl H, and
+// Q = P^{n+1} - P^n the textbook Lorentz/Drude recurrence becomes the (P, Q) pair above,
+// which needs no D array and, for Drude poles, no P array.
+//
+// Layout in HBM: each component is a dense nx*ny*nz array, z fastest.  E and H and the H
+// auxiliaries are double-buffered (read buffer 0, write buffer 1, swapped every step) because
+// neighbouring workgroups recompute one halo row of H from the old state; polarisation state
+// and the E auxiliaries are only touched by the owning thread and are updated in place.
+//
+// Work decomposition (MI355X: 64-wide wavefronts, 8 XCDs with private L2):
+//   * one workgroup spans the whole z line (blockDim = nz rounded up to 64, <= 1024), so the
+//     z neighbours of H come from LDS and the z halo disappears;
+//   * each thread owns TY consecutive y rows (register blocking) and recomputes H on one
+//     extra row below them (the y halo);
+//   * the workgroup marches along x over a chunk of planes, carrying E^n of the next plane
+//     and H^{n+1/2} of the previous plane in registers (no x halo except one plane per chunk);
+//   * workgroup ids are remapped so that consecutive tiles land on the same XCD and share
+//     halo rows in that XCD's L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sjf {
+
+constexpr int kMaxPoles = 4;
+constexpr int kDesignMat = 255;
+constexpr int kMaxDftFreqs = 64;
+
+template <typename T>
+struct MatLut {
+    T inv_eps;
+    int np;
+    int lorentz_mask;
+    int pad;
+    T alpha[kMaxPoles];
+    T beta[kMaxPoles];
+    T c3[kMaxPoles];
+};
+
+template <typename T>
+struct StepArgs {
+    int nx, ny, nz;
+    int px, py, pz;
+    int nty, xc;
+    T cdt;
+    const T* E0[3];
+    const T* H0[3];
+    T* E1[3];
+    T* H1[3];
+    int uniform;  // 1: single non-dispersive material, no material word read
+    int nmat;
+    const uint32_t* matw;
+    const MatLut<T>* lut;
+    T* Q[kMaxPoles][3];
+    T* P[kMaxPoles][3];
+    int pml_on[3];
+    int plo[3], phi[3];
+    const T* bE[3];
+    const T* aE[3];
+    const T* bH[3];
+    const T* aH[3];
+    T* psiE[3][2];
+    const T* psiH0[3][2];
+    T* psiH1[3][2];
+    int dlo[3], dhi[3];
+    const T* dinv;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, long q, bool ok) {
+    return ok ? p[q] : T(0);
+}
+
+__device__ __forceinline__ int wrap_or_ghost(int c, int n, int periodic) {
+    if (c >= 0 && c < n) return c;
+    if (!periodic) return -1;
+    return c < 0 ? c + n : c - n;
+}
+
+template <typename T>
+__device__ __forceinline__ bool in_pml(const StepArgs<T>& a, int d, int c, int n) {
+    return a.pml_on[d] && (c < a.plo[d] || c >= n - a.phi[d]);
+}
+
+template <typename T>
+__device__ __forceinline__ long slab_x(const StepArgs<T>& a, int i, int j, int k) {
+    int s = i < a.plo[0] ? i : i - (a.nx - a.phi[0]) + a.plo[0];
+    return ((long)s * a.ny + j) * a.nz + k;
+}
+template <typename T>
+__device__ __forceinline__ long slab_y(const StepArgs<T>& a, int i, int j, int k) {
+    int s = j < a.plo[1] ? j : j - (a.ny - a.phi[1]) + a.plo[1];
+    return ((long)i * (a.plo[1] + a.phi[1]) + s) * a.nz + k;
+}
+template <typename T>
+__device__ __forceinline__ long slab_z(const StepArgs<T>& a, int i, int j, int k) {
+    int s = k < a.plo[2] ? k : k - (a.nz - a.phi[2]) + a.plo[2];
+    return ((long)i * a.ny + j) * (a.plo[2] + a.phi[2]) + s;
+}
+
+// CPML update of one auxiliary: psi <- b psi + a d; returns d + psi.
+template <typename T>
+__device__ __forceinline__ T cpml(T d, T b, T c, T& psi) {
+    psi = b * psi + c * d;
+    return d + psi;
+}
+
+template <typename T, int TY, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int B = blockDim.x;
+    T* sh = reinterpret_cast<T*>(smem_raw);  // [2 parity][2 comp][TY][B]
+    MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sh + 4 * TY * B);
+
+    for (int q = threadIdx.x; q < a.nmat; q += B) slut[q] = a.lut[q];
+
+    // XCD-aware tile order: hardware places workgroup w on XCD w % 8; map so that each XCD
+    // receives a contiguous run of logical tiles (y-tile fastest, then x-chunk).
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    const int tyi = t % a.nty, xci = t / a.nty;
+    const int j0 = tyi * TY;
+    const int x0 = xci * a.xc;
+    const int x1 = min(x0 + a.xc, a.nx);
+    const int k = threadIdx.x;
+    const bool kin = k < a.nz;
+    const int kp1 = wrap_or_ghost(k + 1, a.nz, a.pz);
+    const int km1 = wrap_or_ghost(k - 1, a.nz, a.pz);
+    const long NZ = a.nz;
+    const long NYZ = (long)a.ny * a.nz;
+    const T cdt = a.cdt;
+
+    int gj[TY + 2];  // global row of local row r-1 (r = 0 is the halo row below the tile)
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) gj[r] = wrap_or_ghost(j0 + r - 1, a.ny, a.py);
+
+    const bool zpml = kin && in_pml(a, 2, k, a.nz);
+    const T bEz = zpml ? a.bE[2][k] : T(0), aEz = zpml ? a.aE[2][k] : T(0);
+    const T bHz = zpml ? a.bH[2][k] : T(0), aHz = zpml ? a.aH[2][k] : T(0);
+
+    __syncthreads();
+
+    T ec[TY + 2][3];  // E^n on plane i, rows -1..TY
+    T en[TY + 2][3];  // E^n on plane i+1
+    T hp[TY][2];      // H^{n+1/2} (Hy, Hz) on plane i-1, rows 0..TY-1
+
+    const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
+    {
+        const int pi = wrap_or_ghost(istart, a.nx, a.px);
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            const bool ok = kin && gj[r] >= 0 && pi >= 0;
+            const long q = (long)pi * NYZ + (long)gj[r] * NZ + k;
+            ec[r][0] = ldg(a.E0[0], q, ok);
+            ec[r][1] = ldg(a.E0[1], q, ok);
+            ec[r][2] = ldg(a.E0[2], q, ok);
+        }
+#pragma unroll
+        for (int r = 0; r < TY; ++r) hp[r][0] = hp[r][1] = T(0);
+    }
+
+    for (int i = istart; i < x1; ++i) {
+        const bool own_plane = i >= x0;
+        const int pi = wrap_or_ghost(i, a.nx, a.px);
+        const int pn = wrap_or_ghost(i + 1, a.nx, a.px);
+        const bool xpml = in_pml(a, 0, pi, a.nx);
+        const T bEx = xpml ? a.bE[0][pi] : T(0), aEx = xpml ? a.aE[0][pi] : T(0);
+        const T bHx = xpml ? a.bH[0][pi] : T(0), aHx = xpml ? a.aH[0][pi] : T(0);
+
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            const bool ok = kin && gj[r] >= 0 && pn >= 0;
+            const long q = (long)pn * NYZ + (long)gj[r] * NZ + k;
+            en[r][0] = ldg(a.E0[0], q, ok);
+            en[r][1] = (r < TY + 1) ? ldg(a.E0[1], q, ok) : T(0);
+            en[r][2] = ldg(a.E0[2], q, ok);
+        }
+
+        // ---- H^{n+1/2} on rows -1..TY-1 of plane i -------------------------------------
+        T h[TY + 1][3];
+#pragma unroll
+        for (int r = 0; r < TY + 1; ++r) {
+            const int j = gj[r];
+            const bool ok = kin && j >= 0;
+            const long q = (long)pi * NYZ + (long)j * NZ + k;
+            const bool wr = own_plane && r >= 1 && (j0 + r - 1) < a.ny && ok;
+            const T hx0 = ldg(a.H0[0], q, ok);
+            const T hy0 = ldg(a.H0[1], q, ok);
+            const T hz0 = ldg(a.H0[2], q, ok);
+            const bool okk = ok && kp1 >= 0;
+            const long qk = q - k + kp1;
+            const T exk1 = ldg(a.E0[0], qk, okk);
+            const T eyk1 = ldg(a.E0[1], qk, okk);
+            T dEz_y = ec[r + 1][2] - ec[r][2];
+            T dEx_y = ec[r + 1][0] - ec[r][0];
+            T dEy_z = eyk1 - ec[r][1];
+            T dEx_z = exk1 - ec[r][0];
+            T dEz_x = en[r][2] - ec[r][2];
+            T dEy_x = en[r][1] - ec[r][1];
+            if (ok && in_pml(a, 1, j, a.ny)) {
+                const long s = slab_y(a, pi, j, k);
+                const T b = a.bH[1][j], c = a.aH[1][j];
+                T p0 = a.psiH0[1][0][s], p1 = a.psiH0[1][1][s];
+                dEz_y = cpml(dEz_y, b, c, p0);
+                dEx_y = cpml(dEx_y, b, c, p1);
+                if (wr) { a.psiH1[1][0][s] = p0; a.psiH1[1][1][s] = p1; }
+            }
+            if (zpml && ok) {
+                const long s = slab_z(a, pi, j, k);
+                T p0 = a.psiH0[2][0][s], p1 = a.psiH0[2][1][s];
+                dEy_z = cpml(dEy_z, bHz, aHz, p0);
+                dEx_z = cpml(dEx_z, bHz, aHz, p1);
+                if (wr) { a.psiH1[2][0][s] = p0; a.psiH1[2][1][s] = p1; }
+            }
+            if (xpml && ok) {
+                const long s = slab_x(a, pi, j, k);
+                T p0 = a.psiH0[0][0][s], p1 = a.psiH0[0][1][s];
+                dEz_x = cpml(dEz_x, bHx, aHx, p0);
+                dEy_x = cpml(dEy_x, bHx, aHx, p1);
+                if (wr) { a.psiH1[0][0][s] = p0; a.psiH1[0][1][s] = p1; }
+            }
+            // outside a metallic edge H is identically zero (ghost row of the halo)
+            h[r][0] = ok ? hx0 - cdt * (dEz_y - dEy_z) : T(0);
+            h[r][1] = ok ? hy0 - cdt * (dEx_z - dEz_x) : T(0);
+            h[r][2] = ok ? hz0 - cdt * (dEy_x - dEx_y) : T(0);
+            if (wr) {
+                a.H1[0][q] = h[r][0];
+                a.H1[1][q] = h[r][1];
+                a.H1[2][q] = h[r][2];
+            }
+        }
+
+        if (own_plane) {
+            T* buf = sh + (i & 1) * 2 * TY * B;
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                buf[r * B + k] = h[r + 1][0];
+                buf[(TY + r) * B + k] = h[r + 1][1];
+            }
+            __syncthreads();
+
+            // ---- E^{n+1} on rows 0..TY-1 of plane i ------------------------------------
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                const int j = gj[r + 1];
+                if (!(kin && (j0 + r) < a.ny)) continue;
+                const long q = (long)pi * NYZ + (long)j * NZ + k;
+                const T hxkm = km1 >= 0 ? buf[r * B + km1] : T(0);
+                const T hykm = km1 >= 0 ? buf[(TY + r) * B + km1] : T(0);
+                T dHz_y = h[r + 1][2] - h[r][2];
+                T dHx_y = h[r + 1][0] - h[r][0];
+                T dHy_z = h[r + 1][1] - hykm;
+                T dHx_z = h[r + 1][0] - hxkm;
+                T dHz_x = h[r + 1][2] - hp[r][1];
+                T dHy_x = h[r + 1][1] - hp[r][0];
+                if (in_pml(a, 1, j, a.ny)) {
+                    const long s = slab_y(a, pi, j, k);
+                    const T b = a.bE[1][j], c = a.aE[1][j];
+                    T p0 = a.psiE[1][0][s], p1 = a.psiE[1][1][s];
+                    dHz_y = cpml(dHz_y, b, c, p0);
+                    dHx_y = cpml(dHx_y, b, c, p1);
+                    a.psiE[1][0][s] = p0;
+                    a.psiE[1][1][s] = p1;
+                }
+                if (zpml) {
+                    const long s = slab_z(a, pi, j, k);
+                    T p0 = a.psiE[2][0][s], p1 = a.psiE[2][1][s];
+                    dHy_z = cpml(dHy_z, bEz, aEz, p0);
+                    dHx_z = cpml(dHx_z, bEz, aEz, p1);
+                    a.psiE[2][0][s] = p0;
+                    a.psiE[2][1][s] = p1;
+                }
+                if (xpml) {
+                    const long s = slab_x(a, pi, j, k);
+                    T p0 = a.psiE[0][0][s], p1 = a.psiE[0][1][s];
+                    dHz_x = cpml(dHz_x, bEx, aEx, p0);
+                    dHy_x = cpml(dHy_x, bEx, aEx, p1);
+                    a.psiE[0][0][s] = p0;
+                    a.psiE[0][1][s] = p1;
+                }
+                const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
+                if (a.uniform) {
+                    const T inv = slut[0].inv_eps;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a.E1[c][q] = ec[r + 1][c] + inv * (cdt * curl[c]);
+                } else {
+                    const uint32_t mw = a.matw[q];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const int m = (mw >> (8 * c)) & 0xff;
+                        const T e0 = ec[r + 1][c];
+                        T inv, sq = T(0);
+                        if (m == kDesignMat) {
+                            const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) +
+                                             (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
+                                            (k - a.dlo[2]);
+                            inv = a.dinv[dq];
+                        } else {
+                            const MatLut<T>& L = slut[m];
+                            inv = L.inv_eps;
+                            for (int s = 0; s < L.np; ++s) {
+                                T* Qp = a.Q[s][c];
+                                T qn = L.alpha[s] * Qp[q] + L.c3[s] * e0;
+                                if ((L.lorentz_mask >> s) & 1) {
+                                    T* Pp = a.P[s][c];
+                                    const T p = Pp[q];
+                                    qn += L.beta[s] * p;
+                                    Pp[q] = p + qn;
+                                }
+                                Qp[q] = qn;
+                                sq += qn;
+                            }
+                        }
+                        a.E1[c][q] = e0 + inv * (cdt * curl[c] - sq);
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            hp[r][0] = h[r + 1][1];
+            hp[r][1] = h[r + 1][2];
+        }
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            ec[r][0] = en[r][0];
+            ec[r][1] = en[r][1];
+            ec[r][2] = en[r][2];
+        }
+    }
+}
+
+// ---- source injection: E -= coef * J on the source's cells (coef = dt * weight * inv_eps)
+template <typename T>
+__global__ void k_source(T* F, const long* idx, const double* coef, long n, double J) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long q = idx[e];
+    F[q] = (T)((double)F[q] - coef[e] * J);
+}
+
+// ---- point monitors: trilinear gather, FP64 result --------------------------------------
+template <typename T>
+struct FieldPtrs {
+    const T* f[6];
+};
+
+template <typename T>
+__global__ void k_monitor(FieldPtrs<T> F, const long* idx8, const double* w8, const int* comp,
+                          int npt, double* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npt) return;
+    const T* f = F.f[comp[p]];
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const long id = idx8[8 * p + q];
+        if (id >= 0) v += w8[8 * p + q] * (double)f[id];
+    }
+    out[p] = v;
+}
+
+// ---- DFT: Neumaier-compensated FP64 accumulation of F exp(i 2 pi f t) dt ----------------
+template <typename T>
+struct DftArgs {
+    FieldPtrs<T> F;
+    int lo[3], ext[3];
+    long ncell;
+    int comps[6];
+    int ncomp;
+    int nf;
+    double2* acc;  // [slot][f][cell]
+    double2* cmp;  // compensation terms, same layout
+    double2 ph[2][kMaxDftFreqs];  // [E/H][f] = (cos, sin) * dt at the component's time
+};
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+template <typename T>
+__global__ void k_dft(const DftArgs<T> a, long nyz, long nz) {
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncell) return;
+    const int kk = (int)(cell % a.ext[2]);
+    const long rest = cell / a.ext[2];
+    const int jj = (int)(rest % a.ext[1]);
+    const int ii = (int)(rest / a.ext[1]);
+    const long q = (long)(a.lo[0] + ii) * nyz + (long)(a.lo[1] + jj) * nz + (a.lo[2] + kk);
+    for (int s = 0; s < a.ncomp; ++s) {
+        const int c = a.comps[s];
+        const double v = (double)a.F.f[c][q];
+        const double2* ph = a.ph[c < 3 ? 0 : 1];
+        for (int f = 0; f < a.nf; ++f) {
+            const long o = ((long)s * a.nf + f) * a.ncell + cell;
+            double2 sm = a.acc[o], cp = a.cmp[o];
+            neumaier(sm.x, cp.x, v * ph[f].x);
+            neumaier(sm.y, cp.y, v * ph[f].y);
+            a.acc[o] = sm;
+            a.cmp[o] = cp;
+        }
+    }
+}
+
+}  // namespace sjf

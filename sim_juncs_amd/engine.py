@@ -1,0 +1,233 @@
+"""Python handle on the HIP engine (libsjf.so) through its C ABI.
+
+`Engine` mirrors the meep calls the reference driver makes (src/disp.cpp:482-749): build a
+grid and structure, add sources and monitors, `step()`, read monitor series and fields.
+Every method calls straight into libsjf; nothing here computes physics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .materials import MaterialSpec
+
+COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def comp_index(c) -> int:
+    if isinstance(c, str):
+        return COMPONENTS.index(c)
+    return int(c)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    arr = (C.c_int * len(a))(*[int(x) for x in a])
+    return arr
+
+
+class Engine:
+    """One simulation on one GPU.  precision: "fp32" or "fp64"."""
+
+    def __init__(self, shape, dx: float, courant: float = 0.5, precision: str = "fp32",
+                 periodic=(0, 0, 0), part: int = 0, device: int = 0, lib_path: str | None = None):
+        self.lib = _lib.load(lib_path)
+        self.shape = tuple(int(s) for s in shape)
+        self.dx = float(dx)
+        self.precision = precision
+        g = _lib.Grid()
+        g.nx, g.ny, g.nz = self.shape
+        g.dx = self.dx
+        g.courant = float(courant)
+        g.precision = 4 if precision == "fp32" else 8
+        for d in range(3):
+            g.periodic[d] = int(periodic[d])
+        g.part = int(part)
+        g.device = int(device)
+        h = C.c_void_p()
+        rc = self.lib.sjf_create(C.byref(g), C.byref(h))
+        if rc != 0:
+            raise _lib.EngineError(
+                f"sjf_create failed ({_lib.STATUS.get(rc, rc)}): "
+                f"{self.lib.sjf_last_error(None).decode()}")
+        self.h = h
+
+    # ------------------------------------------------------------------ plumbing ----------
+    def _chk(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.sjf_last_error(self.h).decode()
+            raise _lib.EngineError(f"{what} failed ({_lib.STATUS.get(rc, rc)}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sjf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n(self) -> int:
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    # ------------------------------------------------------------------ structure ---------
+    def set_materials(self, matw: np.ndarray, lut: list[MaterialSpec]):
+        matw = np.ascontiguousarray(matw, dtype=np.uint32).reshape(-1)
+        if matw.size != self.n:
+            raise ValueError("material word array must have nx*ny*nz entries")
+        arr = (_lib.Material * len(lut))()
+        for m, spec in enumerate(lut):
+            arr[m].eps_inf = spec.eps_inf
+            arr[m].npoles = len(spec.poles)
+            for s, p in enumerate(spec.poles):
+                arr[m].f0[s] = p.f0
+                arr[m].gamma[s] = p.gamma
+                arr[m].sigma[s] = p.sigma
+                arr[m].is_drude[s] = 1 if p.drude else 0
+        rc = self.lib.sjf_set_materials(self.h, matw.ctypes.data_as(C.POINTER(C.c_uint32)), arr,
+                                        len(lut))
+        self._chk(rc, "sjf_set_materials")
+
+    def set_design_eps(self, lo, hi, eps: np.ndarray):
+        eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1)
+        self._chk(self.lib.sjf_set_design_eps(self.h, _ip(lo), _ip(hi), _dp(eps)),
+                  "sjf_set_design_eps")
+
+    def set_pml(self, cells, R: float = 1e-15, power: float = 2.0):
+        self._chk(self.lib.sjf_set_pml(self.h, _ip(cells), float(R), float(power)), "sjf_set_pml")
+
+    # ------------------------------------------------------------------ sources -----------
+    def add_source(self, comp, box, kind: str = "gaussian", freq: float = 1.0,
+                   width: float = 1.0, phase: float = 0.0, t_start: float = 0.0,
+                   t_end: float = 0.0, slowness: float = 3.0, amp: complex = 1.0):
+        p = _lib.Pulse()
+        p.kind = 0 if kind == "gaussian" else 1
+        p.freq, p.width, p.phase = float(freq), float(width), float(phase)
+        p.t_start, p.t_end, p.slowness = float(t_start), float(t_end), float(slowness)
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        amp = complex(amp)
+        self._chk(self.lib.sjf_add_source(self.h, comp_index(comp), _dp(b), C.byref(p), amp.real,
+                                          amp.imag), "sjf_add_source")
+
+    def last_source_time(self) -> float:
+        t = C.c_double()
+        self._chk(self.lib.sjf_last_source_time(self.h, C.byref(t)), "sjf_last_source_time")
+        return t.value
+
+    @property
+    def dt(self) -> float:
+        t = C.c_double()
+        self._chk(self.lib.sjf_dt(self.h, C.byref(t)), "sjf_dt")
+        return t.value
+
+    def time(self) -> float:
+        t = C.c_double()
+        self._chk(self.lib.sjf_time(self.h, C.byref(t)), "sjf_time")
+        return t.value
+
+    def step_count(self) -> int:
+        n = C.c_long()
+        self._chk(self.lib.sjf_step_count(self.h, C.byref(n)), "sjf_step_count")
+        return n.value
+
+    # ------------------------------------------------------------------ monitors ----------
+    def add_monitors(self, comp, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._chk(self.lib.sjf_add_monitors(self.h, comp_index(comp), _dp(xyz), xyz.shape[0]),
+                  "sjf_add_monitors")
+        self._nmon = getattr(self, "_nmon", 0) + xyz.shape[0]
+
+    def set_save_span(self, span: int):
+        self._chk(self.lib.sjf_set_save_span(self.h, int(span)), "sjf_set_save_span")
+
+    def read_monitors(self) -> np.ndarray:
+        """complex array [n_samples, n_points]"""
+        ns = C.c_long()
+        self._chk(self.lib.sjf_read_monitors(self.h, None, 0, C.byref(ns)), "sjf_read_monitors")
+        npt = getattr(self, "_nmon", 0)
+        out = np.zeros((ns.value, npt, 2))
+        if ns.value and npt:
+            self._chk(self.lib.sjf_read_monitors(self.h, _dp(out), out.size, C.byref(ns)),
+                      "sjf_read_monitors")
+        return out[..., 0] + 1j * out[..., 1]
+
+    def get_field(self, comp, xyz) -> complex:
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        out = np.zeros(2)
+        self._chk(self.lib.sjf_get_field(self.h, comp_index(comp), _dp(x), _dp(out)),
+                  "sjf_get_field")
+        return complex(out[0], out[1])
+
+    # ------------------------------------------------------------------ DFT ---------------
+    def add_dft(self, lo, hi, comps, freqs) -> int:
+        mask = 0
+        for c in comps:
+            mask |= 1 << comp_index(c)
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        i = C.c_int()
+        self._chk(self.lib.sjf_add_dft(self.h, _ip(lo), _ip(hi), mask, _dp(f), f.size, C.byref(i)),
+                  "sjf_add_dft")
+        self._dft = getattr(self, "_dft", {})
+        ncomp = bin(mask).count("1")
+        ext = tuple(int(h) - int(l) for l, h in zip(lo, hi))
+        self._dft[i.value] = (ncomp, f.size, ext)
+        return i.value
+
+    def read_dft(self, i: int) -> np.ndarray:
+        """complex array [n_comp (ascending component order), n_freq, *box]"""
+        ncomp, nf, ext = self._dft[i]
+        out = np.zeros((ncomp, nf) + ext + (2,))
+        self._chk(self.lib.sjf_read_dft(self.h, int(i), _dp(out), out.size), "sjf_read_dft")
+        return out[..., 0] + 1j * out[..., 1]
+
+    # ------------------------------------------------------------------ stepping ----------
+    def step(self, n: int = 1):
+        self._chk(self.lib.sjf_step(self.h, int(n)), "sjf_step")
+
+    def synchronize(self):
+        self._chk(self.lib.sjf_synchronize(self.h), "sjf_synchronize")
+
+    def timed_steps(self, n: int):
+        """(ms_total, ms_kernel_mean) from HIP events on the engine's stream."""
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.sjf_timed_steps(self.h, int(n), C.byref(a), C.byref(b)),
+                  "sjf_timed_steps")
+        return a.value, b.value
+
+    # ------------------------------------------------------------------ fields ------------
+    def read_field(self, comp) -> np.ndarray:
+        out = np.zeros(self.shape)
+        self._chk(self.lib.sjf_read_field(self.h, comp_index(comp), _dp(out)), "sjf_read_field")
+        return out
+
+    def write_field(self, comp, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)
+        self._chk(self.lib.sjf_write_field(self.h, comp_index(comp), _dp(v)), "sjf_write_field")
+
+    # ------------------------------------------------------------------ accounting --------
+    def step_bytes(self) -> float:
+        b = C.c_double()
+        self._chk(self.lib.sjf_step_bytes(self.h, C.byref(b)), "sjf_step_bytes")
+        return b.value
+
+    def launch_info(self) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        self._chk(self.lib.sjf_launch_info(self.h, *[C.byref(x) for x in v]), "sjf_launch_info")
+        return dict(ty=v[0].value, xchunk=v[1].value, threads=v[2].value, groups=v[3].value)
+
+    def set_tile(self, ty: int, xchunk: int):
+        self._chk(self.lib.sjf_set_tile(self.h, int(ty), int(xchunk)), "sjf_set_tile")

@@ -1,0 +1,210 @@
+"""Workload descriptions: one `Scenario` drives the HIP engine or the CPU oracle alike.
+
+A scenario is what bound_geom assembles before its time loop (reference src/disp.cpp:482-645):
+grid, per-Yee-point material map, PML, sources, monitors (plus the DFT regions this engine
+adds).  Builders below produce the sim_juncs junction workloads used for parity and
+measurement, with seeded synthetic geometry.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import materials as M
+
+LIGHT_SPEED = 0.299792458  # um per fs; t_meep = t_fs * LIGHT_SPEED * um_scale (src/disp.hpp:157)
+FWHM_TO_WIDTH = 1.0 / (2.0 * math.sqrt(math.log(2.0)))  # intensity FWHM -> field sigma
+
+# Yee offsets (in cells) of Ex, Ey, Ez, Hx, Hy, Hz
+COMP_OFF = np.array([[0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])
+
+
+@dataclass
+class Scenario:
+    name: str
+    shape: tuple
+    dx: float
+    courant: float = 0.5
+    precision: str = "fp32"
+    periodic: tuple = (0, 0, 0)
+    part: int = 0
+    matw: np.ndarray | None = None
+    lut: list = field(default_factory=lambda: [M.VACUUM])
+    design: tuple | None = None          # (lo, hi, eps array)
+    pml: tuple | None = None             # (cells[6], R, power)
+    sources: list = field(default_factory=list)
+    monitors: list = field(default_factory=list)  # (comp, xyz[n,3])
+    save_span: int = 1
+    dfts: list = field(default_factory=list)      # (lo, hi, comps, freqs)
+    steps: int = 100
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    def make(self, cls, **kw):
+        """Instantiate `cls` (Engine or Oracle) and apply the whole scenario to it."""
+        sim = cls(self.shape, self.dx, courant=self.courant, precision=kw.pop("precision", self.precision),
+                  periodic=self.periodic, part=self.part, **kw)
+        self.apply(sim)
+        return sim
+
+    def apply(self, sim):
+        if self.matw is not None:
+            sim.set_materials(self.matw, self.lut)
+        if self.design is not None:
+            sim.set_design_eps(*self.design)
+        if self.pml is not None:
+            sim.set_pml(*self.pml)
+        for s in self.sources:
+            sim.add_source(**s)
+        for comp, xyz in self.monitors:
+            sim.add_monitors(comp, xyz)
+        sim.set_save_span(self.save_span)
+        ids = [sim.add_dft(*d) for d in self.dfts]
+        return ids
+
+
+# ----------------------------------------------------------------------- voxelization ----
+
+def yee_points(shape, comp):
+    """Cell-unit coordinates of component `comp`'s Yee points as broadcastable axes
+    (nx,1,1), (1,ny,1), (1,1,nz)."""
+    nx, ny, nz = shape
+    off = COMP_OFF[comp]
+    return ((np.arange(nx) + off[0])[:, None, None], (np.arange(ny) + off[1])[None, :, None],
+            (np.arange(nz) + off[2])[None, None, :])
+
+
+def paint(mat: np.ndarray, shape, inside, mid: int):
+    """Set material id `mid` on every E Yee point for which inside(x, y, z) (cell units) holds;
+    mat has shape (nx, ny, nz, 3)."""
+    for c in range(3):
+        x, y, z = yee_points(shape, c)
+        m = np.broadcast_to(inside(x, y, z), shape)
+        mat[..., c][m] = mid
+
+
+def box_inside(lo, hi):
+    """Inclusive box test, as box::in (reference src/cgs.cpp:52-67)."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    return lambda x, y, z: ((x >= lo[0]) & (x <= hi[0]) & (y >= lo[1]) & (y <= hi[1]) &
+                            (z >= lo[2]) & (z <= hi[2]))
+
+
+def triangle_prism_inside(tip, direction, side, z0, z1):
+    """Equilateral triangle in x-y with its tip at `tip`, opening along `direction` (+1/-1 in
+    x), extruded over z in [z0, z1]."""
+    tx, ty = tip
+    h = side * math.sqrt(3) / 2
+
+    def f(x, y, z):
+        u = (x - tx) * direction  # distance from the tip along the axis
+        return (u >= 0) & (u <= h) & (np.abs(y - ty) <= u / math.sqrt(3)) & (z >= z0) & (z <= z1)
+
+    return f
+
+
+def pack(mat: np.ndarray) -> np.ndarray:
+    m = mat.astype(np.uint32)
+    return m[..., 0] | (m[..., 1] << 8) | (m[..., 2] << 16)
+
+
+# ----------------------------------------------------------------------- sources ---------
+
+def gaussian_pulse(wavelength_um, fwhm_fs, cep, um_scale=1.0, start_fs=5.0):
+    """Gaussian_source semantics (reference src/cgs_data.cpp:22-53, src/disp.cpp:362-374,
+    604-613): width = field sigma in fs, start_time defaults to 5 fs, cutoff forced to 6 so
+    end = start + 12 width; converted to meep units with c_a = 0.299792458 um_scale."""
+    ca = LIGHT_SPEED * um_scale
+    w_fs = fwhm_fs * FWHM_TO_WIDTH
+    end_fs = start_fs + 2 * 6 * w_fs
+    return dict(kind="gaussian", freq=1.0 / (wavelength_um * um_scale), width=w_fs * ca,
+                phase=float(cep), t_start=start_fs * ca, t_end=end_fs * ca)
+
+
+def plane_source(shape, dx, z_cell, pulse, comp="Ex", amp=1.0):
+    nx, ny, _ = shape
+    box = [0.0, 0.0, z_cell * dx, nx * dx, ny * dx, z_cell * dx]
+    return dict(comp=comp, box=box, amp=amp, **pulse)
+
+
+# ----------------------------------------------------------------------- workloads -------
+
+def plain_box(n=96, steps=2000, precision="fp64", seed=1, pml=8):
+    """Plain 3-D run: n^3 cells (96^3 full size), `pml`-cell PML on all faces, Courant 0.5,
+    Au box (Drude + 2 Lorentz) 24x24x8 cells on a non-dispersive SiO2 slab (eps 2.13) spanning
+    x-y and 20 cells of z, x-polarised plane-wave Gaussian pulse at 800 nm, FWHM 10 fs, CEP 0,
+    one DFT plane at 5 frequencies.  Box origin and edges jitter by +-1 cell (seeded).
+    Sizes scale with n/96 for reduced parity cases."""
+    rng = np.random.default_rng(seed)
+    s = n / 96.0
+    shape = (n, n, n)
+    dx = 0.02  # um_scale = 1: 20 nm cells, lambda0 / dx = 40
+    mat = np.zeros(shape + (3,), np.uint8)
+    slab_lo, slab_hi = pml, pml + max(2, round(20 * s))
+    paint(mat, shape, box_inside((-1, -1, slab_lo), (n + 1, n + 1, slab_hi - 0.5)), 1)
+    bw, bt = max(2, round(24 * s)), max(2, round(8 * s))
+    j = rng.integers(-1, 2, size=6)
+    cx = n // 2
+    x0 = cx - bw // 2 + j[0]
+    x1 = cx + bw // 2 + j[1]
+    y0 = cx - bw // 2 + j[2]
+    y1 = cx + bw // 2 + j[3]
+    z0 = slab_hi
+    z1 = slab_hi + bt + j[4]
+    paint(mat, shape, box_inside((x0, y0, z0 - 0.5), (x1, y1, z1 - 0.5)), 2)
+    lut = [M.VACUUM, M.SIO2_CONST, M.AU.scaled(1.0)]
+    src_z = min(z1 + max(2, round(10 * s)), n - pml - 2)
+    pulse = gaussian_pulse(0.8, 10.0, 0.0)
+    f0 = pulse["freq"]
+    dft_z = min(z1 + 2, n - pml - 1)
+    freqs = [f0 * r for r in (0.8, 0.9, 1.0, 1.1, 1.2)]
+    return Scenario(
+        name=f"plain_box_{n}", shape=shape, dx=dx, precision=precision, matw=pack(mat), lut=lut,
+        pml=([pml] * 6, 1e-15, 2.0), sources=[plane_source(shape, dx, src_z, pulse)],
+        dfts=[((0, 0, dft_z), (n, n, dft_z + 1), ("Ex", "Ey", "Hx", "Hy"), freqs)],
+        monitors=[("Ex", np.array([[(cx + 0.5) * dx, cx * dx, (z1 + 1) * dx]]))], save_span=20,
+        steps=steps, meta=dict(src_z=src_z, dft_z=dft_z, au=(x0, x1, y0, y1, z0, z1)))
+
+
+def bowtie_junction(nx=512, ny=512, nz=256, steps=6000, precision="fp32", pml=12):
+    """Single large instance: nx x ny x nz cells (512x512x256 full size) of 5 nm, `pml`-cell
+    PML, SiO2 substrate (1-pole Lorentz) over z in [pml, pml+64), Au bowtie (Drude + 2
+    Lorentz: two equilateral triangles, 160-cell side, 16 cells thick, tips 8 cells apart) on
+    the substrate, a 1-cell graphene Drude sheet under the gap, Gaussian pulse 800 nm FWHM 6
+    fs, four point DFT monitors x 8 frequencies.  Lateral sizes scale with nx/512."""
+    s = nx / 512.0
+    shape = (nx, ny, nz)
+    dx = 0.005
+    mat = np.zeros(shape + (3,), np.uint8)
+    sub_hi = pml + max(4, round(64 * s))
+    paint(mat, shape, box_inside((-1, -1, pml - 0.5), (nx + 1, ny + 1, sub_hi - 0.5)), 1)
+    side = max(8, round(160 * s))
+    gap = max(2, round(8 * s))
+    thick = max(2, round(16 * s))
+    cx, cy = nx / 2, ny / 2
+    zt0, zt1 = sub_hi - 0.5, sub_hi + thick - 0.5
+    gr_half = max(2, round(40 * s))
+    paint(mat, shape, box_inside((cx - gr_half, cy - gr_half, sub_hi - 0.5), (cx + gr_half, cy + gr_half, sub_hi + 0.5)), 3)
+    paint(mat, shape, triangle_prism_inside((cx + gap / 2, cy), +1, side, zt0, zt1), 2)
+    paint(mat, shape, triangle_prism_inside((cx - gap / 2, cy), -1, side, zt0, zt1), 2)
+    lut = [M.VACUUM, M.SIO2.scaled(1.0), M.AU.scaled(1.0), M.graphene_sheet(dx).scaled(1.0)]
+    src_z = min(sub_hi + thick + max(4, round(40 * s)), nz - pml - 2)
+    pulse = gaussian_pulse(0.8, 6.0, 0.0)
+    f0 = pulse["freq"]
+    freqs = list(np.linspace(0.8, 1.2, 8) * f0)
+    gz = sub_hi + thick // 2
+    d20 = max(1, round(20 * s))
+    pts = [(int(cx), int(cy), gz), (int(cx) + d20, int(cy), gz), (int(cx) - d20, int(cy), gz),
+           (int(cx), int(cy), min(gz + max(1, round(10 * s)), nz - pml - 1))]
+    dfts = [((p[0], p[1], p[2]), (p[0] + 1, p[1] + 1, p[2] + 1), ("Ex", "Ey", "Ez"), freqs) for p in pts]
+    return Scenario(
+        name=f"bowtie_{nx}x{ny}x{nz}", shape=shape, dx=dx, precision=precision, matw=pack(mat),
+        lut=lut, pml=([pml] * 6, 1e-15, 2.0), sources=[plane_source(shape, dx, src_z, pulse)],
+        dfts=dfts, monitors=[("Ex", np.array([[(cx + 0.5) * dx, cy * dx, gz * dx]]))],
+        save_span=20, steps=steps, meta=dict(src_z=src_z, points=pts))

@@ -1,0 +1,183 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 CPU oracle.
+
+FP64 engine runs must match the oracle to rounding (the engine uses the E-direct (P, Q) form
+of the oracle's D/E split, so agreement is to ~1e-12 relative, not bitwise).  FP32 runs must
+stay within 1e-4 relative L2 (the workload tolerance) on the same inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from sim_juncs_amd import materials as M
+from sim_juncs_amd import scenario
+from sim_juncs_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-4
+
+
+def early_pulse(scn, fwhm_fs=2.0, cep=0.3):
+    """Reduced cases run a few hundred steps: use a short pulse that starts at t = 0."""
+    for src in scn.sources:
+        src.update(scenario.gaussian_pulse(0.8, fwhm_fs, cep, start_fs=0.0))
+    return scn
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def run_pair(scn, precision=None, tile=None, steps=None):
+    print(f"[case] {scn.name} shape={scn.shape} prec={precision or scn.precision} tile={tile}", flush=True)
+    eng = scn.make(Engine, precision=precision or scn.precision)
+    if tile:
+        eng.set_tile(*tile)
+    orc = scn.make(Oracle)
+    n = steps if steps is not None else scn.steps
+    eng.step(n)
+    orc.step(n)
+    return eng, orc
+
+
+def field_errors(eng, orc):
+    return [rel(eng.read_field(c), orc.read_field(c)) for c in range(6)]
+
+
+def random_scenario(shape, seed, pml=None, periodic=(0, 0, 0), poles=True, nmat=4, steps=40):
+    """Random materials (dispersive mix), random initial fields, optional PML and source."""
+    rng = np.random.default_rng(seed)
+    mats = rng.integers(0, nmat, size=tuple(shape) + (3,))
+    lut = [M.VACUUM, M.MaterialSpec(2.13), M.AU.scaled(1.0) if poles else M.MaterialSpec(3.0),
+           M.SIO2.scaled(1.0) if poles else M.MaterialSpec(1.7)][:nmat]
+    dx = 0.02
+    scn = scenario.Scenario(name=f"random{shape}", shape=tuple(shape), dx=dx, precision="fp64",
+                            periodic=periodic, matw=scenario.pack(mats), lut=lut, steps=steps)
+    if pml is not None:
+        scn.pml = (pml, 1e-15, 2.0)
+    zc = shape[2] // 2
+    scn.sources = [scenario.plane_source(shape, dx, zc, scenario.gaussian_pulse(0.8, 2.0, 0.3, start_fs=0.0))]
+    return scn, rng
+
+
+@pytest.mark.parametrize("n", [24, 33])
+def test_plain_box_fp64_matches_oracle(n):
+    scn = early_pulse(scenario.plain_box(n=n, steps=120, precision="fp64", pml=4))
+    eng, orc = run_pair(scn)
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP64_TOL, errs
+    assert np.abs(eng.read_field("Ex")).max() > 0
+    m_e, m_o = eng.read_monitors(), orc.read_monitors()
+    assert m_e.shape == m_o.shape == (6, 1)
+    assert rel(m_e, m_o) < FP64_TOL
+    d_e, d_o = eng.read_dft(0), orc.read_dft(0)
+    assert rel(d_e, d_o) < FP64_TOL
+    eng.close()
+
+
+def test_plain_box_fp32_within_tolerance():
+    scn = early_pulse(scenario.plain_box(n=32, steps=300, precision="fp32", pml=6))
+    eng, orc = run_pair(scn)
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP32_TOL, errs
+    assert np.abs(eng.read_field("Ex")).max() > 0
+    assert rel(eng.read_dft(0), orc.read_dft(0)) < FP32_TOL
+    eng.close()
+
+
+@pytest.mark.parametrize("shape,tile", [
+    ((5, 7, 9), (1, 1)), ((5, 7, 9), (4, 2)), ((9, 10, 70), (2, 3)), ((9, 10, 70), (8, 64)),
+    ((12, 3, 65), (4, 5)), ((1, 1, 130), (1, 1)), ((6, 13, 128), (8, 4)),
+])
+def test_random_media_tiles_fp64(shape, tile):
+    scn, rng = random_scenario(shape, seed=sum(shape), pml=[1, 2, 1, 0, 2, 3] if min(shape) > 5 else None)
+    eng = scn.make(Engine)
+    eng.set_tile(*tile)
+    orc = scn.make(Oracle)
+    for c in range(6):
+        f = rng.standard_normal(shape)
+        eng.write_field(c, f)
+        orc.write_field(c, f)
+    eng.step(scn.steps)
+    orc.step(scn.steps)
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP64_TOL, errs
+    eng.close()
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)])
+def test_periodic_axes_fp64(periodic):
+    shape = (8, 6, 66)
+    pml = [0 if periodic[d] else 2 for d in range(3) for _ in range(2)]
+    scn, rng = random_scenario(shape, seed=7, pml=pml, periodic=periodic)
+    eng = scn.make(Engine)
+    eng.set_tile(2, 3)
+    orc = scn.make(Oracle)
+    for c in range(6):
+        f = rng.standard_normal(shape)
+        eng.write_field(c, f)
+        orc.write_field(c, f)
+    eng.step(30)
+    orc.step(30)
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP64_TOL, errs
+    eng.close()
+
+
+def test_bowtie_small_fp32_and_fp64():
+    scn = early_pulse(scenario.bowtie_junction(nx=64, ny=64, nz=64, steps=200, pml=6), fwhm_fs=1.5)
+    for prec, tol in (("fp64", FP64_TOL), ("fp32", FP32_TOL)):
+        eng, orc = run_pair(scn, precision=prec)
+        errs = field_errors(eng, orc)
+        assert max(errs) < tol, (prec, errs)
+        assert np.abs(eng.read_field("Ex")).max() > 0
+        for i in range(len(scn.dfts)):
+            assert rel(eng.read_dft(i), orc.read_dft(i)) < tol
+        eng.close()
+
+
+def test_energy_invariant_closed_cavity_fp64():
+    """Lossless, non-dispersive closed cavity: W = sum eps E^n.E^n + H^{n+1/2}.H^{n-1/2} is
+    exactly conserved by the leapfrog Yee scheme."""
+    rng = np.random.default_rng(11)
+    shape = (10, 9, 70)
+    mats = rng.integers(0, 3, size=shape + (3,))
+    lut = [M.VACUUM, M.MaterialSpec(2.5), M.MaterialSpec(4.0)]
+    eps = np.array([1.0, 2.5, 4.0])[mats]
+    eng = Engine(shape, 0.05, precision="fp64")
+    eng.set_materials(scenario.pack(mats), lut)
+    for c in range(6):
+        eng.write_field(c, rng.standard_normal(shape))
+    W = []
+    for _ in range(50):
+        E = [eng.read_field(c) for c in range(3)]
+        H0 = [eng.read_field(c) for c in range(3, 6)]
+        eng.step(1)
+        H1 = [eng.read_field(c) for c in range(3, 6)]
+        W.append(sum((eps[..., c] * E[c] ** 2).sum() for c in range(3)) +
+                 sum((H0[c] * H1[c]).sum() for c in range(3)))
+    W = np.array(W)
+    assert np.abs(W - W[0]).max() / W[0] < 1e-12
+    eng.close()
+
+
+def test_step_one_at_a_time_equals_batch():
+    """sjf_step(1) n times is the drop-in for n calls of fields.step(): same state as one
+    sjf_step(n), including monitor samples taken before each step with save_span."""
+    scn = early_pulse(scenario.plain_box(n=20, steps=45, precision="fp32", pml=3))
+    scn.save_span = 7
+    a = scn.make(Engine)
+    b = scn.make(Engine)
+    a.step(45)
+    for _ in range(45):
+        b.step(1)
+    assert np.abs(a.read_field("Ex")).max() > 0
+    for c in range(6):
+        assert np.array_equal(a.read_field(c), b.read_field(c))
+    ma, mb = a.read_monitors(), b.read_monitors()
+    assert ma.shape[0] == 7  # samples at steps 0,7,...,42
+    assert np.array_equal(ma, mb)
+    a.close()
+    b.close()
