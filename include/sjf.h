@@ -188,6 +188,11 @@ int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int
 /* Override the autotuned tile shape (ty in {1,2,4,8}, xchunk >= 1). */
 int sjf_set_tile(sjf_engine* e, int ty, int xchunk);
 
+/* Achievable HBM bandwidth: times `iters` launches of a plain streaming copy of `bytes` bytes
+ * on `device` and reports (read + write bytes) / time in GB/s.  New; used as the measured
+ * bandwidth ceiling next to the nominal peak and to calibrate traffic counters. */
+int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ EXPORTS = [
     "sjf_step_count", "sjf_add_monitors", "sjf_set_save_span", "sjf_read_monitors",
     "sjf_get_field", "sjf_add_dft", "sjf_read_dft", "sjf_step", "sjf_synchronize",
     "sjf_timed_steps", "sjf_read_field", "sjf_write_field", "sjf_step_bytes",
-    "sjf_launch_info", "sjf_set_tile",
+    "sjf_launch_info", "sjf_set_tile", "sjf_copy_probe",
 ]
 
 
@@ -108,6 +108,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_step_bytes": (C.c_int, [vp, dp]),
         "sjf_launch_info": (C.c_int, [vp, ip, ip, ip, ip]),
         "sjf_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
+        "sjf_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
     }
     for name, (res, args) in proto.items():
         f = getattr(lib, name)

@@ -1,0 +1,103 @@
+"""Tile-shape autotuner for the fused step kernel.
+
+The fused kernel's launch shape has two free parameters: rows per thread (register blocking
+in y, 1/2/4/8) and the x-chunk each workgroup marches over.  `tune()` times candidates on a
+live engine with HIP events (the simulation keeps advancing; timing does not depend on the
+field values) and returns the fastest.  Results are cached per (shape, precision, material
+mix) in a JSON file next to this module so later runs reuse them.
+
+    python -m sim_juncs_amd.autotune --workload bowtie
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned_tiles.json")
+
+ROWS = (1, 2, 4)
+XCHUNKS = (2, 4, 8, 16, 32, 64)
+
+
+def key(shape, precision, tag="") -> str:
+    return f"{shape[0]}x{shape[1]}x{shape[2]}:{precision}:{tag}"
+
+
+def load_cache() -> dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(shape, precision, tag=""):
+    return load_cache().get(key(shape, precision, tag))
+
+
+def apply_cached(eng, tag="") -> bool:
+    t = lookup(eng.shape, eng.precision, tag)
+    if t:
+        eng.set_tile(t["ty"], t["xchunk"])
+        return True
+    return False
+
+
+def tune(eng, steps: int = 20, rows=ROWS, xchunks=XCHUNKS, verbose: bool = True):
+    """Time every candidate tile on `eng`; returns (best dict, all results)."""
+    results = []
+    eng.step(2)
+    for ty in rows:
+        if ty > eng.shape[1] and ty != 1:
+            continue
+        for xc in xchunks:
+            if xc > eng.shape[0] and xc != xchunks[0]:
+                continue
+            try:
+                eng.set_tile(ty, xc)
+                eng.timed_steps(2)
+                _, ms = eng.timed_steps(steps)
+            except Exception as exc:  # tile rejected (e.g. LDS budget)
+                if verbose:
+                    print(f"[tune] ty={ty} xc={xc}: {exc}", file=sys.stderr, flush=True)
+                continue
+            results.append(dict(ty=ty, xchunk=xc, ms=ms))
+            if verbose:
+                print(f"[tune] ty={ty} xc={xc}: {ms:.4f} ms", file=sys.stderr, flush=True)
+    best = min(results, key=lambda r: r["ms"])
+    eng.set_tile(best["ty"], best["xchunk"])
+    return best, results
+
+
+def store(shape, precision, best, tag=""):
+    c = load_cache()
+    c[key(shape, precision, tag)] = dict(ty=best["ty"], xchunk=best["xchunk"], ms=best["ms"])
+    with open(CACHE, "w") as f:
+        json.dump(c, f, indent=1, sort_keys=True)
+
+
+def main():
+    sys.path.insert(0, os.path.dirname(HERE))
+    from sim_juncs_amd import scenario
+    from sim_juncs_amd.engine import Engine
+
+    p = argparse.ArgumentParser()
+    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain"])
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--store", action="store_true")
+    a = p.parse_args()
+    scn = scenario.bowtie_junction() if a.workload == "bowtie" else scenario.plain_box()
+    eng = scn.make(Engine)
+    best, res = tune(eng, steps=a.steps)
+    bytes_step = eng.step_bytes()
+    for r in sorted(res, key=lambda r: r["ms"])[:8]:
+        print(json.dumps(dict(r, gbs=bytes_step / (r["ms"] * 1e-3) / 1e9)))
+    if a.store:
+        store(eng.shape, eng.precision, best)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

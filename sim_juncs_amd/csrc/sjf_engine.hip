@@ -386,17 +386,35 @@ size_t step_lds(const sjf_engine* e) {
     return sizeof(T) * 4 * e->ty * block_threads(e) + sizeof(sjf::MatLut<T>) * e->nmat;
 }
 
-template <typename T, int MAXT>
-int launch_step_b(sjf_engine* e, const sjf::StepArgs<T>& a, dim3 grid, dim3 block, size_t lds) {
+template <typename T, int NP, int MAXT>
+int launch_step_np(sjf_engine* e, const sjf::StepArgs<T>& a, dim3 grid, dim3 block, size_t lds) {
     switch (e->ty) {
-        case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, MAXT>), grid, block, lds, e->stream, a); break;
-        case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, MAXT>), grid, block, lds, e->stream, a); break;
-        case 4: hipLaunchKernelGGL((sjf::k_step<T, 4, MAXT>), grid, block, lds, e->stream, a); break;
-        case 8: hipLaunchKernelGGL((sjf::k_step<T, 8, MAXT>), grid, block, lds, e->stream, a); break;
-        default: return fail(e, SJF_E_ARG, "tile rows must be 1, 2, 4 or 8");
+        case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, lds, e->stream, a); break;
+        case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, NP, MAXT>), grid, block, lds, e->stream, a); break;
+        case 4: hipLaunchKernelGGL((sjf::k_step<T, 4, NP, MAXT>), grid, block, lds, e->stream, a); break;
+        default: return fail(e, SJF_E_ARG, "tile rows must be 1, 2 or 4");
     }
     HIPCHK(e, hipGetLastError());
     return SJF_OK;
+}
+
+// Pole slots compiled into the kernel: 0, 1, 3 or 4 (a material with 2 poles runs the
+// 3-slot variant; unused slots are predicated off).
+int np_variant(const sjf_engine* e) {
+    if (e->uniform || e->maxp == 0) return 0;
+    if (e->maxp == 1) return 1;
+    if (e->maxp <= 3) return 3;
+    return 4;
+}
+
+template <typename T, int MAXT>
+int launch_step_b(sjf_engine* e, const sjf::StepArgs<T>& a, dim3 grid, dim3 block, size_t lds) {
+    switch (np_variant(e)) {
+        case 0: return launch_step_np<T, 0, MAXT>(e, a, grid, block, lds);
+        case 1: return launch_step_np<T, 1, MAXT>(e, a, grid, block, lds);
+        case 3: return launch_step_np<T, 3, MAXT>(e, a, grid, block, lds);
+        default: return launch_step_np<T, 4, MAXT>(e, a, grid, block, lds);
+    }
 }
 
 template <typename T>
@@ -526,15 +544,15 @@ int run_steps(sjf_engine* e, long n, std::vector<hipEvent_t>* evs) {
 
 void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
-    e->ty = e->prec == 4 ? 4 : 2;
-    if (e->g.ny < e->ty) e->ty = e->g.ny >= 2 ? 2 : 1;
-    const int waves_per_group = block_threads(e) / 64;
+    // Defaults measured on MI355X: one or two rows per thread and short x-chunks (many
+    // workgroups) beat deep register blocking; the autotuner refines per workload.
+    e->ty = (e->prec == 4 && e->g.ny >= 2) ? 2 : 1;
     const int nty = (e->g.ny + e->ty - 1) / e->ty;
-    const long target_groups = std::max(1L, 8192L / waves_per_group);
+    const long target_groups = 16384;
     long nxc = std::max(1L, (target_groups + nty - 1) / nty);
     nxc = std::min<long>(nxc, e->g.nx);
     int xc = (int)((e->g.nx + nxc - 1) / nxc);
-    e->xc = std::max(1, std::min(xc, e->g.nx));
+    e->xc = std::max(std::min(4, e->g.nx), std::min(xc, e->g.nx));
 }
 
 int free_all(sjf_engine* e) {
@@ -1194,11 +1212,43 @@ int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int
 
 int sjf_set_tile(sjf_engine* e, int ty, int xchunk) {
     if (!e) return SJF_E_ARG;
-    if (ty != 1 && ty != 2 && ty != 4 && ty != 8) return fail(e, SJF_E_ARG, "ty must be 1, 2, 4 or 8");
+    if (ty != 1 && ty != 2 && ty != 4) return fail(e, SJF_E_ARG, "ty must be 1, 2 or 4");
     if (xchunk < 1) return fail(e, SJF_E_ARG, "xchunk must be >= 1");
     e->ty = ty;
     e->xc = std::min(xchunk, e->g.nx);
     e->tile_set = true;
+    return SJF_OK;
+}
+
+int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs) {
+    if (!gbs || bytes < 16 || iters < 1) return SJF_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SJF_E_HIP;
+    const long n = (long)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, n * 16) != hipSuccess) return SJF_E_NOMEM;
+    if (hipMalloc((void**)&b, n * 16) != hipSuccess) {
+        hipFree(a);
+        return SJF_E_NOMEM;
+    }
+    hipMemset(a, 0, n * 16);
+    hipMemset(b, 0, n * 16);
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    const int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    hipEventRecord(t0, 0);
+    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    hipEventRecord(t1, 0);
+    hipError_t st = hipEventSynchronize(t1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    hipFree(a);
+    hipFree(b);
+    if (st != hipSuccess) return SJF_E_HIP;
+    *gbs = 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
     return SJF_OK;
 }
 

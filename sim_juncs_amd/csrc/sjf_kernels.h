@@ -113,7 +113,20 @@ __device__ __forceinline__ T cpml(T d, T b, T c, T& psi) {
     return d + psi;
 }
 
-template <typename T, int TY, int MAXT>
+// LDS-scoped workgroup barrier: orders only the LDS exchange, so global loads issued before it
+// (next-plane prefetch) and global stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Fused H / polarisation / E update of one time step.  TY: y rows per thread; NP: pole slots
+// compiled in (>= the largest pole count of any material); MAXT: launch bound.
+// Each plane iteration issues every global load it needs first (fields, PML auxiliaries,
+// polarisation state, and the next plane's E and material words), then computes H, exchanges
+// the z neighbours of H through LDS, and computes E.
+template <typename T, int TY, int NP, int MAXT>
 __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int B = blockDim.x;
@@ -137,100 +150,201 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
     const long NZ = a.nz;
     const long NYZ = (long)a.ny * a.nz;
     const T cdt = a.cdt;
+    const bool uniform = a.uniform != 0;
 
     int gj[TY + 2];  // global row of local row r-1 (r = 0 is the halo row below the tile)
+    bool own_row[TY];
+    bool ypml[TY + 1];
 #pragma unroll
     for (int r = 0; r < TY + 2; ++r) gj[r] = wrap_or_ghost(j0 + r - 1, a.ny, a.py);
+#pragma unroll
+    for (int r = 0; r < TY; ++r) own_row[r] = kin && (j0 + r) < a.ny;
+#pragma unroll
+    for (int r = 0; r < TY + 1; ++r) ypml[r] = gj[r] >= 0 && in_pml(a, 1, gj[r], a.ny);
 
     const bool zpml = kin && in_pml(a, 2, k, a.nz);
     const T bEz = zpml ? a.bE[2][k] : T(0), aEz = zpml ? a.aE[2][k] : T(0);
     const T bHz = zpml ? a.bH[2][k] : T(0), aHz = zpml ? a.aH[2][k] : T(0);
 
-    __syncthreads();
+    lds_barrier();
 
     T ec[TY + 2][3];  // E^n on plane i, rows -1..TY
     T en[TY + 2][3];  // E^n on plane i+1
     T hp[TY][2];      // H^{n+1/2} (Hy, Hz) on plane i-1, rows 0..TY-1
+    uint32_t mw[TY];  // material words of plane i, owned rows
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
     {
-        const int pi = wrap_or_ghost(istart, a.nx, a.px);
+        const int p0 = wrap_or_ghost(istart, a.nx, a.px);
+        const int p1 = wrap_or_ghost(istart + 1, a.nx, a.px);
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            const bool ok = kin && gj[r] >= 0 && pi >= 0;
-            const long q = (long)pi * NYZ + (long)gj[r] * NZ + k;
-            ec[r][0] = ldg(a.E0[0], q, ok);
-            ec[r][1] = ldg(a.E0[1], q, ok);
-            ec[r][2] = ldg(a.E0[2], q, ok);
+            const bool ok0 = kin && gj[r] >= 0 && p0 >= 0;
+            const bool ok1 = kin && gj[r] >= 0 && p1 >= 0;
+            const long q0 = (long)p0 * NYZ + (long)gj[r] * NZ + k;
+            const long q1 = (long)p1 * NYZ + (long)gj[r] * NZ + k;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ec[r][c] = ldg(a.E0[c], q0, ok0);
+                en[r][c] = ldg(a.E0[c], q1, ok1);
+            }
         }
 #pragma unroll
-        for (int r = 0; r < TY; ++r) hp[r][0] = hp[r][1] = T(0);
+        for (int r = 0; r < TY; ++r) {
+            hp[r][0] = hp[r][1] = T(0);
+            const long q0 = (long)p0 * NYZ + (long)gj[r + 1] * NZ + k;
+            mw[r] = (!uniform && own_row[r] && istart >= x0) ? a.matw[q0] : 0u;
+        }
     }
 
     for (int i = istart; i < x1; ++i) {
         const bool own_plane = i >= x0;
+        const bool has_next = i + 1 < x1;
         const int pi = wrap_or_ghost(i, a.nx, a.px);
         const int pn = wrap_or_ghost(i + 1, a.nx, a.px);
+        const int pnn = wrap_or_ghost(i + 2, a.nx, a.px);
         const bool xpml = in_pml(a, 0, pi, a.nx);
         const T bEx = xpml ? a.bE[0][pi] : T(0), aEx = xpml ? a.aE[0][pi] : T(0);
         const T bHx = xpml ? a.bH[0][pi] : T(0), aHx = xpml ? a.aH[0][pi] : T(0);
 
+        // ---------------------------------------------------------------- loads ----------
+        T enn[TY + 2][3];
+        uint32_t mwn[TY];
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            const bool ok = kin && gj[r] >= 0 && pn >= 0;
-            const long q = (long)pn * NYZ + (long)gj[r] * NZ + k;
-            en[r][0] = ldg(a.E0[0], q, ok);
-            en[r][1] = (r < TY + 1) ? ldg(a.E0[1], q, ok) : T(0);
-            en[r][2] = ldg(a.E0[2], q, ok);
+            const bool ok = has_next && kin && gj[r] >= 0 && pnn >= 0;
+            const long q = (long)pnn * NYZ + (long)gj[r] * NZ + k;
+            enn[r][0] = ldg(a.E0[0], q, ok);
+            enn[r][1] = (r < TY + 1) ? ldg(a.E0[1], q, ok) : T(0);
+            enn[r][2] = ldg(a.E0[2], q, ok);
+        }
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const long q = (long)pn * NYZ + (long)gj[r + 1] * NZ + k;
+            mwn[r] = (!uniform && has_next && own_row[r] && pn >= 0) ? a.matw[q] : 0u;
+        }
+        T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
+#pragma unroll
+        for (int r = 0; r < TY + 1; ++r) {
+            const int j = gj[r];
+            const bool ok = kin && j >= 0;
+            const long q = (long)pi * NYZ + (long)j * NZ + k;
+            h0[r][0] = ldg(a.H0[0], q, ok);
+            h0[r][1] = ldg(a.H0[1], q, ok);
+            h0[r][2] = ldg(a.H0[2], q, ok);
+            const bool okk = ok && kp1 >= 0;
+            const long qk = q - k + kp1;
+            ek1[r][0] = ldg(a.E0[0], qk, okk);
+            ek1[r][1] = ldg(a.E0[1], qk, okk);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
+            if (ok && xpml) {
+                const long s = slab_x(a, pi, j, k);
+                pH[r][0][0] = a.psiH0[0][0][s];
+                pH[r][0][1] = a.psiH0[0][1][s];
+            }
+            if (ok && ypml[r]) {
+                const long s = slab_y(a, pi, j, k);
+                pH[r][1][0] = a.psiH0[1][0][s];
+                pH[r][1][1] = a.psiH0[1][1][s];
+            }
+            if (ok && zpml) {
+                const long s = slab_z(a, pi, j, k);
+                pH[r][2][0] = a.psiH0[2][0][s];
+                pH[r][2][1] = a.psiH0[2][1][s];
+            }
+        }
+        T pE[TY][3][2];
+        T Qv[TY][3][NP > 0 ? NP : 1], Pv[TY][3][NP > 0 ? NP : 1];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) pE[r][d][0] = pE[r][d][1] = T(0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[r][c][s] = Pv[r][c][s] = T(0);
+            if (!(own_plane && own_row[r])) continue;
+            const int j = gj[r + 1];
+            const long q = (long)pi * NYZ + (long)j * NZ + k;
+            if (xpml) {
+                const long s = slab_x(a, pi, j, k);
+                pE[r][0][0] = a.psiE[0][0][s];
+                pE[r][0][1] = a.psiE[0][1][s];
+            }
+            if (ypml[r + 1]) {
+                const long s = slab_y(a, pi, j, k);
+                pE[r][1][0] = a.psiE[1][0][s];
+                pE[r][1][1] = a.psiE[1][1][s];
+            }
+            if (zpml) {
+                const long s = slab_z(a, pi, j, k);
+                pE[r][2][0] = a.psiE[2][0][s];
+                pE[r][2][1] = a.psiE[2][1][s];
+            }
+            if (NP > 0 && !uniform) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int m = (mw[r] >> (8 * c)) & 0xff;
+                    const int np = m == kDesignMat ? 0 : slut[m].np;
+                    const int lm = m == kDesignMat ? 0 : slut[m].lorentz_mask;
+#pragma unroll
+                    for (int s = 0; s < NP; ++s) {
+                        if (s < np) {
+                            Qv[r][c][s] = a.Q[s][c][q];
+                            if ((lm >> s) & 1) Pv[r][c][s] = a.P[s][c][q];
+                        }
+                    }
+                }
+            }
         }
 
-        // ---- H^{n+1/2} on rows -1..TY-1 of plane i -------------------------------------
+        // ---------------------------------------------------------------- H^{n+1/2} -------
         T h[TY + 1][3];
 #pragma unroll
         for (int r = 0; r < TY + 1; ++r) {
             const int j = gj[r];
             const bool ok = kin && j >= 0;
             const long q = (long)pi * NYZ + (long)j * NZ + k;
-            const bool wr = own_plane && r >= 1 && (j0 + r - 1) < a.ny && ok;
-            const T hx0 = ldg(a.H0[0], q, ok);
-            const T hy0 = ldg(a.H0[1], q, ok);
-            const T hz0 = ldg(a.H0[2], q, ok);
-            const bool okk = ok && kp1 >= 0;
-            const long qk = q - k + kp1;
-            const T exk1 = ldg(a.E0[0], qk, okk);
-            const T eyk1 = ldg(a.E0[1], qk, okk);
+            const bool wr = own_plane && r >= 1 && own_row[r > 0 ? r - 1 : 0] && ok;
             T dEz_y = ec[r + 1][2] - ec[r][2];
             T dEx_y = ec[r + 1][0] - ec[r][0];
-            T dEy_z = eyk1 - ec[r][1];
-            T dEx_z = exk1 - ec[r][0];
+            T dEy_z = ek1[r][1] - ec[r][1];
+            T dEx_z = ek1[r][0] - ec[r][0];
             T dEz_x = en[r][2] - ec[r][2];
             T dEy_x = en[r][1] - ec[r][1];
-            if (ok && in_pml(a, 1, j, a.ny)) {
-                const long s = slab_y(a, pi, j, k);
+            if (ok && ypml[r]) {
                 const T b = a.bH[1][j], c = a.aH[1][j];
-                T p0 = a.psiH0[1][0][s], p1 = a.psiH0[1][1][s];
-                dEz_y = cpml(dEz_y, b, c, p0);
-                dEx_y = cpml(dEx_y, b, c, p1);
-                if (wr) { a.psiH1[1][0][s] = p0; a.psiH1[1][1][s] = p1; }
+                dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
+                dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
+                if (wr) {
+                    const long s = slab_y(a, pi, j, k);
+                    a.psiH1[1][0][s] = pH[r][1][0];
+                    a.psiH1[1][1][s] = pH[r][1][1];
+                }
             }
-            if (zpml && ok) {
-                const long s = slab_z(a, pi, j, k);
-                T p0 = a.psiH0[2][0][s], p1 = a.psiH0[2][1][s];
-                dEy_z = cpml(dEy_z, bHz, aHz, p0);
-                dEx_z = cpml(dEx_z, bHz, aHz, p1);
-                if (wr) { a.psiH1[2][0][s] = p0; a.psiH1[2][1][s] = p1; }
+            if (ok && zpml) {
+                dEy_z = cpml(dEy_z, bHz, aHz, pH[r][2][0]);
+                dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
+                if (wr) {
+                    const long s = slab_z(a, pi, j, k);
+                    a.psiH1[2][0][s] = pH[r][2][0];
+                    a.psiH1[2][1][s] = pH[r][2][1];
+                }
             }
-            if (xpml && ok) {
-                const long s = slab_x(a, pi, j, k);
-                T p0 = a.psiH0[0][0][s], p1 = a.psiH0[0][1][s];
-                dEz_x = cpml(dEz_x, bHx, aHx, p0);
-                dEy_x = cpml(dEy_x, bHx, aHx, p1);
-                if (wr) { a.psiH1[0][0][s] = p0; a.psiH1[0][1][s] = p1; }
+            if (ok && xpml) {
+                dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
+                dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
+                if (wr) {
+                    const long s = slab_x(a, pi, j, k);
+                    a.psiH1[0][0][s] = pH[r][0][0];
+                    a.psiH1[0][1][s] = pH[r][0][1];
+                }
             }
             // outside a metallic edge H is identically zero (ghost row of the halo)
-            h[r][0] = ok ? hx0 - cdt * (dEz_y - dEy_z) : T(0);
-            h[r][1] = ok ? hy0 - cdt * (dEx_z - dEz_x) : T(0);
-            h[r][2] = ok ? hz0 - cdt * (dEy_x - dEx_y) : T(0);
+            h[r][0] = ok ? h0[r][0] - cdt * (dEz_y - dEy_z) : T(0);
+            h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
+            h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
             if (wr) {
                 a.H1[0][q] = h[r][0];
                 a.H1[1][q] = h[r][1];
@@ -245,13 +359,13 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 buf[r * B + k] = h[r + 1][0];
                 buf[(TY + r) * B + k] = h[r + 1][1];
             }
-            __syncthreads();
+            lds_barrier();
 
-            // ---- E^{n+1} on rows 0..TY-1 of plane i ------------------------------------
+            // ------------------------------------------------------------ E^{n+1} ---------
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
+                if (!own_row[r]) continue;
                 const int j = gj[r + 1];
-                if (!(kin && (j0 + r) < a.ny)) continue;
                 const long q = (long)pi * NYZ + (long)j * NZ + k;
                 const T hxkm = km1 >= 0 ? buf[r * B + km1] : T(0);
                 const T hykm = km1 >= 0 ? buf[(TY + r) * B + km1] : T(0);
@@ -261,41 +375,37 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 T dHx_z = h[r + 1][0] - hxkm;
                 T dHz_x = h[r + 1][2] - hp[r][1];
                 T dHy_x = h[r + 1][1] - hp[r][0];
-                if (in_pml(a, 1, j, a.ny)) {
-                    const long s = slab_y(a, pi, j, k);
+                if (ypml[r + 1]) {
                     const T b = a.bE[1][j], c = a.aE[1][j];
-                    T p0 = a.psiE[1][0][s], p1 = a.psiE[1][1][s];
-                    dHz_y = cpml(dHz_y, b, c, p0);
-                    dHx_y = cpml(dHx_y, b, c, p1);
-                    a.psiE[1][0][s] = p0;
-                    a.psiE[1][1][s] = p1;
+                    dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
+                    dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
+                    const long s = slab_y(a, pi, j, k);
+                    a.psiE[1][0][s] = pE[r][1][0];
+                    a.psiE[1][1][s] = pE[r][1][1];
                 }
                 if (zpml) {
+                    dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
+                    dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
                     const long s = slab_z(a, pi, j, k);
-                    T p0 = a.psiE[2][0][s], p1 = a.psiE[2][1][s];
-                    dHy_z = cpml(dHy_z, bEz, aEz, p0);
-                    dHx_z = cpml(dHx_z, bEz, aEz, p1);
-                    a.psiE[2][0][s] = p0;
-                    a.psiE[2][1][s] = p1;
+                    a.psiE[2][0][s] = pE[r][2][0];
+                    a.psiE[2][1][s] = pE[r][2][1];
                 }
                 if (xpml) {
+                    dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
+                    dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
                     const long s = slab_x(a, pi, j, k);
-                    T p0 = a.psiE[0][0][s], p1 = a.psiE[0][1][s];
-                    dHz_x = cpml(dHz_x, bEx, aEx, p0);
-                    dHy_x = cpml(dHy_x, bEx, aEx, p1);
-                    a.psiE[0][0][s] = p0;
-                    a.psiE[0][1][s] = p1;
+                    a.psiE[0][0][s] = pE[r][0][0];
+                    a.psiE[0][1][s] = pE[r][0][1];
                 }
                 const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
-                if (a.uniform) {
+                if (uniform) {
                     const T inv = slut[0].inv_eps;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) a.E1[c][q] = ec[r + 1][c] + inv * (cdt * curl[c]);
                 } else {
-                    const uint32_t mw = a.matw[q];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const int m = (mw >> (8 * c)) & 0xff;
+                        const int m = (mw[r] >> (8 * c)) & 0xff;
                         const T e0 = ec[r + 1][c];
                         T inv, sq = T(0);
                         if (m == kDesignMat) {
@@ -306,17 +416,17 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                         } else {
                             const MatLut<T>& L = slut[m];
                             inv = L.inv_eps;
-                            for (int s = 0; s < L.np; ++s) {
-                                T* Qp = a.Q[s][c];
-                                T qn = L.alpha[s] * Qp[q] + L.c3[s] * e0;
-                                if ((L.lorentz_mask >> s) & 1) {
-                                    T* Pp = a.P[s][c];
-                                    const T p = Pp[q];
-                                    qn += L.beta[s] * p;
-                                    Pp[q] = p + qn;
+#pragma unroll
+                            for (int s = 0; s < NP; ++s) {
+                                if (s < L.np) {
+                                    T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * e0;
+                                    if ((L.lorentz_mask >> s) & 1) {
+                                        qn += L.beta[s] * Pv[r][c][s];
+                                        a.P[s][c][q] = Pv[r][c][s] + qn;
+                                    }
+                                    a.Q[s][c][q] = qn;
+                                    sq += qn;
                                 }
-                                Qp[q] = qn;
-                                sq += qn;
                             }
                         }
                         a.E1[c][q] = e0 + inv * (cdt * curl[c] - sq);
@@ -329,12 +439,15 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
         for (int r = 0; r < TY; ++r) {
             hp[r][0] = h[r + 1][1];
             hp[r][1] = h[r + 1][2];
+            mw[r] = mwn[r];
         }
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            ec[r][0] = en[r][0];
-            ec[r][1] = en[r][1];
-            ec[r][2] = en[r][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ec[r][c] = en[r][c];
+                en[r][c] = enn[r][c];
+            }
         }
     }
 }
@@ -412,6 +525,13 @@ __global__ void k_dft(const DftArgs<T> a, long nyz, long nz) {
             a.cmp[o] = cp;
         }
     }
+}
+
+// ---- plain streaming copy (achievable-bandwidth probe and counter calibration) -----------
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                              long n) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q];
 }
 
 }  // namespace sjf

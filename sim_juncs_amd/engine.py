@@ -31,6 +31,16 @@ def _ip(a):
     return arr
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:
+    """Achievable HBM bandwidth (GB/s, read + write) of a plain device copy of `nbytes`."""
+    lib = _lib.load()
+    g = C.c_double()
+    rc = lib.sjf_copy_probe(int(device), int(nbytes), int(iters), C.byref(g))
+    if rc != 0:
+        raise _lib.EngineError(f"sjf_copy_probe failed ({_lib.STATUS.get(rc, rc)})")
+    return g.value
+
+
 class Engine:
     """One simulation on one GPU.  precision: "fp32" or "fp64"."""
 
