@@ -320,10 +320,15 @@ static int axis_weights(int n, double dx, double off, int periodic, double lo, d
         }
         return cnt;
     }
+    /* on a periodic axis the cell of lattice point p also has images p +- n dx */
     for (int i = 0; i < n; i++) {
-        double p = (i + off) * dx;
-        double a = fmax(lo, p - 0.5 * dx), b = fmin(hi, p + 0.5 * dx);
-        if (b > a) { ids[cnt] = i; w[cnt] = (b - a) / dx; cnt++; }
+        double ww = 0;
+        for (int m = periodic ? -1 : 0; m <= (periodic ? 1 : 0); m++) {
+            double p = (i + off) * dx + m * n * dx;
+            double a = fmax(lo, p - 0.5 * dx), b = fmin(hi, p + 0.5 * dx);
+            if (b > a) ww += (b - a) / dx;
+        }
+        if (ww > 0) { ids[cnt] = i; w[cnt] = ww; cnt++; }
     }
     return cnt;
 }

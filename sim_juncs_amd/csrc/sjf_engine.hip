@@ -62,7 +62,8 @@ struct sjf_engine {
     hipStream_t stream = nullptr;
     std::string err;
 
-    void* E[2][3] = {};
+    void* Fbuf[2] = {};  // field sets [Ex Ey Ez Hx Hy Hz][N], read/write alternate every step
+    void* E[2][3] = {};  // views into Fbuf
     void* H[2][3] = {};
     int ping = 0;
 
@@ -75,8 +76,8 @@ struct sjf_engine {
     int nmat = 1;
     int maxp = 0;
     bool lorentz_slot[kMaxPoles] = {};
-    void* Q[kMaxPoles][3] = {};
-    void* P[kMaxPoles][3] = {};
+    void* Qbuf = nullptr;  // [slot][comp][N]
+    void* Pbuf = nullptr;  // [slot][comp][N], present when a slot holds a Lorentz pole
     bool uniform = true;
     // design region
     int dlo[3] = {0, 0, 0}, dhi[3] = {0, 0, 0};
@@ -86,9 +87,10 @@ struct sjf_engine {
     // PML
     int pml[6] = {0, 0, 0, 0, 0, 0};
     bool pml_on[3] = {false, false, false};
-    void* coef[3][4] = {};  // bE, aE, bH, aH
-    void* psiE[3][2] = {};
-    void* psiH[2][3][2] = {};
+    void* coefbuf = nullptr;  // per axis: bE[n], aE[n], bH[n], aH[n]
+    int coff[3] = {0, 0, 0};
+    long slab[3] = {0, 0, 0};
+    void* psibuf[3] = {};     // per axis [6][slab]: E aux x2, H aux pair A x2, pair B x2
 
     std::vector<Source> src;
     bool src_dirty = true;
@@ -191,12 +193,17 @@ void axis_weights(int n, double dx, double off, int periodic, double lo, double 
         }
         return;
     }
+    // on a periodic axis the cell of lattice point p also has images p +- n dx
     for (int i = 0; i < n; ++i) {
-        const double p = (i + off) * dx;
-        const double a = std::max(lo, p - 0.5 * dx), b = std::min(hi, p + 0.5 * dx);
-        if (b > a) {
+        double ww = 0;
+        for (int m = periodic ? -1 : 0; m <= (periodic ? 1 : 0); ++m) {
+            const double p = (i + off) * dx + m * n * dx;
+            const double a = std::max(lo, p - 0.5 * dx), b = std::min(hi, p + 0.5 * dx);
+            if (b > a) ww += (b - a) / dx;
+        }
+        if (ww > 0) {
             ids.push_back(i);
-            w.push_back((b - a) / dx);
+            w.push_back(ww);
         }
     }
 }
@@ -336,39 +343,30 @@ sjf::StepArgs<T> step_args(const sjf_engine* e) {
     a.pz = e->g.periodic[2];
     a.nty = (e->g.ny + e->ty - 1) / e->ty;
     a.xc = e->xc;
+    a.N = e->N;
     a.cdt = (T)(e->dt / e->g.dx);
-    const int b0 = e->ping, b1 = e->ping ^ 1;
-    for (int c = 0; c < 3; ++c) {
-        a.E0[c] = (const T*)e->E[b0][c];
-        a.H0[c] = (const T*)e->H[b0][c];
-        a.E1[c] = (T*)e->E[b1][c];
-        a.H1[c] = (T*)e->H[b1][c];
-    }
+    a.F0 = (const T*)e->Fbuf[e->ping];
+    a.F1 = (T*)e->Fbuf[e->ping ^ 1];
     a.uniform = e->uniform ? 1 : 0;
     a.nmat = e->nmat;
     a.matw = e->matw;
     a.lut = (const sjf::MatLut<T>*)e->lut;
-    for (int s = 0; s < kMaxPoles; ++s)
-        for (int c = 0; c < 3; ++c) {
-            a.Q[s][c] = (T*)e->Q[s][c];
-            a.P[s][c] = (T*)e->P[s][c];
-        }
+    a.Q = (T*)e->Qbuf;
+    a.P = (T*)e->Pbuf;
     for (int d = 0; d < 3; ++d) {
         a.pml_on[d] = e->pml_on[d] ? 1 : 0;
         a.plo[d] = e->pml[2 * d];
         a.phi[d] = e->pml[2 * d + 1];
-        a.bE[d] = (const T*)e->coef[d][0];
-        a.aE[d] = (const T*)e->coef[d][1];
-        a.bH[d] = (const T*)e->coef[d][2];
-        a.aH[d] = (const T*)e->coef[d][3];
-        for (int s = 0; s < 2; ++s) {
-            a.psiE[d][s] = (T*)e->psiE[d][s];
-            a.psiH0[d][s] = (const T*)e->psiH[b0][d][s];
-            a.psiH1[d][s] = (T*)e->psiH[b1][d][s];
-        }
+        a.S[d] = e->slab[d];
+        a.psi[d] = (T*)e->psibuf[d];
+        a.coff[d] = e->coff[d];
         a.dlo[d] = e->dlo[d];
         a.dhi[d] = e->dhi[d];
     }
+    // the H auxiliaries alternate between segment pairs 2-3 and 4-5 with the field sets
+    a.hr = e->ping == 0 ? 2 : 4;
+    a.hw = e->ping == 0 ? 4 : 2;
+    a.coef = (const T*)e->coefbuf;
     a.dinv = (const T*)e->dinv;
     return a;
 }
@@ -546,7 +544,7 @@ void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
     // Defaults measured on MI355X: one or two rows per thread and short x-chunks (many
     // workgroups) beat deep register blocking; the autotuner refines per workload.
-    e->ty = (e->prec == 4 && e->g.ny >= 2) ? 2 : 1;
+    e->ty = 1;
     const int nty = (e->g.ny + e->ty - 1) / e->ty;
     const long target_groups = 16384;
     long nxc = std::max(1L, (target_groups + nty - 1) / nty);
@@ -556,27 +554,14 @@ void choose_tile(sjf_engine* e) {
 }
 
 int free_all(sjf_engine* e) {
-    for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 3; ++c) {
-            dev_free(e->E[b][c]);
-            dev_free(e->H[b][c]);
-        }
+    for (int b = 0; b < 2; ++b) dev_free(e->Fbuf[b]);
     dev_free((void*&)e->matw);
     dev_free(e->lut);
     dev_free(e->dinv);
-    for (int s = 0; s < kMaxPoles; ++s)
-        for (int c = 0; c < 3; ++c) {
-            dev_free(e->Q[s][c]);
-            dev_free(e->P[s][c]);
-        }
-    for (int d = 0; d < 3; ++d) {
-        for (int q = 0; q < 4; ++q) dev_free(e->coef[d][q]);
-        for (int s = 0; s < 2; ++s) {
-            dev_free(e->psiE[d][s]);
-            dev_free(e->psiH[0][d][s]);
-            dev_free(e->psiH[1][d][s]);
-        }
-    }
+    dev_free(e->Qbuf);
+    dev_free(e->Pbuf);
+    dev_free(e->coefbuf);
+    for (int d = 0; d < 3; ++d) dev_free(e->psibuf[d]);
     for (auto& s : e->src) {
         dev_free((void*&)s.d_idx);
         dev_free((void*&)s.d_coef);
@@ -730,17 +715,44 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
         delete e;
         return SJF_E_HIP;
     }
-    for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 3; ++c) {
-            if (dev_alloc(e, &e->E[b][c], (size_t)e->prec * e->N) ||
-                dev_alloc(e, &e->H[b][c], (size_t)e->prec * e->N)) {
-                g_create_error = e->err;
-                free_all(e);
-                hipStreamDestroy(e->stream);
-                delete e;
-                return SJF_E_NOMEM;
-            }
+    for (int b = 0; b < 2; ++b) {
+        if (dev_alloc(e, &e->Fbuf[b], (size_t)e->prec * 6 * e->N)) {
+            g_create_error = e->err;
+            free_all(e);
+            hipStreamDestroy(e->stream);
+            delete e;
+            return SJF_E_NOMEM;
         }
+        for (int c = 0; c < 3; ++c) {
+            e->E[b][c] = (char*)e->Fbuf[b] + (size_t)e->prec * c * e->N;
+            e->H[b][c] = (char*)e->Fbuf[b] + (size_t)e->prec * (3 + c) * e->N;
+        }
+    }
+    // PML tables for all axes (b = 1, a = 0 until sjf_set_pml fills an axis)
+    {
+        const int n3[3] = {g->nx, g->ny, g->nz};
+        int off = 0;
+        for (int d = 0; d < 3; ++d) {
+            e->coff[d] = off;
+            off += 4 * n3[d];
+        }
+        std::vector<double> tab(off, 0.0);
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n3[d]; ++i) {
+                tab[e->coff[d] + i] = 1.0;
+                tab[e->coff[d] + 2 * n3[d] + i] = 1.0;
+            }
+        int rc = dev_alloc(e, &e->coefbuf, (size_t)e->prec * off);
+        if (!rc) rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
+                                   : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
+        if (rc) {
+            g_create_error = e->err;
+            free_all(e);
+            hipStreamDestroy(e->stream);
+            delete e;
+            return rc;
+        }
+    }
     // default material: vacuum
     e->lut_h.assign(1, sjf_material{});
     e->lut_h[0].eps_inf = 1.0;
@@ -805,16 +817,15 @@ int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_materia
     rc = h2d(e, e->matw, mat_word, sizeof(uint32_t) * e->N);
     if (rc) return rc;
     // polarisation state: dense per-slot arrays, zeroed
-    for (int s = 0; s < kMaxPoles; ++s)
-        for (int c = 0; c < 3; ++c) {
-            dev_free(e->Q[s][c]);
-            dev_free(e->P[s][c]);
-            if (s < maxp) {
-                rc = dev_alloc(e, &e->Q[s][c], (size_t)e->prec * e->N);
-                if (!rc && lor[s]) rc = dev_alloc(e, &e->P[s][c], (size_t)e->prec * e->N);
-                if (rc) return rc;
-            }
-        }
+    dev_free(e->Qbuf);
+    dev_free(e->Pbuf);
+    if (maxp > 0) {
+        bool anylor = false;
+        for (int s = 0; s < maxp; ++s) anylor |= lor[s];
+        rc = dev_alloc(e, &e->Qbuf, (size_t)e->prec * 3 * maxp * e->N);
+        if (!rc && anylor) rc = dev_alloc(e, &e->Pbuf, (size_t)e->prec * 3 * maxp * e->N);
+        if (rc) return rc;
+    }
     for (int s = 0; s < kMaxPoles; ++s) e->lorentz_slot[s] = lor[s];
     rc = upload_lut(e);
     if (rc) return rc;
@@ -880,21 +891,18 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         if ((lo > 0 || hi > 0) && e->g.periodic[d])
             return fail(e, SJF_E_ARG, "PML on a periodic axis");
     }
+    std::vector<double> tab;
+    int off = 0;
+    for (int d = 0; d < 3; ++d) off += 4 * axis_n(e, d);
+    tab.assign(off, 0.0);
     for (int d = 0; d < 3; ++d) {
         const int n = axis_n(e, d);
         const int lo = cells[2 * d], hi = cells[2 * d + 1];
         e->pml[2 * d] = lo;
         e->pml[2 * d + 1] = hi;
         e->pml_on[d] = (lo > 0 || hi > 0);
-        for (int q = 0; q < 4; ++q) dev_free(e->coef[d][q]);
-        for (int s = 0; s < 2; ++s) {
-            dev_free(e->psiE[d][s]);
-            dev_free(e->psiH[0][d][s]);
-            dev_free(e->psiH[1][d][s]);
-        }
-        if (!e->pml_on[d]) continue;
-        std::vector<double> tab[4];
-        for (int q = 0; q < 4; ++q) tab[q].resize(n);
+        dev_free(e->psibuf[d]);
+        e->slab[d] = 0;
         const double dx = e->g.dx;
         auto sigma = [&](double x) {
             double s = 0;
@@ -911,25 +919,20 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         for (int i = 0; i < n; ++i) {
             const double bE = std::exp(-sigma(i * dx) * e->dt);
             const double bH = std::exp(-sigma((i + 0.5) * dx) * e->dt);
-            tab[0][i] = bE;
-            tab[1][i] = bE - 1.0;
-            tab[2][i] = bH;
-            tab[3][i] = bH - 1.0;
+            tab[e->coff[d] + i] = bE;
+            tab[e->coff[d] + n + i] = bE - 1.0;
+            tab[e->coff[d] + 2 * n + i] = bH;
+            tab[e->coff[d] + 3 * n + i] = bH - 1.0;
         }
-        for (int q = 0; q < 4; ++q) {
-            int rc = dev_alloc(e, &e->coef[d][q], (size_t)e->prec * n);
-            if (rc) return rc;
-            rc = e->prec == 4 ? convert_to_device<float>(e, e->coef[d][q], tab[q].data(), n)
-                              : convert_to_device<double>(e, e->coef[d][q], tab[q].data(), n);
-            if (rc) return rc;
-        }
-        long slab = (long)(lo + hi) * (e->N / n);
-        for (int s = 0; s < 2; ++s) {
-            int rc = dev_alloc(e, &e->psiE[d][s], (size_t)e->prec * slab);
-            if (!rc) rc = dev_alloc(e, &e->psiH[0][d][s], (size_t)e->prec * slab);
-            if (!rc) rc = dev_alloc(e, &e->psiH[1][d][s], (size_t)e->prec * slab);
-            if (rc) return rc;
-        }
+        if (!e->pml_on[d]) continue;
+        e->slab[d] = (long)(lo + hi) * (e->N / n);
+        int rc = dev_alloc(e, &e->psibuf[d], (size_t)e->prec * 6 * e->slab[d]);
+        if (rc) return rc;
+    }
+    {
+        int rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
+                              : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
+        if (rc) return rc;
     }
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return SJF_OK;

@@ -45,31 +45,31 @@ struct MatLut {
     T c3[kMaxPoles];
 };
 
+// Kernel arguments of the fused step.  Few base pointers on purpose: the component, pole-slot
+// and PML-segment arrays are addressed as offsets from them, so the whole argument block stays
+// in scalar registers (a per-array pointer table overflows the SGPR file and spills into
+// vector-register lanes).
 template <typename T>
 struct StepArgs {
     int nx, ny, nz;
     int px, py, pz;
     int nty, xc;
-    T cdt;
-    const T* E0[3];
-    const T* H0[3];
-    T* E1[3];
-    T* H1[3];
+    long N;       // cells
+    T cdt;        // dt / dx
+    const T* F0;  // read state  [Ex Ey Ez Hx Hy Hz][N]
+    T* F1;        // write state [Ex Ey Ez Hx Hy Hz][N]
     int uniform;  // 1: single non-dispersive material, no material word read
     int nmat;
     const uint32_t* matw;
     const MatLut<T>* lut;
-    T* Q[kMaxPoles][3];
-    T* P[kMaxPoles][3];
-    int pml_on[3];
-    int plo[3], phi[3];
-    const T* bE[3];
-    const T* aE[3];
-    const T* bH[3];
-    const T* aH[3];
-    T* psiE[3][2];
-    const T* psiH0[3][2];
-    T* psiH1[3][2];
+    T* Q;  // [slot][comp][N]
+    T* P;  // [slot][comp][N] (Lorentz slots)
+    int pml_on[3], plo[3], phi[3];
+    long S[3];      // PML slab sizes (cells) per axis
+    T* psi[3];      // per axis [6][S]: E aux (2 comps), H aux pair A (2), H aux pair B (2)
+    int hr, hw;     // first segment of the H aux pair read / written this step (2 or 4)
+    const T* coef;  // PML tables: for axis d at coff[d]: bE[n], aE[n], bH[n], aH[n]
+    int coff[3];
     int dlo[3], dhi[3];
     const T* dinv;
 };
@@ -163,8 +163,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
     for (int r = 0; r < TY + 1; ++r) ypml[r] = gj[r] >= 0 && in_pml(a, 1, gj[r], a.ny);
 
     const bool zpml = kin && in_pml(a, 2, k, a.nz);
-    const T bEz = zpml ? a.bE[2][k] : T(0), aEz = zpml ? a.aE[2][k] : T(0);
-    const T bHz = zpml ? a.bH[2][k] : T(0), aHz = zpml ? a.aH[2][k] : T(0);
+    const T* cz = a.coef + a.coff[2];
+    const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[a.nz + k] : T(0);
+    const T bHz = zpml ? cz[2 * a.nz + k] : T(0), aHz = zpml ? cz[3 * a.nz + k] : T(0);
 
     lds_barrier();
 
@@ -185,8 +186,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             const long q1 = (long)p1 * NYZ + (long)gj[r] * NZ + k;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                ec[r][c] = ldg(a.E0[c], q0, ok0);
-                en[r][c] = ldg(a.E0[c], q1, ok1);
+                ec[r][c] = ldg((a.F0 + c * a.N), q0, ok0);
+                en[r][c] = ldg((a.F0 + c * a.N), q1, ok1);
             }
         }
 #pragma unroll
@@ -204,8 +205,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
         const int pn = wrap_or_ghost(i + 1, a.nx, a.px);
         const int pnn = wrap_or_ghost(i + 2, a.nx, a.px);
         const bool xpml = in_pml(a, 0, pi, a.nx);
-        const T bEx = xpml ? a.bE[0][pi] : T(0), aEx = xpml ? a.aE[0][pi] : T(0);
-        const T bHx = xpml ? a.bH[0][pi] : T(0), aHx = xpml ? a.aH[0][pi] : T(0);
+        const T* cx = a.coef + a.coff[0];
+        const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[a.nx + pi] : T(0);
+        const T bHx = xpml ? cx[2 * a.nx + pi] : T(0), aHx = xpml ? cx[3 * a.nx + pi] : T(0);
 
         // ---------------------------------------------------------------- loads ----------
         T enn[TY + 2][3];
@@ -214,9 +216,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
         for (int r = 0; r < TY + 2; ++r) {
             const bool ok = has_next && kin && gj[r] >= 0 && pnn >= 0;
             const long q = (long)pnn * NYZ + (long)gj[r] * NZ + k;
-            enn[r][0] = ldg(a.E0[0], q, ok);
-            enn[r][1] = (r < TY + 1) ? ldg(a.E0[1], q, ok) : T(0);
-            enn[r][2] = ldg(a.E0[2], q, ok);
+            enn[r][0] = ldg((a.F0 + 0 * a.N), q, ok);
+            enn[r][1] = (r < TY + 1) ? ldg((a.F0 + 1 * a.N), q, ok) : T(0);
+            enn[r][2] = ldg((a.F0 + 2 * a.N), q, ok);
         }
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
@@ -229,29 +231,29 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             const int j = gj[r];
             const bool ok = kin && j >= 0;
             const long q = (long)pi * NYZ + (long)j * NZ + k;
-            h0[r][0] = ldg(a.H0[0], q, ok);
-            h0[r][1] = ldg(a.H0[1], q, ok);
-            h0[r][2] = ldg(a.H0[2], q, ok);
+            h0[r][0] = ldg((a.F0 + 3 * a.N), q, ok);
+            h0[r][1] = ldg((a.F0 + 4 * a.N), q, ok);
+            h0[r][2] = ldg((a.F0 + 5 * a.N), q, ok);
             const bool okk = ok && kp1 >= 0;
             const long qk = q - k + kp1;
-            ek1[r][0] = ldg(a.E0[0], qk, okk);
-            ek1[r][1] = ldg(a.E0[1], qk, okk);
+            ek1[r][0] = ldg((a.F0 + 0 * a.N), qk, okk);
+            ek1[r][1] = ldg((a.F0 + 1 * a.N), qk, okk);
 #pragma unroll
             for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
             if (ok && xpml) {
                 const long s = slab_x(a, pi, j, k);
-                pH[r][0][0] = a.psiH0[0][0][s];
-                pH[r][0][1] = a.psiH0[0][1][s];
+                pH[r][0][0] = a.psi[0][(a.hr + 0) * a.S[0] + s];
+                pH[r][0][1] = a.psi[0][(a.hr + 1) * a.S[0] + s];
             }
             if (ok && ypml[r]) {
                 const long s = slab_y(a, pi, j, k);
-                pH[r][1][0] = a.psiH0[1][0][s];
-                pH[r][1][1] = a.psiH0[1][1][s];
+                pH[r][1][0] = a.psi[1][(a.hr + 0) * a.S[1] + s];
+                pH[r][1][1] = a.psi[1][(a.hr + 1) * a.S[1] + s];
             }
             if (ok && zpml) {
                 const long s = slab_z(a, pi, j, k);
-                pH[r][2][0] = a.psiH0[2][0][s];
-                pH[r][2][1] = a.psiH0[2][1][s];
+                pH[r][2][0] = a.psi[2][(a.hr + 0) * a.S[2] + s];
+                pH[r][2][1] = a.psi[2][(a.hr + 1) * a.S[2] + s];
             }
         }
         T pE[TY][3][2];
@@ -269,18 +271,18 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             const long q = (long)pi * NYZ + (long)j * NZ + k;
             if (xpml) {
                 const long s = slab_x(a, pi, j, k);
-                pE[r][0][0] = a.psiE[0][0][s];
-                pE[r][0][1] = a.psiE[0][1][s];
+                pE[r][0][0] = a.psi[0][0 * a.S[0] + s];
+                pE[r][0][1] = a.psi[0][1 * a.S[0] + s];
             }
             if (ypml[r + 1]) {
                 const long s = slab_y(a, pi, j, k);
-                pE[r][1][0] = a.psiE[1][0][s];
-                pE[r][1][1] = a.psiE[1][1][s];
+                pE[r][1][0] = a.psi[1][0 * a.S[1] + s];
+                pE[r][1][1] = a.psi[1][1 * a.S[1] + s];
             }
             if (zpml) {
                 const long s = slab_z(a, pi, j, k);
-                pE[r][2][0] = a.psiE[2][0][s];
-                pE[r][2][1] = a.psiE[2][1][s];
+                pE[r][2][0] = a.psi[2][0 * a.S[2] + s];
+                pE[r][2][1] = a.psi[2][1 * a.S[2] + s];
             }
             if (NP > 0 && !uniform) {
 #pragma unroll
@@ -291,8 +293,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
                         if (s < np) {
-                            Qv[r][c][s] = a.Q[s][c][q];
-                            if ((lm >> s) & 1) Pv[r][c][s] = a.P[s][c][q];
+                            Qv[r][c][s] = a.Q[(s * 3 + c) * a.N + q];
+                            if ((lm >> s) & 1) Pv[r][c][s] = a.P[(s * 3 + c) * a.N + q];
                         }
                     }
                 }
@@ -314,13 +316,14 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             T dEz_x = en[r][2] - ec[r][2];
             T dEy_x = en[r][1] - ec[r][1];
             if (ok && ypml[r]) {
-                const T b = a.bH[1][j], c = a.aH[1][j];
+                const T* cy = a.coef + a.coff[1];
+                const T b = cy[2 * a.ny + j], c = cy[3 * a.ny + j];
                 dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
                 dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
                 if (wr) {
                     const long s = slab_y(a, pi, j, k);
-                    a.psiH1[1][0][s] = pH[r][1][0];
-                    a.psiH1[1][1][s] = pH[r][1][1];
+                    a.psi[1][(a.hw + 0) * a.S[1] + s] = pH[r][1][0];
+                    a.psi[1][(a.hw + 1) * a.S[1] + s] = pH[r][1][1];
                 }
             }
             if (ok && zpml) {
@@ -328,8 +331,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                 if (wr) {
                     const long s = slab_z(a, pi, j, k);
-                    a.psiH1[2][0][s] = pH[r][2][0];
-                    a.psiH1[2][1][s] = pH[r][2][1];
+                    a.psi[2][(a.hw + 0) * a.S[2] + s] = pH[r][2][0];
+                    a.psi[2][(a.hw + 1) * a.S[2] + s] = pH[r][2][1];
                 }
             }
             if (ok && xpml) {
@@ -337,8 +340,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
                 if (wr) {
                     const long s = slab_x(a, pi, j, k);
-                    a.psiH1[0][0][s] = pH[r][0][0];
-                    a.psiH1[0][1][s] = pH[r][0][1];
+                    a.psi[0][(a.hw + 0) * a.S[0] + s] = pH[r][0][0];
+                    a.psi[0][(a.hw + 1) * a.S[0] + s] = pH[r][0][1];
                 }
             }
             // outside a metallic edge H is identically zero (ghost row of the halo)
@@ -346,9 +349,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
             h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
             if (wr) {
-                a.H1[0][q] = h[r][0];
-                a.H1[1][q] = h[r][1];
-                a.H1[2][q] = h[r][2];
+                a.F1[3 * a.N + q] = h[r][0];
+                a.F1[4 * a.N + q] = h[r][1];
+                a.F1[5 * a.N + q] = h[r][2];
             }
         }
 
@@ -376,32 +379,33 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 T dHz_x = h[r + 1][2] - hp[r][1];
                 T dHy_x = h[r + 1][1] - hp[r][0];
                 if (ypml[r + 1]) {
-                    const T b = a.bE[1][j], c = a.aE[1][j];
+                    const T* cy = a.coef + a.coff[1];
+                    const T b = cy[j], c = cy[a.ny + j];
                     dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
                     dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
                     const long s = slab_y(a, pi, j, k);
-                    a.psiE[1][0][s] = pE[r][1][0];
-                    a.psiE[1][1][s] = pE[r][1][1];
+                    a.psi[1][0 * a.S[1] + s] = pE[r][1][0];
+                    a.psi[1][1 * a.S[1] + s] = pE[r][1][1];
                 }
                 if (zpml) {
                     dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
                     dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
                     const long s = slab_z(a, pi, j, k);
-                    a.psiE[2][0][s] = pE[r][2][0];
-                    a.psiE[2][1][s] = pE[r][2][1];
+                    a.psi[2][0 * a.S[2] + s] = pE[r][2][0];
+                    a.psi[2][1 * a.S[2] + s] = pE[r][2][1];
                 }
                 if (xpml) {
                     dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
                     dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
                     const long s = slab_x(a, pi, j, k);
-                    a.psiE[0][0][s] = pE[r][0][0];
-                    a.psiE[0][1][s] = pE[r][0][1];
+                    a.psi[0][0 * a.S[0] + s] = pE[r][0][0];
+                    a.psi[0][1 * a.S[0] + s] = pE[r][0][1];
                 }
                 const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
                 if (uniform) {
                     const T inv = slut[0].inv_eps;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) a.E1[c][q] = ec[r + 1][c] + inv * (cdt * curl[c]);
+                    for (int c = 0; c < 3; ++c) a.F1[c * a.N + q] = ec[r + 1][c] + inv * (cdt * curl[c]);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -422,14 +426,14 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                                     T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * e0;
                                     if ((L.lorentz_mask >> s) & 1) {
                                         qn += L.beta[s] * Pv[r][c][s];
-                                        a.P[s][c][q] = Pv[r][c][s] + qn;
+                                        a.P[(s * 3 + c) * a.N + q] = Pv[r][c][s] + qn;
                                     }
-                                    a.Q[s][c][q] = qn;
+                                    a.Q[(s * 3 + c) * a.N + q] = qn;
                                     sq += qn;
                                 }
                             }
                         }
-                        a.E1[c][q] = e0 + inv * (cdt * curl[c] - sq);
+                        a.F1[c * a.N + q] = e0 + inv * (cdt * curl[c] - sq);
                     }
                 }
             }
