@@ -30,6 +30,7 @@ extern "C" {
 #endif
 
 typedef struct sjf_engine sjf_engine;
+typedef struct sjf_batch sjf_batch;
 
 /* status codes (mirror the reference's parse_ercode style: src/cgs_read.hpp:32) */
 enum {
@@ -165,6 +166,9 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap);
 int sjf_step(sjf_engine* e, long n);
 /* Waits for all work enqueued on the engine's stream. */
 int sjf_synchronize(sjf_engine* e);
+/* Steps per captured hipGraph replayed by sjf_step (rounded up to a multiple of the save span);
+ * 0 launches every step directly.  Results are identical either way. */
+int sjf_set_graph(sjf_engine* e, int chunk);
 /* Runs n steps and reports, from HIP events on the engine's stream, the wall time of the
  * whole region and the mean duration of the fused update kernel (ms). */
 int sjf_timed_steps(sjf_engine* e, long n, double* ms_total, double* ms_kernel_mean);
@@ -187,6 +191,23 @@ int sjf_step_bytes(const sjf_engine* e, double* bytes);
 int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int* groups);
 /* Override the autotuned tile shape (ty in {1,2,4,8}, xchunk >= 1). */
 int sjf_set_tile(sjf_engine* e, int ty, int xchunk);
+
+/* ---- batches ----------------------------------------------------------------------------
+ * New (the reference sweeps parameters with one meep process per SLURM task,
+ * scripts/run.sh:47-78): engines of any grid shapes, same device, precision, step count and
+ * save span, advanced together.  Each time step runs as one launch of the fused update over
+ * every engine's tiles, one source launch, one DFT launch and (every save span) one monitor
+ * launch; runs of steps are replayed from a captured hipGraph.  Engines stay readable with the
+ * sjf_read_* calls (which wait for the batch) but cannot be stepped on their own until the
+ * batch is destroyed. */
+int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out);
+int sjf_batch_step(sjf_batch* b, long n);
+int sjf_batch_synchronize(sjf_batch* b);
+int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean);
+/* Total cells and algorithmic bytes of one fused launch over the batch. */
+int sjf_batch_info(const sjf_batch* b, long* cells, double* bytes_per_step);
+const char* sjf_batch_last_error(const sjf_batch* b);
+int sjf_batch_destroy(sjf_batch* b);
 
 /* Achievable HBM bandwidth: times `iters` launches of a plain streaming copy of `bytes` bytes
  * on `device` and reports (read + write bytes) / time in GB/s.  New; used as the measured

@@ -1,11 +1,14 @@
 // sjf_engine.hip -- host side of the MI355X FDTD engine behind include/sjf.h.
 //
-// Owns the device state of one simulation (fields, materials, PML auxiliaries, sources,
-// monitors, DFT accumulators) and enqueues, per time step, on the engine's own stream:
-//   [monitor gather when step % save_span == 0] -> k_step -> k_source (per source) ->
-//   k_dft (per DFT region), then swaps the double-buffered E/H/psi_H arrays.
-// This is the sequence bound_geom::run performs around fields.step()
-// (reference src/disp.cpp:719-741).
+// An engine owns the device state of one simulation (fields, materials, PML auxiliaries,
+// polarisation state, sources, monitors, DFT accumulators).  Stepping goes through a launch
+// set: one engine, or a batch of engines, compiled into small device tables so that each
+// time step is a fixed sequence of launches on one stream,
+//   [k_monitor when step % save_span == 0] -> k_step -> k_source -> k_dft -> k_tick,
+// whatever the number of instances.  The per-step scalars (step index, pulse value, DFT
+// phase) are computed on the device from each engine's step counter, so a run of steps is
+// captured once in a hipGraph and replayed.  This is the sequence bound_geom::run performs
+// around fields.step() (reference src/disp.cpp:719-741).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,12 +30,9 @@ thread_local std::string g_create_error;
 struct Source {
     int comp;
     sjf_pulse p;
-    double omega = 0, phi = 0, peak = 0, cutoff = 0;  // gaussian, derived
-    double amp_re = 1, amp_im = 0;
+    sjf::PulseDev dev;
     std::vector<long> idx;
     std::vector<double> wt;
-    long* d_idx = nullptr;
-    double* d_coef = nullptr;
 };
 
 struct Monitor {
@@ -50,7 +50,10 @@ struct Dft {
     int comps[6];
     double2* acc = nullptr;
     double2* cmp = nullptr;
+    double* d_freq = nullptr;
 };
+
+struct LaunchSet;
 
 }  // namespace
 
@@ -61,11 +64,11 @@ struct sjf_engine {
     double dt = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    long version = 0;  // bumped by every change that invalidates compiled launch tables
 
-    void* Fbuf[2] = {};  // field sets [Ex Ey Ez Hx Hy Hz][N], read/write alternate every step
-    void* E[2][3] = {};  // views into Fbuf
-    void* H[2][3] = {};
-    int ping = 0;
+    void* Fbuf[2] = {};  // field sets [Ex Ey Ez Hx Hy Hz][N]; step n reads set n&1
+    long* d_step = nullptr;
+    long step = 0;
 
     // materials
     bool have_mat = false;
@@ -75,11 +78,9 @@ struct sjf_engine {
     void* lut = nullptr;
     int nmat = 1;
     int maxp = 0;
-    bool lorentz_slot[kMaxPoles] = {};
     void* Qbuf = nullptr;  // [slot][comp][N]
     void* Pbuf = nullptr;  // [slot][comp][N], present when a slot holds a Lorentz pole
     bool uniform = true;
-    // design region
     int dlo[3] = {0, 0, 0}, dhi[3] = {0, 0, 0};
     std::vector<double> deps_h;
     void* dinv = nullptr;
@@ -90,25 +91,21 @@ struct sjf_engine {
     void* coefbuf = nullptr;  // per axis: bE[n], aE[n], bH[n], aH[n]
     int coff[3] = {0, 0, 0};
     long slab[3] = {0, 0, 0};
-    void* psibuf[3] = {};     // per axis [6][slab]: E aux x2, H aux pair A x2, pair B x2
+    void* psibuf[3] = {};  // per axis [6][slab]: E aux x2, H aux pair A x2, pair B x2
 
     std::vector<Source> src;
-    bool src_dirty = true;
-
     std::vector<Monitor> mon;
     int save_span = 1;
     long nsamp = 0, samp_cap = 0;
-    long* d_mon_idx = nullptr;
-    double* d_mon_w = nullptr;
-    int* d_mon_comp = nullptr;
     double* d_samples = nullptr;
-    bool mon_dirty = false;
-
     std::vector<Dft> dft;
 
-    long step = 0;
-    int ty = 4, xc = 32;
+    int ty = 1, xc = 16;
     bool tile_set = false;
+    int graph_chunk = 32;  // steps per captured graph (0: no graphs)
+
+    LaunchSet* own = nullptr;
+    sjf_batch* batch = nullptr;
 };
 
 namespace {
@@ -141,8 +138,9 @@ int dev_alloc(sjf_engine* e, void** p, size_t bytes) {
     return SJF_OK;
 }
 
-void dev_free(void*& p) {
-    if (p) hipFree(p);
+template <typename P>
+void dev_free(P*& p) {
+    if (p) (void)hipFree((void*)p);
     p = nullptr;
 }
 
@@ -208,37 +206,6 @@ void axis_weights(int n, double dx, double off, int periodic, double lo, double 
     }
 }
 
-// gaussian_src_time_phase::dipole / continuous_src_time::dipole (see sjf.h)
-void dipole(const Source& s, double t, double* re, double* im) {
-    if (s.p.kind == SJF_SRC_GAUSSIAN) {
-        const double tt = t - s.peak;
-        if ((double)(float)std::fabs(tt) > s.cutoff) {
-            *re = *im = 0.0;
-            return;
-        }
-        const double g = std::exp(-tt * tt / (2 * s.p.width * s.p.width));
-        const double th = -s.omega * tt - s.phi;
-        *re = -g * std::sin(th) / s.omega;
-        *im = g * std::cos(th) / s.omega;
-    } else {
-        const float rt = (float)t;
-        if (rt < s.p.t_start || rt > s.p.t_end) {
-            *re = *im = 0.0;
-            return;
-        }
-        const double w = 2 * M_PI * s.p.freq;
-        double env = 1.0;
-        if (s.p.width != 0.0) {
-            const double ts = (t - s.p.t_start) / s.p.width - s.p.slowness;
-            const double te = (s.p.t_end - t) / s.p.width - s.p.slowness;
-            env = (1.0 + std::tanh(ts)) * (1.0 + std::tanh(te)) * 0.25;
-        }
-        const double th = -w * t;
-        *re = -env * std::sin(th) / w;
-        *im = env * std::cos(th) / w;
-    }
-}
-
 double inv_eps_at(const sjf_engine* e, long q, int c) {
     if (!e->have_mat) return 1.0;
     const int m = (e->matw_h[q] >> (8 * c)) & 0xff;
@@ -254,85 +221,25 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
     return 1.0 / e->lut_h[m].eps_inf;
 }
 
-int upload_sources(sjf_engine* e) {
-    if (!e->src_dirty) return SJF_OK;
-    for (auto& s : e->src) {
-        const long n = (long)s.idx.size();
-        std::vector<double> coef(n);
-        for (long q = 0; q < n; ++q) coef[q] = e->dt * s.wt[q] * inv_eps_at(e, s.idx[q], s.comp);
-        if (!s.d_idx) {
-            int rc = dev_alloc(e, (void**)&s.d_idx, sizeof(long) * std::max(n, 1L));
-            if (rc) return rc;
-            rc = dev_alloc(e, (void**)&s.d_coef, sizeof(double) * std::max(n, 1L));
-            if (rc) return rc;
-            HIPCHK(e, hipMemcpyAsync(s.d_idx, s.idx.data(), sizeof(long) * n,
-                                     hipMemcpyHostToDevice, e->stream));
-        }
-        HIPCHK(e, hipMemcpyAsync(s.d_coef, coef.data(), sizeof(double) * n,
-                                 hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipStreamSynchronize(e->stream));
-    }
-    e->src_dirty = false;
-    return SJF_OK;
+int block_threads(const sjf_engine* e) { return ((e->g.nz + 63) / 64) * 64; }
+
+int tiles_of(const sjf_engine* e, int ty) {
+    const int nty = (e->g.ny + ty - 1) / ty;
+    const int nxc = (e->g.nx + e->xc - 1) / e->xc;
+    return nty * nxc;
 }
 
-int upload_monitors(sjf_engine* e) {
-    if (!e->mon_dirty) return SJF_OK;
-    const int n = (int)e->mon.size();
-    std::vector<long> idx(8L * n);
-    std::vector<double> w(8L * n);
-    std::vector<int> comp(n);
-    for (int p = 0; p < n; ++p) {
-        comp[p] = e->mon[p].comp;
-        for (int q = 0; q < 8; ++q) {
-            idx[8 * p + q] = e->mon[p].idx[q];
-            w[8 * p + q] = e->mon[p].w[q];
-        }
-    }
-    dev_free((void*&)e->d_mon_idx);
-    dev_free((void*&)e->d_mon_w);
-    dev_free((void*&)e->d_mon_comp);
-    int rc = dev_alloc(e, (void**)&e->d_mon_idx, sizeof(long) * 8 * n);
-    if (!rc) rc = dev_alloc(e, (void**)&e->d_mon_w, sizeof(double) * 8 * n);
-    if (!rc) rc = dev_alloc(e, (void**)&e->d_mon_comp, sizeof(int) * n);
-    if (rc) return rc;
-    HIPCHK(e, hipMemcpyAsync(e->d_mon_idx, idx.data(), sizeof(long) * 8 * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(e->d_mon_w, w.data(), sizeof(double) * 8 * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(e->d_mon_comp, comp.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    e->mon_dirty = false;
-    return SJF_OK;
-}
-
-int ensure_sample_capacity(sjf_engine* e, long need) {
-    const int npt = (int)e->mon.size();
-    if (npt == 0 || need <= e->samp_cap) return SJF_OK;
-    long cap = std::max(need, 2 * e->samp_cap);
-    double* p = nullptr;
-    hipError_t st = hipMalloc((void**)&p, sizeof(double) * cap * npt);
-    if (st != hipSuccess) return fail(e, SJF_E_NOMEM, "sample buffer allocation failed");
-    if (e->d_samples && e->nsamp > 0)
-        HIPCHK(e, hipMemcpyAsync(p, e->d_samples, sizeof(double) * e->nsamp * npt,
-                                 hipMemcpyDeviceToDevice, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    dev_free((void*&)e->d_samples);
-    e->d_samples = p;
-    e->samp_cap = cap;
-    return SJF_OK;
+// Pole slots compiled into the step kernel: 0, 1, 3 or 4 (a material with 2 poles runs the
+// 3-slot variant; unused slots are predicated off).
+int np_variant_of(int maxp, bool uniform) {
+    if (uniform || maxp == 0) return 0;
+    if (maxp == 1) return 1;
+    if (maxp <= 3) return 3;
+    return 4;
 }
 
 template <typename T>
-sjf::FieldPtrs<T> field_ptrs(const sjf_engine* e, int buf) {
-    sjf::FieldPtrs<T> f;
-    for (int c = 0; c < 3; ++c) {
-        f.f[c] = (const T*)e->E[buf][c];
-        f.f[3 + c] = (const T*)e->H[buf][c];
-    }
-    return f;
-}
-
-template <typename T>
-sjf::StepArgs<T> step_args(const sjf_engine* e) {
+sjf::StepArgs<T> step_args(const sjf_engine* e, int ty) {
     sjf::StepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.nx = e->g.nx;
@@ -341,12 +248,13 @@ sjf::StepArgs<T> step_args(const sjf_engine* e) {
     a.px = e->g.periodic[0];
     a.py = e->g.periodic[1];
     a.pz = e->g.periodic[2];
-    a.nty = (e->g.ny + e->ty - 1) / e->ty;
+    a.nty = (e->g.ny + ty - 1) / ty;
     a.xc = e->xc;
     a.N = e->N;
     a.cdt = (T)(e->dt / e->g.dx);
-    a.F0 = (const T*)e->Fbuf[e->ping];
-    a.F1 = (T*)e->Fbuf[e->ping ^ 1];
+    a.F[0] = (T*)e->Fbuf[0];
+    a.F[1] = (T*)e->Fbuf[1];
+    a.d_step = e->d_step;
     a.uniform = e->uniform ? 1 : 0;
     a.nmat = e->nmat;
     a.matw = e->matw;
@@ -363,217 +271,537 @@ sjf::StepArgs<T> step_args(const sjf_engine* e) {
         a.dlo[d] = e->dlo[d];
         a.dhi[d] = e->dhi[d];
     }
-    // the H auxiliaries alternate between segment pairs 2-3 and 4-5 with the field sets
-    a.hr = e->ping == 0 ? 2 : 4;
-    a.hw = e->ping == 0 ? 4 : 2;
     a.coef = (const T*)e->coefbuf;
     a.dinv = (const T*)e->dinv;
     return a;
 }
 
-int block_threads(const sjf_engine* e) { return ((e->g.nz + 63) / 64) * 64; }
+// ======================================================================== launch sets =====
 
-int n_groups(const sjf_engine* e) {
-    const int nty = (e->g.ny + e->ty - 1) / e->ty;
-    const int nxc = (e->g.nx + e->xc - 1) / e->xc;
-    return nty * nxc;
+struct LaunchSet {
+    std::vector<sjf_engine*> eng;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int prec = 4, ty = 1, npv = 0, block = 64;
+    size_t lds = 0;
+    std::vector<long> versions;
+    bool built = false;
+    // step kernel tables
+    void* d_args = nullptr;
+    int* d_tile_first = nullptr;
+    int ntiles = 0;
+    // sources
+    void* d_src = nullptr;
+    int* d_blk_src = nullptr;
+    long* d_blk_first = nullptr;
+    long* d_blk_end = nullptr;
+    long* d_src_idx = nullptr;
+    double* d_src_coef = nullptr;
+    int nsrcblk = 0;
+    // monitors
+    void* d_mon = nullptr;
+    int nmon = 0;
+    // DFT
+    void* d_dft = nullptr;
+    int ndft = 0;
+    long dft_total = 0;
+    // step counters
+    long** d_counters = nullptr;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    long gK = 0;
+};
+
+void set_free_tables(LaunchSet* s) {
+    dev_free(s->d_args);
+    dev_free(s->d_tile_first);
+    dev_free(s->d_src);
+    dev_free(s->d_blk_src);
+    dev_free(s->d_blk_first);
+    dev_free(s->d_blk_end);
+    dev_free(s->d_src_idx);
+    dev_free(s->d_src_coef);
+    dev_free(s->d_mon);
+    dev_free(s->d_dft);
+    dev_free(s->d_counters);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    s->gK = 0;
+    s->built = false;
 }
 
-template <typename T>
-size_t step_lds(const sjf_engine* e) {
-    return sizeof(T) * 4 * e->ty * block_threads(e) + sizeof(sjf::MatLut<T>) * e->nmat;
-}
-
-template <typename T, int NP, int MAXT>
-int launch_step_np(sjf_engine* e, const sjf::StepArgs<T>& a, dim3 grid, dim3 block, size_t lds) {
-    switch (e->ty) {
-        case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, lds, e->stream, a); break;
-        case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, NP, MAXT>), grid, block, lds, e->stream, a); break;
-        case 4: hipLaunchKernelGGL((sjf::k_step<T, 4, NP, MAXT>), grid, block, lds, e->stream, a); break;
-        default: return fail(e, SJF_E_ARG, "tile rows must be 1, 2 or 4");
+template <typename V>
+int upload_vec(sjf_engine* e, hipStream_t st, void** dst, const std::vector<V>& v) {
+    const size_t bytes = sizeof(V) * std::max<size_t>(v.size(), 1);
+    hipError_t rc = hipMalloc(dst, bytes);
+    if (rc != hipSuccess) return fail(e, SJF_E_NOMEM, "launch table allocation failed");
+    if (!v.empty()) {
+        rc = hipMemcpyAsync(*dst, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, st);
+        if (rc == hipSuccess) rc = hipStreamSynchronize(st);
+        if (rc != hipSuccess) return fail(e, SJF_E_HIP, hipGetErrorString(rc));
     }
-    HIPCHK(e, hipGetLastError());
     return SJF_OK;
 }
 
-// Pole slots compiled into the kernel: 0, 1, 3 or 4 (a material with 2 poles runs the
-// 3-slot variant; unused slots are predicated off).
-int np_variant(const sjf_engine* e) {
-    if (e->uniform || e->maxp == 0) return 0;
-    if (e->maxp == 1) return 1;
-    if (e->maxp <= 3) return 3;
-    return 4;
+template <typename T>
+int build_set_t(LaunchSet* s) {
+    sjf_engine* e0 = s->eng[0];
+    set_free_tables(s);
+    // step kernel
+    std::vector<sjf::StepArgs<T>> args;
+    std::vector<int> first;
+    int maxp = 0, block = 64;
+    bool all_uniform = true;
+    int t = 0;
+    for (sjf_engine* e : s->eng) {
+        args.push_back(step_args<T>(e, s->ty));
+        first.push_back(t);
+        t += tiles_of(e, s->ty);
+        maxp = std::max(maxp, e->maxp);
+        all_uniform = all_uniform && e->uniform;
+        block = std::max(block, block_threads(e));
+    }
+    first.push_back(t);
+    s->ntiles = t;
+    s->block = block;
+    s->npv = np_variant_of(maxp, all_uniform);
+    int nmat = 1;
+    for (sjf_engine* e : s->eng) nmat = std::max(nmat, e->nmat);
+    s->lds = sizeof(T) * 4 * s->ty * block + sizeof(sjf::MatLut<T>) * nmat;
+    if (s->lds > 65536)
+        return fail(e0, SJF_E_UNSUPPORTED, "tile needs more than 64 KiB of LDS; lower the rows per thread");
+    int rc = upload_vec(e0, s->stream, &s->d_args, args);
+    if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_tile_first, first);
+    if (rc) return rc;
+
+    // sources: entries of every source of every engine, in blocks of 256 of one source
+    std::vector<sjf::SrcDev<T>> srcs;
+    std::vector<int> blk_src;
+    std::vector<long> blk_first, blk_end, idx;
+    std::vector<double> coef;
+    for (sjf_engine* e : s->eng) {
+        for (const Source& src : e->src) {
+            if (src.idx.empty()) continue;
+            sjf::SrcDev<T> d;
+            std::memset(&d, 0, sizeof(d));
+            d.p = src.dev;
+            d.p.part = e->g.part;
+            d.p.dt = e->dt;
+            for (int b = 0; b < 2; ++b) d.F[b] = (T*)e->Fbuf[b] + (size_t)src.comp * e->N;
+            d.d_step = e->d_step;
+            const int sid = (int)srcs.size();
+            srcs.push_back(d);
+            const long base = (long)idx.size();
+            for (size_t q = 0; q < src.idx.size(); ++q) {
+                idx.push_back(src.idx[q]);
+                coef.push_back(e->dt * src.wt[q] * inv_eps_at(e, src.idx[q], src.comp));
+            }
+            const long n = (long)src.idx.size();
+            for (long b = 0; b < n; b += 256) {
+                blk_src.push_back(sid);
+                blk_first.push_back(base + b);
+                blk_end.push_back(base + std::min(n, b + 256));
+            }
+        }
+    }
+    s->nsrcblk = (int)blk_src.size();
+    if (s->nsrcblk) {
+        rc = upload_vec(e0, s->stream, &s->d_src, srcs);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_src, blk_src);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_first, blk_first);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_end, blk_end);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_src_idx, idx);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_src_coef, coef);
+        if (rc) return rc;
+    }
+
+    // monitors
+    std::vector<sjf::MonDev<T>> mons;
+    for (sjf_engine* e : s->eng) {
+        for (size_t p = 0; p < e->mon.size(); ++p) {
+            sjf::MonDev<T> m;
+            std::memset(&m, 0, sizeof(m));
+            const int c = e->mon[p].comp;
+            for (int b = 0; b < 2; ++b) m.F[b] = (const T*)e->Fbuf[b] + (size_t)c * e->N;
+            m.d_step = e->d_step;
+            m.out = e->d_samples;
+            m.npt = (int)e->mon.size();
+            m.p = (int)p;
+            m.span = e->save_span;
+            for (int q = 0; q < 8; ++q) {
+                m.idx[q] = e->mon[p].idx[q];
+                m.w[q] = e->mon[p].w[q];
+            }
+            mons.push_back(m);
+        }
+    }
+    s->nmon = (int)mons.size();
+    if (s->nmon) {
+        rc = upload_vec(e0, s->stream, &s->d_mon, mons);
+        if (rc) return rc;
+    }
+
+    // DFT regions
+    std::vector<sjf::DftDev<T>> regs;
+    long tot = 0;
+    for (sjf_engine* e : s->eng) {
+        for (const Dft& d : e->dft) {
+            sjf::DftDev<T> r;
+            std::memset(&r, 0, sizeof(r));
+            for (int b = 0; b < 2; ++b) r.F[b] = (const T*)e->Fbuf[b];
+            r.d_step = e->d_step;
+            r.N = e->N;
+            r.nyz = (long)e->g.ny * e->g.nz;
+            r.nz = e->g.nz;
+            for (int q = 0; q < 3; ++q) {
+                r.lo[q] = d.lo[q];
+                r.ext[q] = d.hi[q] - d.lo[q];
+            }
+            r.ncell = d.ncell;
+            r.first = tot;
+            for (int q = 0; q < 6; ++q) r.comps[q] = d.comps[q];
+            r.ncomp = d.ncomp;
+            r.nf = (int)d.freqs.size();
+            r.freq = d.d_freq;
+            r.dt = e->dt;
+            r.acc = d.acc;
+            r.cmp = d.cmp;
+            regs.push_back(r);
+            tot += d.ncell;
+        }
+    }
+    s->ndft = (int)regs.size();
+    s->dft_total = tot;
+    if (s->ndft) {
+        rc = upload_vec(e0, s->stream, &s->d_dft, regs);
+        if (rc) return rc;
+    }
+
+    std::vector<long*> ctr;
+    for (sjf_engine* e : s->eng) ctr.push_back(e->d_step);
+    rc = upload_vec(e0, s->stream, (void**)&s->d_counters, ctr);
+    if (rc) return rc;
+
+    s->versions.clear();
+    for (sjf_engine* e : s->eng) s->versions.push_back(e->version);
+    s->built = true;
+    return SJF_OK;
+}
+
+int build_set(LaunchSet* s) { return s->prec == 4 ? build_set_t<float>(s) : build_set_t<double>(s); }
+
+bool set_stale(const LaunchSet* s) {
+    if (!s->built) return true;
+    for (size_t i = 0; i < s->eng.size(); ++i)
+        if (s->eng[i]->version != s->versions[i]) return true;
+    return false;
+}
+
+#define SETCHK(s, call)                                                                  \
+    do {                                                                                 \
+        hipError_t _st = (call);                                                         \
+        if (_st != hipSuccess) {                                                         \
+            (s)->eng[0]->err = std::string(#call) + ": " + hipGetErrorString(_st);       \
+            return SJF_E_HIP;                                                            \
+        }                                                                                \
+    } while (0)
+
+template <typename T, int NP, int MAXT>
+int launch_step_np(LaunchSet* s) {
+    const dim3 grid(s->ntiles), block(s->block);
+    const auto* args = (const sjf::StepArgs<T>*)s->d_args;
+    const int n = (int)s->eng.size();
+    switch (s->ty) {
+        case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
+        case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
+        case 4: hipLaunchKernelGGL((sjf::k_step<T, 4, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
+        default: return fail(s->eng[0], SJF_E_ARG, "tile rows must be 1, 2 or 4");
+    }
+    SETCHK(s, hipGetLastError());
+    return SJF_OK;
 }
 
 template <typename T, int MAXT>
-int launch_step_b(sjf_engine* e, const sjf::StepArgs<T>& a, dim3 grid, dim3 block, size_t lds) {
-    switch (np_variant(e)) {
-        case 0: return launch_step_np<T, 0, MAXT>(e, a, grid, block, lds);
-        case 1: return launch_step_np<T, 1, MAXT>(e, a, grid, block, lds);
-        case 3: return launch_step_np<T, 3, MAXT>(e, a, grid, block, lds);
-        default: return launch_step_np<T, 4, MAXT>(e, a, grid, block, lds);
+int launch_step_b(LaunchSet* s) {
+    switch (s->npv) {
+        case 0: return launch_step_np<T, 0, MAXT>(s);
+        case 1: return launch_step_np<T, 1, MAXT>(s);
+        case 3: return launch_step_np<T, 3, MAXT>(s);
+        default: return launch_step_np<T, 4, MAXT>(s);
     }
 }
 
 template <typename T>
-int launch_step_t(sjf_engine* e) {
-    const sjf::StepArgs<T> a = step_args<T>(e);
-    const int bt = block_threads(e);
-    const dim3 grid(n_groups(e)), block(bt);
-    const size_t lds = step_lds<T>(e);
-    if (lds > 65536) return fail(e, SJF_E_UNSUPPORTED, "tile needs more than 64 KiB of LDS; lower the rows per thread");
-    if (bt <= 256) return launch_step_b<T, 256>(e, a, grid, block, lds);
-    if (bt <= 512) return launch_step_b<T, 512>(e, a, grid, block, lds);
-    return launch_step_b<T, 1024>(e, a, grid, block, lds);
+int launch_step_t(LaunchSet* s) {
+    if (s->block <= 256) return launch_step_b<T, 256>(s);
+    if (s->block <= 512) return launch_step_b<T, 512>(s);
+    return launch_step_b<T, 1024>(s);
 }
 
-int launch_step(sjf_engine* e) { return e->prec == 4 ? launch_step_t<float>(e) : launch_step_t<double>(e); }
-
+// One time step of every engine in the set.  `sample` launches the monitor gather first.
 template <typename T>
-int launch_monitors_t(sjf_engine* e) {
+int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_b) {
+    if (sample && s->nmon) {
+        hipLaunchKernelGGL((sjf::k_monitor<T>), dim3((s->nmon + 255) / 256), dim3(256), 0, s->stream,
+                           (const sjf::MonDev<T>*)s->d_mon, s->nmon);
+        SETCHK(s, hipGetLastError());
+    }
+    if (ev_a) SETCHK(s, hipEventRecord(ev_a, s->stream));
+    int rc = launch_step_t<T>(s);
+    if (rc) return rc;
+    if (ev_b) SETCHK(s, hipEventRecord(ev_b, s->stream));
+    if (s->nsrcblk) {
+        hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream,
+                           (const sjf::SrcDev<T>*)s->d_src, s->d_blk_src, s->d_blk_first, s->d_blk_end,
+                           s->d_src_idx, s->d_src_coef);
+        SETCHK(s, hipGetLastError());
+    }
+    if (s->ndft) {
+        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256)), dim3(256), 0,
+                           s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total);
+        SETCHK(s, hipGetLastError());
+    }
+    const int n = (int)s->eng.size();
+    hipLaunchKernelGGL(sjf::k_tick, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->d_counters, n);
+    SETCHK(s, hipGetLastError());
+    return SJF_OK;
+}
+
+int launch_one_step(LaunchSet* s, bool sample, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
+    return s->prec == 4 ? launch_one_step_t<float>(s, sample, a, b) : launch_one_step_t<double>(s, sample, a, b);
+}
+
+bool samples_at(const LaunchSet* s, long step) {
+    for (const sjf_engine* e : s->eng)
+        if (!e->mon.empty() && step % e->save_span == 0) return true;
+    return false;
+}
+
+int sync_engine(sjf_engine* e);
+
+int ensure_sample_capacity(sjf_engine* e, long need) {
     const int npt = (int)e->mon.size();
-    hipLaunchKernelGGL((sjf::k_monitor<T>), dim3((npt + 255) / 256), dim3(256), 0, e->stream,
-                       field_ptrs<T>(e, e->ping), e->d_mon_idx, e->d_mon_w, e->d_mon_comp, npt,
-                       e->d_samples + e->nsamp * npt);
-    HIPCHK(e, hipGetLastError());
-    e->nsamp++;
+    if (npt == 0 || need <= e->samp_cap) return SJF_OK;
+    // the old samples may still be in flight on a batch stream
+    if (int rc = sync_engine(e)) return rc;
+    const long cap = std::max(need, 2 * e->samp_cap);
+    double* p = nullptr;
+    if (hipMalloc((void**)&p, sizeof(double) * cap * npt) != hipSuccess)
+        return fail(e, SJF_E_NOMEM, "sample buffer allocation failed");
+    if (e->d_samples && e->nsamp > 0)
+        HIPCHK(e, hipMemcpyAsync(p, e->d_samples, sizeof(double) * e->nsamp * npt, hipMemcpyDeviceToDevice,
+                                 e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    dev_free(e->d_samples);
+    e->d_samples = p;
+    e->samp_cap = cap;
+    e->version++;
     return SJF_OK;
 }
 
-template <typename T>
-int launch_sources_t(sjf_engine* e) {
-    // after the step: E lives in buffer ping^1; current J^{n+1/2}
-    const double t0 = e->step * e->dt, t1 = (e->step + 1) * e->dt;
-    for (auto& s : e->src) {
-        if (s.idx.empty()) continue;
-        double r0, i0, r1, i1;
-        dipole(s, t0, &r0, &i0);
-        dipole(s, t1, &r1, &i1);
-        const double jr = (r1 - r0) / e->dt, ji = (i1 - i0) / e->dt;
-        const double ar = s.amp_re * jr - s.amp_im * ji;
-        const double ai = s.amp_re * ji + s.amp_im * jr;
-        const double J = e->g.part ? ai : ar;
-        if (J == 0.0) continue;
-        const long n = (long)s.idx.size();
-        hipLaunchKernelGGL((sjf::k_source<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           e->stream, (T*)e->E[e->ping ^ 1][s.comp], s.d_idx, s.d_coef, n, J);
-        HIPCHK(e, hipGetLastError());
+long samples_in(long a, long b, int span) {
+    // multiples of span in [a, b)
+    if (b <= a) return 0;
+    const long first = (a + span - 1) / span, last = (b - 1) / span;
+    return std::max(0L, last - first + 1);
+}
+
+int prepare_set(LaunchSet* s, long n) {
+    for (sjf_engine* e : s->eng) {
+        if (e->mon.empty()) continue;
+        const long need = (e->step + n + e->save_span - 1) / e->save_span;
+        int rc = ensure_sample_capacity(e, need);
+        if (rc) return rc;
     }
+    if (set_stale(s)) return build_set(s);
     return SJF_OK;
 }
 
-template <typename T>
-int launch_dft_t(sjf_engine* e) {
-    for (auto& d : e->dft) {
-        sjf::DftArgs<T> a;
-        std::memset(&a, 0, sizeof(a));
-        a.F = field_ptrs<T>(e, e->ping ^ 1);
-        for (int q = 0; q < 3; ++q) {
-            a.lo[q] = d.lo[q];
-            a.ext[q] = d.hi[q] - d.lo[q];
+void advance_hosts(LaunchSet* s, long n) {
+    for (sjf_engine* e : s->eng) {
+        if (!e->mon.empty()) e->nsamp += samples_in(e->step, e->step + n, e->save_span);
+        e->step += n;
+    }
+}
+
+long set_span(const LaunchSet* s) {
+    long span = 1;
+    for (const sjf_engine* e : s->eng)
+        if (!e->mon.empty()) span = e->save_span;
+    return span;
+}
+
+// Steps per captured graph: a multiple of the save span so every replay starts aligned.
+long graph_chunk(const LaunchSet* s) {
+    const sjf_engine* e0 = s->eng[0];
+    if (e0->graph_chunk <= 0) return 0;
+    const long span = set_span(s);
+    return std::max(1L, (e0->graph_chunk + span - 1) / span) * span;
+}
+
+int capture_graph(LaunchSet* s, long K) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    const long s0 = s->eng[0]->step;
+    SETCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = SJF_OK;
+    for (long q = 0; q < K && !rc; ++q) rc = launch_one_step(s, samples_at(s, s0 + q));
+    hipGraph_t g = nullptr;
+    const hipError_t st = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    SETCHK(s, st);
+    s->graph = g;
+    SETCHK(s, hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    s->gK = K;
+    return SJF_OK;
+}
+
+int run_set(LaunchSet* s, long n) {
+    int rc = prepare_set(s, n);
+    if (rc) return rc;
+    const long K = graph_chunk(s);
+    const long span = set_span(s);
+    long done = 0;
+    auto eager = [&](long m) -> int {
+        for (long q = 0; q < m; ++q) {
+            const int r = launch_one_step(s, samples_at(s, s->eng[0]->step));
+            if (r) return r;
+            advance_hosts(s, 1);
         }
-        a.ncell = d.ncell;
-        a.ncomp = d.ncomp;
-        for (int s = 0; s < 6; ++s) a.comps[s] = d.comps[s];
-        a.nf = (int)d.freqs.size();
-        a.acc = d.acc;
-        a.cmp = d.cmp;
-        const double tE = (e->step + 1) * e->dt, tH = (e->step + 0.5) * e->dt;
-        for (int f = 0; f < a.nf; ++f) {
-            const double w = 2 * M_PI * d.freqs[f];
-            a.ph[0][f] = make_double2(std::cos(w * tE) * e->dt, std::sin(w * tE) * e->dt);
-            a.ph[1][f] = make_double2(std::cos(w * tH) * e->dt, std::sin(w * tH) * e->dt);
+        return SJF_OK;
+    };
+    if (K > 0 && n >= 2 * K) {
+        // align the first replay to the save span, then replay whole chunks
+        const long lead = (span - s->eng[0]->step % span) % span;
+        rc = eager(lead);
+        if (rc) return rc;
+        done += lead;
+        if (!s->gexec || s->gK != K) {
+            rc = capture_graph(s, K);
+            if (rc) return rc;
         }
-        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((d.ncell + 255) / 256)), dim3(256), 0,
-                           e->stream, a, (long)e->g.ny * e->g.nz, (long)e->g.nz);
-        HIPCHK(e, hipGetLastError());
+        while (n - done >= K) {
+            SETCHK(s, hipGraphLaunch(s->gexec, s->stream));
+            advance_hosts(s, K);
+            done += K;
+        }
     }
+    return eager(n - done);
+}
+
+LaunchSet* own_set(sjf_engine* e) {
+    if (!e->own) {
+        e->own = new LaunchSet();
+        e->own->eng.push_back(e);
+        e->own->stream = e->stream;
+        e->own->prec = e->prec;
+    }
+    if (e->own->ty != e->ty) {
+        e->own->ty = e->ty;
+        e->own->built = false;
+    }
+    return e->own;
+}
+
+void destroy_set(LaunchSet* s) {
+    if (!s) return;
+    set_free_tables(s);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int timed_set(LaunchSet* s, long n, double* ms_total, double* ms_kernel_mean) {
+    int rc = prepare_set(s, n);
+    if (rc) return rc;
+    std::vector<hipEvent_t> evs(2 * n);
+    for (auto& ev : evs) SETCHK(s, hipEventCreate(&ev));
+    hipEvent_t t0, t1;
+    SETCHK(s, hipEventCreate(&t0));
+    SETCHK(s, hipEventCreate(&t1));
+    SETCHK(s, hipStreamSynchronize(s->stream));
+    SETCHK(s, hipEventRecord(t0, s->stream));
+    for (long q = 0; q < n; ++q) {
+        rc = launch_one_step(s, samples_at(s, s->eng[0]->step), evs[2 * q], evs[2 * q + 1]);
+        if (rc) return rc;
+        advance_hosts(s, 1);
+    }
+    SETCHK(s, hipEventRecord(t1, s->stream));
+    SETCHK(s, hipEventSynchronize(t1));
+    float tot = 0;
+    SETCHK(s, hipEventElapsedTime(&tot, t0, t1));
+    double ksum = 0;
+    for (long q = 0; q < n; ++q) {
+        float ms = 0;
+        SETCHK(s, hipEventElapsedTime(&ms, evs[2 * q], evs[2 * q + 1]));
+        ksum += ms;
+    }
+    for (auto& ev : evs) (void)hipEventDestroy(ev);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    *ms_total = tot;
+    *ms_kernel_mean = ksum / n;
     return SJF_OK;
 }
 
-template <typename T>
-int one_step_t(sjf_engine* e, hipEvent_t ev_a, hipEvent_t ev_b) {
-    int rc;
-    if (!e->mon.empty() && e->step % e->save_span == 0) {
-        rc = launch_monitors_t<T>(e);
-        if (rc) return rc;
-    }
-    if (ev_a) HIPCHK(e, hipEventRecord(ev_a, e->stream));
-    rc = launch_step_t<T>(e);
-    if (rc) return rc;
-    if (ev_b) HIPCHK(e, hipEventRecord(ev_b, e->stream));
-    rc = launch_sources_t<T>(e);
-    if (rc) return rc;
-    rc = launch_dft_t<T>(e);
-    if (rc) return rc;
-    e->ping ^= 1;
-    e->step++;
-    return SJF_OK;
-}
-
-int prepare(sjf_engine* e, long n) {
-    int rc = upload_sources(e);
-    if (rc) return rc;
-    rc = upload_monitors(e);
-    if (rc) return rc;
-    if (!e->mon.empty()) {
-        // samples taken by steps [step, step+n): multiples of save_span in that range
-        const long a = e->step, b = e->step + n;
-        const long first = (a + e->save_span - 1) / e->save_span;
-        const long last = (b - 1) / e->save_span;
-        const long cnt = n > 0 ? std::max(0L, last - first + 1) : 0;
-        rc = ensure_sample_capacity(e, e->nsamp + cnt);
-        if (rc) return rc;
-    }
-    return SJF_OK;
-}
-
-int run_steps(sjf_engine* e, long n, std::vector<hipEvent_t>* evs) {
-    int rc = prepare(e, n);
-    if (rc) return rc;
-    for (long s = 0; s < n; ++s) {
-        hipEvent_t a = evs ? (*evs)[2 * s] : nullptr;
-        hipEvent_t b = evs ? (*evs)[2 * s + 1] : nullptr;
-        rc = e->prec == 4 ? one_step_t<float>(e, a, b) : one_step_t<double>(e, a, b);
-        if (rc) return rc;
-    }
-    return SJF_OK;
-}
+// ======================================================================== engine helpers ==
 
 void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
-    // Defaults measured on MI355X: one or two rows per thread and short x-chunks (many
-    // workgroups) beat deep register blocking; the autotuner refines per workload.
+    // Defaults measured on MI355X: one row per thread and short x-chunks (many workgroups)
+    // beat deeper register blocking; the autotuner refines per workload.
     e->ty = 1;
-    const int nty = (e->g.ny + e->ty - 1) / e->ty;
+    const int nty = e->g.ny;
     const long target_groups = 16384;
     long nxc = std::max(1L, (target_groups + nty - 1) / nty);
     nxc = std::min<long>(nxc, e->g.nx);
-    int xc = (int)((e->g.nx + nxc - 1) / nxc);
+    const int xc = (int)((e->g.nx + nxc - 1) / nxc);
     e->xc = std::max(std::min(4, e->g.nx), std::min(xc, e->g.nx));
 }
 
 int free_all(sjf_engine* e) {
+    destroy_set(e->own);
+    e->own = nullptr;
     for (int b = 0; b < 2; ++b) dev_free(e->Fbuf[b]);
-    dev_free((void*&)e->matw);
+    dev_free(e->d_step);
+    dev_free(e->matw);
     dev_free(e->lut);
     dev_free(e->dinv);
     dev_free(e->Qbuf);
     dev_free(e->Pbuf);
     dev_free(e->coefbuf);
     for (int d = 0; d < 3; ++d) dev_free(e->psibuf[d]);
-    for (auto& s : e->src) {
-        dev_free((void*&)s.d_idx);
-        dev_free((void*&)s.d_coef);
-    }
-    dev_free((void*&)e->d_mon_idx);
-    dev_free((void*&)e->d_mon_w);
-    dev_free((void*&)e->d_mon_comp);
-    dev_free((void*&)e->d_samples);
+    dev_free(e->d_samples);
     for (auto& d : e->dft) {
-        dev_free((void*&)d.acc);
-        dev_free((void*&)d.cmp);
+        dev_free(d.acc);
+        dev_free(d.cmp);
+        dev_free(d.d_freq);
     }
+    return SJF_OK;
+}
+
+template <typename T>
+int convert_to_device(sjf_engine* e, void* dst, const double* src, long n) {
+    if (sizeof(T) == sizeof(double)) return h2d(e, dst, src, sizeof(double) * n);
+    std::vector<T> tmp(n);
+    for (long q = 0; q < n; ++q) tmp[q] = (T)src[q];
+    return h2d(e, dst, tmp.data(), sizeof(T) * n);
+}
+
+template <typename T>
+int convert_from_device(sjf_engine* e, double* dst, const void* src, long n) {
+    if (sizeof(T) == sizeof(double)) return d2h(e, dst, src, sizeof(double) * n);
+    std::vector<T> tmp(n);
+    int rc = d2h(e, tmp.data(), src, sizeof(T) * n);
+    if (rc) return rc;
+    for (long q = 0; q < n; ++q) dst[q] = (double)tmp[q];
     return SJF_OK;
 }
 
@@ -602,38 +830,10 @@ int upload_lut_t(sjf_engine* e) {
     dev_free(e->lut);
     int rc = dev_alloc(e, &e->lut, sizeof(sjf::MatLut<T>) * e->nmat);
     if (rc) return rc;
-    HIPCHK(e, hipMemcpyAsync(e->lut, l.data(), sizeof(sjf::MatLut<T>) * e->nmat,
-                             hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    return SJF_OK;
+    return h2d(e, e->lut, l.data(), sizeof(sjf::MatLut<T>) * e->nmat);
 }
 
 int upload_lut(sjf_engine* e) { return e->prec == 4 ? upload_lut_t<float>(e) : upload_lut_t<double>(e); }
-
-template <typename T>
-int convert_to_device(sjf_engine* e, void* dst, const double* src, long n) {
-    if (sizeof(T) == sizeof(double)) {
-        return h2d(e, dst, src, sizeof(double) * n);
-    } else {
-        std::vector<T> tmp(n);
-        for (long q = 0; q < n; ++q) tmp[q] = (T)src[q];
-        return h2d(e, dst, tmp.data(), sizeof(T) * n);
-    }
-    return SJF_OK;
-}
-
-template <typename T>
-int convert_from_device(sjf_engine* e, double* dst, const void* src, long n) {
-    if (sizeof(T) == sizeof(double)) {
-        return d2h(e, dst, src, sizeof(double) * n);
-    } else {
-        std::vector<T> tmp(n);
-        int rc = d2h(e, tmp.data(), src, sizeof(T) * n);
-        if (rc) return rc;
-        for (long q = 0; q < n; ++q) dst[q] = (double)tmp[q];
-    }
-    return SJF_OK;
-}
 
 void trilinear(const sjf_engine* e, int comp, const double* xyz, long* idx, double* w) {
     int i0[3];
@@ -661,10 +861,48 @@ void trilinear(const sjf_engine* e, int comp, const double* xyz, long* idx, doub
             }
 }
 
+// current buffer of a component (the set read by the next step)
 void* field_buf(sjf_engine* e, int comp) {
-    return comp < 3 ? e->E[e->ping][comp] : e->H[e->ping][comp - 3];
+    return (char*)e->Fbuf[e->step & 1] + (size_t)e->prec * comp * e->N;
 }
 
+double engine_step_bytes(const sjf_engine* e) {
+    const double p = e->prec;
+    double b = e->N * (12 * p + (e->uniform ? 0 : 4));
+    // polarisation: Drude pole = Q read+write, Lorentz pole = P and Q read+write
+    if (e->have_mat && e->maxp > 0) {
+        for (long q = 0; q < e->N; ++q)
+            for (int c = 0; c < 3; ++c) {
+                const int m = (e->matw_h[q] >> (8 * c)) & 0xff;
+                if (m == sjf::kDesignMat) continue;
+                const sjf_material& M = e->lut_h[m];
+                for (int s = 0; s < M.npoles; ++s) b += (M.is_drude[s] ? 2 : 4) * p;
+            }
+    }
+    if (!e->deps_h.empty()) b += p * (double)e->deps_h.size();
+    // PML: per cell and PML axis, 2 E + 2 H auxiliaries, each read + written
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    for (int d = 0; d < 3; ++d)
+        if (e->pml_on[d]) b += 8 * p * (double)(e->pml[2 * d] + e->pml[2 * d + 1]) * (e->N / n[d]);
+    return b;
+}
+
+}  // namespace
+
+// ============================================================================ batches ====
+
+struct sjf_batch {
+    LaunchSet* set = nullptr;
+    std::string err;
+};
+
+namespace {
+int sync_engine(sjf_engine* e) {
+    HIPCHK(e, hipSetDevice(e->g.device));
+    if (e->batch) HIPCHK(e, hipStreamSynchronize(e->batch->set->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SJF_OK;
+}
 }  // namespace
 
 // =========================================================================== C ABI ====
@@ -688,6 +926,10 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
     if (g->precision != SJF_FP32 && g->precision != SJF_FP64) {
         g_create_error = "precision must be SJF_FP32 (4) or SJF_FP64 (8)";
         return SJF_E_ARG;
+    }
+    if ((double)g->nx * g->ny * g->nz * 6 >= 4294967296.0) {
+        g_create_error = "grid too large for one engine (6 * cells must stay below 2^32)";
+        return SJF_E_UNSUPPORTED;
     }
     int ndev = 0;
     hipError_t st = hipGetDeviceCount(&ndev);
@@ -715,19 +957,18 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
         delete e;
         return SJF_E_HIP;
     }
+    auto bail = [&](int rc) {
+        g_create_error = e->err;
+        free_all(e);
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return rc;
+    };
     for (int b = 0; b < 2; ++b) {
-        if (dev_alloc(e, &e->Fbuf[b], (size_t)e->prec * 6 * e->N)) {
-            g_create_error = e->err;
-            free_all(e);
-            hipStreamDestroy(e->stream);
-            delete e;
-            return SJF_E_NOMEM;
-        }
-        for (int c = 0; c < 3; ++c) {
-            e->E[b][c] = (char*)e->Fbuf[b] + (size_t)e->prec * c * e->N;
-            e->H[b][c] = (char*)e->Fbuf[b] + (size_t)e->prec * (3 + c) * e->N;
-        }
+        int rc = dev_alloc(e, &e->Fbuf[b], (size_t)e->prec * 6 * e->N);
+        if (rc) return bail(rc);
     }
+    if (int rc = dev_alloc(e, (void**)&e->d_step, sizeof(long))) return bail(rc);
     // PML tables for all axes (b = 1, a = 0 until sjf_set_pml fills an axis)
     {
         const int n3[3] = {g->nx, g->ny, g->nz};
@@ -745,58 +986,44 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
         int rc = dev_alloc(e, &e->coefbuf, (size_t)e->prec * off);
         if (!rc) rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
                                    : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
-        if (rc) {
-            g_create_error = e->err;
-            free_all(e);
-            hipStreamDestroy(e->stream);
-            delete e;
-            return rc;
-        }
+        if (rc) return bail(rc);
     }
     // default material: vacuum
     e->lut_h.assign(1, sjf_material{});
     e->lut_h[0].eps_inf = 1.0;
     e->nmat = 1;
-    if (upload_lut(e)) {
-        g_create_error = e->err;
-        free_all(e);
-        hipStreamDestroy(e->stream);
-        delete e;
-        return SJF_E_HIP;
-    }
+    if (int rc = upload_lut(e)) return bail(rc);
     choose_tile(e);
-    hipStreamSynchronize(e->stream);
     *out = e;
     return SJF_OK;
 }
 
 int sjf_destroy(sjf_engine* e) {
     if (!e) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    hipStreamSynchronize(e->stream);
+    if (e->batch) return fail(e, SJF_E_STATE, "destroy the batch before its engines");
+    (void)hipSetDevice(e->g.device);
+    (void)hipStreamSynchronize(e->stream);
     free_all(e);
-    hipStreamDestroy(e->stream);
+    (void)hipStreamDestroy(e->stream);
     delete e;
     return SJF_OK;
 }
 
-const char* sjf_last_error(const sjf_engine* e) {
-    return e ? e->err.c_str() : g_create_error.c_str();
-}
+const char* sjf_last_error(const sjf_engine* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 
 int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_material* lut, int n_mat) {
     if (!e || !mat_word || !lut) return SJF_E_ARG;
     if (n_mat < 1 || n_mat > SJF_MAX_MATERIALS) return fail(e, SJF_E_ARG, "n_mat must be in [1, 255]");
-    hipSetDevice(e->g.device);
+    if (int rc = sync_engine(e)) return rc;
     int maxp = 0;
-    bool lor[kMaxPoles] = {};
+    bool anylor = false;
     for (int m = 0; m < n_mat; ++m) {
         if (lut[m].npoles < 0 || lut[m].npoles > SJF_MAX_POLES)
             return fail(e, SJF_E_ARG, "npoles must be in [0, 4]");
         if (!(lut[m].eps_inf > 0)) return fail(e, SJF_E_ARG, "eps_inf must be positive");
         maxp = std::max(maxp, lut[m].npoles);
         for (int s = 0; s < lut[m].npoles; ++s)
-            if (!lut[m].is_drude[s]) lor[s] = true;
+            if (!lut[m].is_drude[s]) anylor = true;
     }
     for (long q = 0; q < e->N; ++q)
         for (int c = 0; c < 3; ++c) {
@@ -813,24 +1040,19 @@ int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_materia
     e->maxp = maxp;
     e->uniform = (n_mat == 1 && maxp == 0 && e->deps_h.empty());
     int rc = dev_alloc(e, (void**)&e->matw, sizeof(uint32_t) * e->N);
+    if (!rc) rc = h2d(e, e->matw, mat_word, sizeof(uint32_t) * e->N);
     if (rc) return rc;
-    rc = h2d(e, e->matw, mat_word, sizeof(uint32_t) * e->N);
-    if (rc) return rc;
-    // polarisation state: dense per-slot arrays, zeroed
+    // polarisation state, zeroed
     dev_free(e->Qbuf);
     dev_free(e->Pbuf);
     if (maxp > 0) {
-        bool anylor = false;
-        for (int s = 0; s < maxp; ++s) anylor |= lor[s];
         rc = dev_alloc(e, &e->Qbuf, (size_t)e->prec * 3 * maxp * e->N);
         if (!rc && anylor) rc = dev_alloc(e, &e->Pbuf, (size_t)e->prec * 3 * maxp * e->N);
         if (rc) return rc;
     }
-    for (int s = 0; s < kMaxPoles; ++s) e->lorentz_slot[s] = lor[s];
     rc = upload_lut(e);
     if (rc) return rc;
-    e->src_dirty = true;
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->version++;
     return SJF_OK;
 }
 
@@ -842,14 +1064,13 @@ int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const do
         if (lo[d] < 0 || hi[d] > n[d] || hi[d] <= lo[d]) return fail(e, SJF_E_ARG, "design box outside grid");
         cnt *= hi[d] - lo[d];
     }
-    if (!e->deps_h.empty()) {
+    if (!e->deps_h.empty())
         for (int d = 0; d < 3; ++d)
             if (lo[d] != e->dlo[d] || hi[d] != e->dhi[d])
                 return fail(e, SJF_E_STATE, "design box cannot change once set");
-    }
     for (long q = 0; q < cnt; ++q)
         if (!(eps[q] > 0)) return fail(e, SJF_E_ARG, "design eps must be positive");
-    hipSetDevice(e->g.device);
+    if (int rc = sync_engine(e)) return rc;
     for (int d = 0; d < 3; ++d) {
         e->dlo[d] = lo[d];
         e->dhi[d] = hi[d];
@@ -875,7 +1096,7 @@ int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const do
     rc = h2d(e, e->matw, e->matw_h.data(), sizeof(uint32_t) * e->N);
     if (rc) return rc;
     e->uniform = false;
-    e->src_dirty = true;
+    e->version++;
     return SJF_OK;
 }
 
@@ -883,18 +1104,16 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
     if (!e || !cells) return SJF_E_ARG;
     if (!(R > 0 && R < 1) || !(power >= 0)) return fail(e, SJF_E_ARG, "R must be in (0,1), power >= 0");
     if (e->step != 0) return fail(e, SJF_E_STATE, "PML must be set before the first step");
-    hipSetDevice(e->g.device);
     for (int d = 0; d < 3; ++d) {
         const int n = axis_n(e, d);
         const int lo = cells[2 * d], hi = cells[2 * d + 1];
         if (lo < 0 || hi < 0 || lo + hi > n) return fail(e, SJF_E_ARG, "PML thicker than the grid");
-        if ((lo > 0 || hi > 0) && e->g.periodic[d])
-            return fail(e, SJF_E_ARG, "PML on a periodic axis");
+        if ((lo > 0 || hi > 0) && e->g.periodic[d]) return fail(e, SJF_E_ARG, "PML on a periodic axis");
     }
-    std::vector<double> tab;
+    if (int rc = sync_engine(e)) return rc;
     int off = 0;
     for (int d = 0; d < 3; ++d) off += 4 * axis_n(e, d);
-    tab.assign(off, 0.0);
+    std::vector<double> tab(off, 0.0);
     for (int d = 0; d < 3; ++d) {
         const int n = axis_n(e, d);
         const int lo = cells[2 * d], hi = cells[2 * d + 1];
@@ -904,6 +1123,7 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         dev_free(e->psibuf[d]);
         e->slab[d] = 0;
         const double dx = e->g.dx;
+        // sigma(u) = sigma_max (u/d)^p, sigma_max = -(p+1) ln R / (2 d): meep::pml(d) defaults
         auto sigma = [&](double x) {
             double s = 0;
             if (lo > 0) {
@@ -929,12 +1149,10 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         int rc = dev_alloc(e, &e->psibuf[d], (size_t)e->prec * 6 * e->slab[d]);
         if (rc) return rc;
     }
-    {
-        int rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
-                              : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
-        if (rc) return rc;
-    }
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    int rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
+                          : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
+    if (rc) return rc;
+    e->version++;
     return SJF_OK;
 }
 
@@ -948,21 +1166,30 @@ int sjf_add_source(sjf_engine* e, int comp, const double box[6], const sjf_pulse
     Source s;
     s.comp = comp;
     s.p = *p;
-    s.amp_re = amp_re;
-    s.amp_im = amp_im;
+    std::memset(&s.dev, 0, sizeof(s.dev));
+    s.dev.kind = p->kind;
+    s.dev.amp_re = amp_re;
+    s.dev.amp_im = amp_im;
     if (p->kind == SJF_SRC_GAUSSIAN) {
-        s.omega = 2 * M_PI * p->freq;
-        s.phi = p->phase + M_PI;
-        s.peak = 0.5 * (p->t_start + p->t_end);
-        s.cutoff = (p->t_end - p->t_start) * 0.5;
-        while (std::exp(-s.cutoff * s.cutoff / (2 * p->width * p->width)) < 1e-100) s.cutoff *= 0.9;
-        s.cutoff = (double)(float)s.cutoff;
+        // gaussian_src_time_phase constructor (reference src/disp.cpp:378-393)
+        s.dev.omega = 2 * M_PI * p->freq;
+        s.dev.width = p->width;
+        s.dev.phi = p->phase + M_PI;
+        s.dev.peak = 0.5 * (p->t_start + p->t_end);
+        double cut = (p->t_end - p->t_start) * 0.5;
+        while (std::exp(-cut * cut / (2 * p->width * p->width)) < 1e-100) cut *= 0.9;
+        s.dev.cutoff = (double)(float)cut;
+    } else {
+        s.dev.freq = p->freq;
+        s.dev.cw_width = p->width;
+        s.dev.t_start = p->t_start;
+        s.dev.t_end = p->t_end;
+        s.dev.slow = p->slowness;
     }
     std::vector<int> ids[3];
     std::vector<double> ws[3];
     for (int d = 0; d < 3; ++d)
-        axis_weights(axis_n(e, d), e->g.dx, kCompOff[comp][d], e->g.periodic[d], box[d], box[3 + d],
-                     ids[d], ws[d]);
+        axis_weights(axis_n(e, d), e->g.dx, kCompOff[comp][d], e->g.periodic[d], box[d], box[3 + d], ids[d], ws[d]);
     for (size_t a = 0; a < ids[0].size(); ++a)
         for (size_t b = 0; b < ids[1].size(); ++b)
             for (size_t c = 0; c < ids[2].size(); ++c) {
@@ -970,7 +1197,7 @@ int sjf_add_source(sjf_engine* e, int comp, const double box[6], const sjf_pulse
                 s.wt.push_back(ws[0][a] * ws[1][b] * ws[2][c]);
             }
     e->src.push_back(std::move(s));
-    e->src_dirty = true;
+    e->version++;
     return SJF_OK;
 }
 
@@ -978,7 +1205,7 @@ int sjf_last_source_time(const sjf_engine* e, double* t) {
     if (!e || !t) return SJF_E_ARG;
     double m = 0;
     for (const auto& s : e->src) {
-        const double lt = s.p.kind == SJF_SRC_GAUSSIAN ? (double)(float)(s.peak + s.cutoff) : s.p.t_end;
+        const double lt = s.p.kind == SJF_SRC_GAUSSIAN ? (double)(float)(s.dev.peak + s.dev.cutoff) : s.p.t_end;
         m = std::max(m, lt);
     }
     *t = m;
@@ -1013,13 +1240,18 @@ int sjf_add_monitors(sjf_engine* e, int comp, const double* xyz, int n) {
         trilinear(e, comp, xyz + 3 * p, m.idx, m.w);
         e->mon.push_back(m);
     }
-    e->mon_dirty = true;
+    dev_free(e->d_samples);
+    e->samp_cap = 0;
+    e->version++;
     return SJF_OK;
 }
 
 int sjf_set_save_span(sjf_engine* e, int span) {
     if (!e) return SJF_E_ARG;
-    e->save_span = span > 0 ? span : 1;
+    const int s = span > 0 ? span : 1;
+    if (e->nsamp > 0 && s != e->save_span) return fail(e, SJF_E_STATE, "save span cannot change once sampling started");
+    e->save_span = s;
+    e->version++;
     return SJF_OK;
 }
 
@@ -1030,8 +1262,7 @@ int sjf_read_monitors(sjf_engine* e, double* out, size_t cap, long* n_samples) {
     const long tot = e->nsamp * npt;
     if (!out || tot == 0) return SJF_OK;
     if ((size_t)(2 * tot) > cap) return fail(e, SJF_E_ARG, "output buffer too small");
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (int rc = sync_engine(e)) return rc;
     std::vector<double> tmp(tot);
     int rc = d2h(e, tmp.data(), e->d_samples, sizeof(double) * tot);
     if (rc) return rc;
@@ -1044,8 +1275,7 @@ int sjf_read_monitors(sjf_engine* e, double* out, size_t cap, long* n_samples) {
 
 int sjf_get_field(sjf_engine* e, int comp, const double xyz[3], double out[2]) {
     if (!e || !xyz || !out || comp < 0 || comp > 5) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (int rc = sync_engine(e)) return rc;
     long idx[8];
     double w[8];
     trilinear(e, comp, xyz, idx, w);
@@ -1070,8 +1300,8 @@ int sjf_get_field(sjf_engine* e, int comp, const double xyz[3], double out[2]) {
     return SJF_OK;
 }
 
-int sjf_add_dft(sjf_engine* e, const int lo[3], const int hi[3], int comp_mask, const double* freqs,
-                int nf, int* id) {
+int sjf_add_dft(sjf_engine* e, const int lo[3], const int hi[3], int comp_mask, const double* freqs, int nf,
+                int* id) {
     if (!e || !lo || !hi || !freqs || !id) return SJF_E_ARG;
     if (nf < 1 || nf > SJF_MAX_DFT_FREQS) return fail(e, SJF_E_ARG, "nf must be in [1, 64]");
     if (comp_mask <= 0 || comp_mask >= 64) return fail(e, SJF_E_ARG, "comp_mask must select components 0..5");
@@ -1089,14 +1319,16 @@ int sjf_add_dft(sjf_engine* e, const int lo[3], const int hi[3], int comp_mask, 
     d.ncomp = 0;
     for (int c = 0; c < 6; ++c)
         if (comp_mask >> c & 1) d.comps[d.ncomp++] = c;
-    hipSetDevice(e->g.device);
+    if (int rc = sync_engine(e)) return rc;
     const size_t bytes = sizeof(double2) * d.ncomp * nf * d.ncell;
     int rc = dev_alloc(e, (void**)&d.acc, bytes);
     if (!rc) rc = dev_alloc(e, (void**)&d.cmp, bytes);
+    if (!rc) rc = dev_alloc(e, (void**)&d.d_freq, sizeof(double) * nf);
+    if (!rc) rc = h2d(e, d.d_freq, d.freqs.data(), sizeof(double) * nf);
     if (rc) return rc;
-    HIPCHK(e, hipStreamSynchronize(e->stream));
     e->dft.push_back(d);
     *id = (int)e->dft.size() - 1;
+    e->version++;
     return SJF_OK;
 }
 
@@ -1105,8 +1337,7 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap) {
     const Dft& d = e->dft[id];
     const long n = (long)d.ncomp * (long)d.freqs.size() * d.ncell;
     if ((size_t)(2 * n) > cap) return fail(e, SJF_E_ARG, "output buffer too small");
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (int rc = sync_engine(e)) return rc;
     std::vector<double2> a(n), c(n);
     int rc = d2h(e, a.data(), d.acc, sizeof(double2) * n);
     if (!rc) rc = d2h(e, c.data(), d.cmp, sizeof(double2) * n);
@@ -1120,87 +1351,47 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap) {
 
 int sjf_step(sjf_engine* e, long n) {
     if (!e || n < 0) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    return run_steps(e, n, nullptr);
+    if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
+    HIPCHK(e, hipSetDevice(e->g.device));
+    return run_set(own_set(e), n);
 }
 
 int sjf_synchronize(sjf_engine* e) {
     if (!e) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return sync_engine(e);
+}
+
+int sjf_set_graph(sjf_engine* e, int chunk) {
+    if (!e || chunk < 0) return SJF_E_ARG;
+    e->graph_chunk = chunk;
+    e->version++;
     return SJF_OK;
 }
 
 int sjf_timed_steps(sjf_engine* e, long n, double* ms_total, double* ms_kernel_mean) {
     if (!e || n < 1 || !ms_total || !ms_kernel_mean) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    int rc = prepare(e, n);
-    if (rc) return rc;
-    std::vector<hipEvent_t> evs(2 * n);
-    for (auto& ev : evs) HIPCHK(e, hipEventCreate(&ev));
-    hipEvent_t t0, t1;
-    HIPCHK(e, hipEventCreate(&t0));
-    HIPCHK(e, hipEventCreate(&t1));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    HIPCHK(e, hipEventRecord(t0, e->stream));
-    rc = run_steps(e, n, &evs);
-    if (rc) return rc;
-    HIPCHK(e, hipEventRecord(t1, e->stream));
-    HIPCHK(e, hipEventSynchronize(t1));
-    float tot = 0;
-    HIPCHK(e, hipEventElapsedTime(&tot, t0, t1));
-    double ksum = 0;
-    for (long s = 0; s < n; ++s) {
-        float ms = 0;
-        HIPCHK(e, hipEventElapsedTime(&ms, evs[2 * s], evs[2 * s + 1]));
-        ksum += ms;
-    }
-    for (auto& ev : evs) hipEventDestroy(ev);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    *ms_total = tot;
-    *ms_kernel_mean = ksum / n;
-    return SJF_OK;
+    if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
+    HIPCHK(e, hipSetDevice(e->g.device));
+    return timed_set(own_set(e), n, ms_total, ms_kernel_mean);
 }
 
 int sjf_read_field(sjf_engine* e, int comp, double* out) {
     if (!e || !out || comp < 0 || comp > 5) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (int rc = sync_engine(e)) return rc;
     void* src = field_buf(e, comp);
-    return e->prec == 4 ? convert_from_device<float>(e, out, src, e->N)
-                        : convert_from_device<double>(e, out, src, e->N);
+    return e->prec == 4 ? convert_from_device<float>(e, out, src, e->N) : convert_from_device<double>(e, out, src, e->N);
 }
 
 int sjf_write_field(sjf_engine* e, int comp, const double* in) {
     if (!e || !in || comp < 0 || comp > 5) return SJF_E_ARG;
-    hipSetDevice(e->g.device);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (int rc = sync_engine(e)) return rc;
     void* dst = field_buf(e, comp);
-    return e->prec == 4 ? convert_to_device<float>(e, dst, in, e->N)
-                        : convert_to_device<double>(e, dst, in, e->N);
+    return e->prec == 4 ? convert_to_device<float>(e, dst, in, e->N) : convert_to_device<double>(e, dst, in, e->N);
 }
 
 int sjf_step_bytes(const sjf_engine* e, double* bytes) {
     if (!e || !bytes) return SJF_E_ARG;
-    const double p = e->prec;
-    double b = e->N * (12 * p + (e->uniform ? 0 : 4));
-    // polarisation: Drude pole = Q read+write, Lorentz pole = P and Q read+write
-    if (e->have_mat && e->maxp > 0) {
-        for (long q = 0; q < e->N; ++q)
-            for (int c = 0; c < 3; ++c) {
-                const int m = (e->matw_h[q] >> (8 * c)) & 0xff;
-                if (m == sjf::kDesignMat) continue;
-                const sjf_material& M = e->lut_h[m];
-                for (int s = 0; s < M.npoles; ++s) b += (M.is_drude[s] ? 2 : 4) * p;
-            }
-    }
-    if (!e->deps_h.empty()) b += p * (double)e->deps_h.size();
-    // PML: per cell and PML axis, 2 E + 2 H auxiliaries, each read + written
-    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
-    for (int d = 0; d < 3; ++d)
-        if (e->pml_on[d]) b += 8 * p * (double)(e->pml[2 * d] + e->pml[2 * d + 1]) * (e->N / n[d]);
-    *bytes = b;
+    *bytes = engine_step_bytes(e);
     return SJF_OK;
 }
 
@@ -1209,7 +1400,7 @@ int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int
     if (ty) *ty = e->ty;
     if (xchunk) *xchunk = e->xc;
     if (threads) *threads = block_threads(e);
-    if (groups) *groups = n_groups(e);
+    if (groups) *groups = tiles_of(e, e->ty);
     return SJF_OK;
 }
 
@@ -1220,8 +1411,120 @@ int sjf_set_tile(sjf_engine* e, int ty, int xchunk) {
     e->ty = ty;
     e->xc = std::min(xchunk, e->g.nx);
     e->tile_set = true;
+    e->version++;
     return SJF_OK;
 }
+
+// ---- batches --------------------------------------------------------------------------------
+
+int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out) {
+    if (!engines || n < 1 || !out) {
+        g_create_error = "null argument";
+        return SJF_E_ARG;
+    }
+    *out = nullptr;
+    sjf_engine* e0 = engines[0];
+    if (!e0) return SJF_E_ARG;
+    long span = -1;
+    for (int i = 0; i < n; ++i) {
+        sjf_engine* e = engines[i];
+        if (!e) return SJF_E_ARG;
+        std::string why;
+        if (e->batch) why = "engine already belongs to a batch";
+        else if (e->g.device != e0->g.device) why = "engines of a batch must share a device";
+        else if (e->prec != e0->prec) why = "engines of a batch must share a precision";
+        else if (e->step != e0->step) why = "engines of a batch must be at the same step";
+        else if (!e->mon.empty()) {
+            if (span >= 0 && span != e->save_span) why = "engines of a batch must share a save span";
+            span = e->save_span;
+        }
+        for (int j = 0; j < i && why.empty(); ++j)
+            if (engines[j] == e) why = "engine listed twice";
+        if (!why.empty()) {
+            g_create_error = why;
+            return SJF_E_ARG;
+        }
+    }
+    if (hipSetDevice(e0->g.device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return SJF_E_HIP;
+    }
+    for (int i = 0; i < n; ++i)
+        if (hipStreamSynchronize(engines[i]->stream) != hipSuccess) {
+            g_create_error = "stream synchronisation failed";
+            return SJF_E_HIP;
+        }
+    sjf_batch* b = new sjf_batch();
+    LaunchSet* s = new LaunchSet();
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        delete b;
+        g_create_error = "hipStreamCreate failed";
+        return SJF_E_HIP;
+    }
+    s->own_stream = true;
+    s->prec = e0->prec;
+    s->ty = 1;
+    for (int i = 0; i < n; ++i) {
+        s->eng.push_back(engines[i]);
+        engines[i]->batch = b;
+    }
+    b->set = s;
+    *out = b;
+    return SJF_OK;
+}
+
+int sjf_batch_step(sjf_batch* b, long n) {
+    if (!b || n < 0) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    if (hipSetDevice(s->eng[0]->g.device) != hipSuccess) return SJF_E_HIP;
+    const int rc = run_set(s, n);
+    if (rc) b->err = s->eng[0]->err;
+    return rc;
+}
+
+int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean) {
+    if (!b || n < 1 || !ms_total || !ms_kernel_mean) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    if (hipSetDevice(s->eng[0]->g.device) != hipSuccess) return SJF_E_HIP;
+    const int rc = timed_set(s, n, ms_total, ms_kernel_mean);
+    if (rc) b->err = s->eng[0]->err;
+    return rc;
+}
+
+int sjf_batch_synchronize(sjf_batch* b) {
+    if (!b) return SJF_E_ARG;
+    if (hipStreamSynchronize(b->set->stream) != hipSuccess) return SJF_E_HIP;
+    return SJF_OK;
+}
+
+int sjf_batch_info(const sjf_batch* b, long* cells, double* bytes_per_step) {
+    if (!b) return SJF_E_ARG;
+    long c = 0;
+    double by = 0;
+    for (const sjf_engine* e : b->set->eng) {
+        c += e->N;
+        by += engine_step_bytes(e);
+    }
+    if (cells) *cells = c;
+    if (bytes_per_step) *bytes_per_step = by;
+    return SJF_OK;
+}
+
+const char* sjf_batch_last_error(const sjf_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int sjf_batch_destroy(sjf_batch* b) {
+    if (!b) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    (void)hipSetDevice(s->eng[0]->g.device);
+    (void)hipStreamSynchronize(s->stream);
+    for (sjf_engine* e : s->eng) e->batch = nullptr;
+    destroy_set(s);
+    delete b;
+    return SJF_OK;
+}
+
+// ---- measurement probe ------------------------------------------------------------------------
 
 int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs) {
     if (!gbs || bytes < 16 || iters < 1) return SJF_E_ARG;
@@ -1230,26 +1533,26 @@ int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs) {
     float4 *a = nullptr, *b = nullptr;
     if (hipMalloc((void**)&a, n * 16) != hipSuccess) return SJF_E_NOMEM;
     if (hipMalloc((void**)&b, n * 16) != hipSuccess) {
-        hipFree(a);
+        (void)hipFree(a);
         return SJF_E_NOMEM;
     }
-    hipMemset(a, 0, n * 16);
-    hipMemset(b, 0, n * 16);
+    (void)hipMemset(a, 0, n * 16);
+    (void)hipMemset(b, 0, n * 16);
     hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
     const int blocks = 256 * 8 * 4;
     hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
-    hipEventRecord(t0, 0);
+    (void)hipEventRecord(t0, 0);
     for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
-    hipEventRecord(t1, 0);
-    hipError_t st = hipEventSynchronize(t1);
+    (void)hipEventRecord(t1, 0);
+    const hipError_t st = hipEventSynchronize(t1);
     float ms = 0;
-    hipEventElapsedTime(&ms, t0, t1);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    hipFree(a);
-    hipFree(b);
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     if (st != hipSuccess) return SJF_E_HIP;
     *gbs = 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
     return SJF_OK;

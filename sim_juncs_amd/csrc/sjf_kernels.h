@@ -56,8 +56,8 @@ struct StepArgs {
     int nty, xc;
     long N;       // cells
     T cdt;        // dt / dx
-    const T* F0;  // read state  [Ex Ey Ez Hx Hy Hz][N]
-    T* F1;        // write state [Ex Ey Ez Hx Hy Hz][N]
+    T* F[2];              // field sets [Ex Ey Ez Hx Hy Hz][N]; step n reads F[n&1], writes F[~n&1]
+    const long* d_step;   // the engine's step counter
     int uniform;  // 1: single non-dispersive material, no material word read
     int nmat;
     const uint32_t* matw;
@@ -66,8 +66,8 @@ struct StepArgs {
     T* P;  // [slot][comp][N] (Lorentz slots)
     int pml_on[3], plo[3], phi[3];
     long S[3];      // PML slab sizes (cells) per axis
-    T* psi[3];      // per axis [6][S]: E aux (2 comps), H aux pair A (2), H aux pair B (2)
-    int hr, hw;     // first segment of the H aux pair read / written this step (2 or 4)
+    T* psi[3];      // per axis [6][S]: E aux (2 comps), H aux pair A (2), H aux pair B (2);
+                    // step n reads pair A and writes pair B when n is even, the reverse when odd
     const T* coef;  // PML tables: for axis d at coff[d]: bE[n], aE[n], bH[n], aH[n]
     int coff[3];
     int dlo[3], dhi[3];
@@ -126,8 +126,8 @@ __device__ __forceinline__ void lds_barrier() {
 // Each plane iteration issues every global load it needs first (fields, PML auxiliaries,
 // polarisation state, and the next plane's E and material words), then computes H, exchanges
 // the z neighbours of H through LDS, and computes E.
-template <typename T, int TY, int NP, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
+template <typename T, int TY, int NP>
+__device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int B = blockDim.x;
     T* sh = reinterpret_cast<T*>(smem_raw);  // [2 parity][2 comp][TY][B]
@@ -135,10 +135,11 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
 
     for (int q = threadIdx.x; q < a.nmat; q += B) slut[q] = a.lut[q];
 
-    // XCD-aware tile order: hardware places workgroup w on XCD w % 8; map so that each XCD
-    // receives a contiguous run of logical tiles (y-tile fastest, then x-chunk).
-    const int w = blockIdx.x, nw = gridDim.x;
-    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    const long step = *a.d_step;
+    const T* F0 = a.F[step & 1];
+    T* F1 = a.F[(step + 1) & 1];
+    const int hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
+    const int hw = (step & 1) ? 2 : 4;  // and written
     const int tyi = t % a.nty, xci = t / a.nty;
     const int j0 = tyi * TY;
     const int x0 = xci * a.xc;
@@ -186,8 +187,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             const long q1 = (long)p1 * NYZ + (long)gj[r] * NZ + k;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                ec[r][c] = ldg((a.F0 + c * a.N), q0, ok0);
-                en[r][c] = ldg((a.F0 + c * a.N), q1, ok1);
+                ec[r][c] = ldg((F0 + c * a.N), q0, ok0);
+                en[r][c] = ldg((F0 + c * a.N), q1, ok1);
             }
         }
 #pragma unroll
@@ -216,9 +217,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
         for (int r = 0; r < TY + 2; ++r) {
             const bool ok = has_next && kin && gj[r] >= 0 && pnn >= 0;
             const long q = (long)pnn * NYZ + (long)gj[r] * NZ + k;
-            enn[r][0] = ldg((a.F0 + 0 * a.N), q, ok);
-            enn[r][1] = (r < TY + 1) ? ldg((a.F0 + 1 * a.N), q, ok) : T(0);
-            enn[r][2] = ldg((a.F0 + 2 * a.N), q, ok);
+            enn[r][0] = ldg((F0 + 0 * a.N), q, ok);
+            enn[r][1] = (r < TY + 1) ? ldg((F0 + 1 * a.N), q, ok) : T(0);
+            enn[r][2] = ldg((F0 + 2 * a.N), q, ok);
         }
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
@@ -231,29 +232,29 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             const int j = gj[r];
             const bool ok = kin && j >= 0;
             const long q = (long)pi * NYZ + (long)j * NZ + k;
-            h0[r][0] = ldg((a.F0 + 3 * a.N), q, ok);
-            h0[r][1] = ldg((a.F0 + 4 * a.N), q, ok);
-            h0[r][2] = ldg((a.F0 + 5 * a.N), q, ok);
+            h0[r][0] = ldg((F0 + 3 * a.N), q, ok);
+            h0[r][1] = ldg((F0 + 4 * a.N), q, ok);
+            h0[r][2] = ldg((F0 + 5 * a.N), q, ok);
             const bool okk = ok && kp1 >= 0;
             const long qk = q - k + kp1;
-            ek1[r][0] = ldg((a.F0 + 0 * a.N), qk, okk);
-            ek1[r][1] = ldg((a.F0 + 1 * a.N), qk, okk);
+            ek1[r][0] = ldg((F0 + 0 * a.N), qk, okk);
+            ek1[r][1] = ldg((F0 + 1 * a.N), qk, okk);
 #pragma unroll
             for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
             if (ok && xpml) {
                 const long s = slab_x(a, pi, j, k);
-                pH[r][0][0] = a.psi[0][(a.hr + 0) * a.S[0] + s];
-                pH[r][0][1] = a.psi[0][(a.hr + 1) * a.S[0] + s];
+                pH[r][0][0] = a.psi[0][(hr + 0) * a.S[0] + s];
+                pH[r][0][1] = a.psi[0][(hr + 1) * a.S[0] + s];
             }
             if (ok && ypml[r]) {
                 const long s = slab_y(a, pi, j, k);
-                pH[r][1][0] = a.psi[1][(a.hr + 0) * a.S[1] + s];
-                pH[r][1][1] = a.psi[1][(a.hr + 1) * a.S[1] + s];
+                pH[r][1][0] = a.psi[1][(hr + 0) * a.S[1] + s];
+                pH[r][1][1] = a.psi[1][(hr + 1) * a.S[1] + s];
             }
             if (ok && zpml) {
                 const long s = slab_z(a, pi, j, k);
-                pH[r][2][0] = a.psi[2][(a.hr + 0) * a.S[2] + s];
-                pH[r][2][1] = a.psi[2][(a.hr + 1) * a.S[2] + s];
+                pH[r][2][0] = a.psi[2][(hr + 0) * a.S[2] + s];
+                pH[r][2][1] = a.psi[2][(hr + 1) * a.S[2] + s];
             }
         }
         T pE[TY][3][2];
@@ -322,8 +323,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
                 if (wr) {
                     const long s = slab_y(a, pi, j, k);
-                    a.psi[1][(a.hw + 0) * a.S[1] + s] = pH[r][1][0];
-                    a.psi[1][(a.hw + 1) * a.S[1] + s] = pH[r][1][1];
+                    a.psi[1][(hw + 0) * a.S[1] + s] = pH[r][1][0];
+                    a.psi[1][(hw + 1) * a.S[1] + s] = pH[r][1][1];
                 }
             }
             if (ok && zpml) {
@@ -331,8 +332,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                 if (wr) {
                     const long s = slab_z(a, pi, j, k);
-                    a.psi[2][(a.hw + 0) * a.S[2] + s] = pH[r][2][0];
-                    a.psi[2][(a.hw + 1) * a.S[2] + s] = pH[r][2][1];
+                    a.psi[2][(hw + 0) * a.S[2] + s] = pH[r][2][0];
+                    a.psi[2][(hw + 1) * a.S[2] + s] = pH[r][2][1];
                 }
             }
             if (ok && xpml) {
@@ -340,8 +341,8 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
                 if (wr) {
                     const long s = slab_x(a, pi, j, k);
-                    a.psi[0][(a.hw + 0) * a.S[0] + s] = pH[r][0][0];
-                    a.psi[0][(a.hw + 1) * a.S[0] + s] = pH[r][0][1];
+                    a.psi[0][(hw + 0) * a.S[0] + s] = pH[r][0][0];
+                    a.psi[0][(hw + 1) * a.S[0] + s] = pH[r][0][1];
                 }
             }
             // outside a metallic edge H is identically zero (ghost row of the halo)
@@ -349,9 +350,9 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
             h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
             h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
             if (wr) {
-                a.F1[3 * a.N + q] = h[r][0];
-                a.F1[4 * a.N + q] = h[r][1];
-                a.F1[5 * a.N + q] = h[r][2];
+                F1[3 * a.N + q] = h[r][0];
+                F1[4 * a.N + q] = h[r][1];
+                F1[5 * a.N + q] = h[r][2];
             }
         }
 
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                 if (uniform) {
                     const T inv = slut[0].inv_eps;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) a.F1[c * a.N + q] = ec[r + 1][c] + inv * (cdt * curl[c]);
+                    for (int c = 0; c < 3; ++c) F1[c * a.N + q] = ec[r + 1][c] + inv * (cdt * curl[c]);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -433,7 +434,7 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
                                 }
                             }
                         }
-                        a.F1[c * a.N + q] = e0 + inv * (cdt * curl[c] - sq);
+                        F1[c * a.N + q] = e0 + inv * (cdt * curl[c] - sq);
                     }
                 }
             }
@@ -456,50 +457,147 @@ __global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T> a) {
     }
 }
 
+// Fused step over all instances of a launch set: logical tile t (after the XCD-aware remap)
+// belongs to instance b with tile_first[b] <= t < tile_first[b+1].
+template <typename T, int TY, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T>* __restrict__ args,
+                                               const int* __restrict__ tile_first, int ninst) {
+    // XCD-aware tile order: hardware places workgroup w on XCD w % 8; map so that each XCD
+    // receives a contiguous run of logical tiles (y-tile fastest, then x-chunk).
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tile_first[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    step_tile<T, TY, NP>(args[lo], t - tile_first[lo]);
+}
+
+// ---- per-step scalars live on the device --------------------------------------------------
+// Every engine owns a device step counter.  Kernels read the step from it (which also fixes
+// the read/write field buffer by its parity) and the last kernel of a step increments it, so a
+// sequence of steps can be captured once in a hipGraph and replayed at any step.
+
+__global__ void k_tick(long* const* counters, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i][0] += 1;
+}
+
+// ---- source time functions (FP64) ---------------------------------------------------------
+// gaussian_src_time_phase::dipole (reference src/disp.cpp:395-400) and meep's
+// continuous_src_time::dipole, evaluated in FP64 on the device.
+struct PulseDev {
+    int kind;  // 0 gaussian, 1 continuous
+    int part;  // 0 real part, 1 imaginary part of amp * current
+    double omega, width, phi, peak, cutoff;
+    double freq, cw_width, t_start, t_end, slow;
+    double amp_re, amp_im;
+    double dt;
+};
+
+__device__ inline void dipole_dev(const PulseDev& s, double t, double* re, double* im) {
+    if (s.kind == 0) {
+        const double tt = t - s.peak;
+        if ((double)(float)fabs(tt) > s.cutoff) {
+            *re = *im = 0.0;
+            return;
+        }
+        const double g = exp(-tt * tt / (2 * s.width * s.width));
+        double sn, cs;
+        sincos(-s.omega * tt - s.phi, &sn, &cs);
+        // polar(1, th) / (-i omega) = (cos th + i sin th) (i / omega)
+        *re = -g * sn / s.omega;
+        *im = g * cs / s.omega;
+    } else {
+        const float rt = (float)t;
+        if (rt < s.t_start || rt > s.t_end) {
+            *re = *im = 0.0;
+            return;
+        }
+        const double w = 2 * M_PI * s.freq;
+        double env = 1.0;
+        if (s.cw_width != 0.0) {
+            const double ts = (t - s.t_start) / s.cw_width - s.slow;
+            const double te = (s.t_end - t) / s.cw_width - s.slow;
+            env = (1.0 + tanh(ts)) * (1.0 + tanh(te)) * 0.25;
+        }
+        double sn, cs;
+        sincos(-w * t, &sn, &cs);
+        *re = -env * sn / w;
+        *im = env * cs / w;
+    }
+}
+
+// amp * J^{n+1/2}, J = (dipole(t_{n+1}) - dipole(t_n)) / dt, real or imaginary part
+__device__ inline double source_current(const PulseDev& s, long step) {
+    double r0, i0, r1, i1;
+    dipole_dev(s, step * s.dt, &r0, &i0);
+    dipole_dev(s, (step + 1) * s.dt, &r1, &i1);
+    const double jr = (r1 - r0) / s.dt, ji = (i1 - i0) / s.dt;
+    return s.part ? s.amp_re * ji + s.amp_im * jr : s.amp_re * jr - s.amp_im * ji;
+}
+
 // ---- source injection: E -= coef * J on the source's cells (coef = dt * weight * inv_eps)
+// One block covers entries of one source; the block's source and entry range come from
+// per-block tables, so all sources of all engines in a launch set go in one launch.
 template <typename T>
-__global__ void k_source(T* F, const long* idx, const double* coef, long n, double J) {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+struct SrcDev {
+    PulseDev p;
+    T* F[2];             // the source component in field set 0 / 1
+    const long* d_step;  // the owning engine's step counter
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ src,
+                                                const int* __restrict__ blk_src,
+                                                const long* __restrict__ blk_first,
+                                                const long* __restrict__ blk_end,
+                                                const long* __restrict__ idx,
+                                                const double* __restrict__ coef) {
+    __shared__ double J;
+    const int sid = blk_src[blockIdx.x];
+    const SrcDev<T>& S = src[sid];
+    const long step = *S.d_step;
+    if (threadIdx.x == 0) J = source_current(S.p, step);
+    __syncthreads();
+    const long e = blk_first[blockIdx.x] + threadIdx.x;
+    if (e >= blk_end[blockIdx.x] || J == 0.0) return;
+    T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
     const long q = idx[e];
     F[q] = (T)((double)F[q] - coef[e] * J);
 }
 
 // ---- point monitors: trilinear gather, FP64 result --------------------------------------
 template <typename T>
-struct FieldPtrs {
-    const T* f[6];
+struct MonDev {
+    const T* F[2];       // the sampled component in field set 0 / 1
+    const long* d_step;  // owning engine's counter
+    double* out;         // sample buffer of the owning engine: [sample][npt]
+    int npt;             // points of the owning engine
+    int p;               // index of this point within its engine
+    int span;            // save span
+    long idx[8];
+    double w[8];
 };
 
 template <typename T>
-__global__ void k_monitor(FieldPtrs<T> F, const long* idx8, const double* w8, const int* comp,
-                          int npt, double* out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npt) return;
-    const T* f = F.f[comp[p]];
+__global__ void __launch_bounds__(256) k_monitor(const MonDev<T>* __restrict__ mon, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MonDev<T>& m = mon[i];
+    const long step = *m.d_step;
+    if (step % m.span) return;
+    const T* f = m.F[step & 1];
     double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const long id = idx8[8 * p + q];
-        if (id >= 0) v += w8[8 * p + q] * (double)f[id];
-    }
-    out[p] = v;
+    for (int q = 0; q < 8; ++q)
+        if (m.idx[q] >= 0) v += m.w[q] * (double)f[m.idx[q]];
+    m.out[(step / m.span) * m.npt + m.p] = v;
 }
 
 // ---- DFT: Neumaier-compensated FP64 accumulation of F exp(i 2 pi f t) dt ----------------
-template <typename T>
-struct DftArgs {
-    FieldPtrs<T> F;
-    int lo[3], ext[3];
-    long ncell;
-    int comps[6];
-    int ncomp;
-    int nf;
-    double2* acc;  // [slot][f][cell]
-    double2* cmp;  // compensation terms, same layout
-    double2 ph[2][kMaxDftFreqs];  // [E/H][f] = (cos, sin) * dt at the component's time
-};
-
 __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
     const double t = s + x;
     if (fabs(s) >= fabs(x)) c += (s - t) + x;
@@ -508,23 +606,55 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
 }
 
 template <typename T>
-__global__ void k_dft(const DftArgs<T> a, long nyz, long nz) {
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= a.ncell) return;
+struct DftDev {
+    const T* F[2];       // field set 0 / 1 base, [6][N]
+    const long* d_step;  // owning engine's counter
+    long N, nyz, nz;
+    int lo[3], ext[3];
+    long ncell;
+    long first;          // first global cell index of this region in the launch
+    int comps[6];
+    int ncomp;
+    int nf;
+    const double* freq;  // nf frequencies
+    double dt;
+    double2* acc;        // [slot][f][cell]
+    double2* cmp;        // compensation terms, same layout
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int lo = 0, hi = nreg - 1;
+    while (lo < hi) {  // region with first <= g
+        const int mid = (lo + hi + 1) / 2;
+        if (regs[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const DftDev<T>& a = regs[lo];
+    const long cell = g - a.first;
+    const long step = *a.d_step;
+    const T* F = a.F[(step + 1) & 1];
     const int kk = (int)(cell % a.ext[2]);
     const long rest = cell / a.ext[2];
     const int jj = (int)(rest % a.ext[1]);
     const int ii = (int)(rest / a.ext[1]);
-    const long q = (long)(a.lo[0] + ii) * nyz + (long)(a.lo[1] + jj) * nz + (a.lo[2] + kk);
-    for (int s = 0; s < a.ncomp; ++s) {
-        const int c = a.comps[s];
-        const double v = (double)a.F.f[c][q];
-        const double2* ph = a.ph[c < 3 ? 0 : 1];
-        for (int f = 0; f < a.nf; ++f) {
+    const long q = (long)(a.lo[0] + ii) * a.nyz + (long)(a.lo[1] + jj) * a.nz + (a.lo[2] + kk);
+    // E after the step is at t = (n+1) dt, H at (n + 1/2) dt
+    const double tE = (step + 1) * a.dt, tH = (step + 0.5) * a.dt;
+    for (int f = 0; f < a.nf; ++f) {
+        const double w = 2 * M_PI * a.freq[f];
+        double sE, cE, sH, cH;
+        sincos(w * tE, &sE, &cE);
+        sincos(w * tH, &sH, &cH);
+        for (int s = 0; s < a.ncomp; ++s) {
+            const int c = a.comps[s];
+            const double v = (double)F[(long)c * a.N + q] * a.dt;
             const long o = ((long)s * a.nf + f) * a.ncell + cell;
             double2 sm = a.acc[o], cp = a.cmp[o];
-            neumaier(sm.x, cp.x, v * ph[f].x);
-            neumaier(sm.y, cp.y, v * ph[f].y);
+            neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
+            neumaier(sm.y, cp.y, v * (c < 3 ? sE : sH));
             a.acc[o] = sm;
             a.cmp[o] = cp;
         }

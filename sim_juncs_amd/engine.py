@@ -75,8 +75,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.sjf_destroy(self.h)
-            self.h = None
+            if self.lib.sjf_destroy(self.h) == 0:
+                self.h = None
 
     def __del__(self):
         try:
@@ -241,3 +241,56 @@ class Engine:
 
     def set_tile(self, ty: int, xchunk: int):
         self._chk(self.lib.sjf_set_tile(self.h, int(ty), int(xchunk)), "sjf_set_tile")
+
+    def set_graph(self, chunk: int):
+        """Steps per replayed hipGraph (0: launch every step directly)."""
+        self._chk(self.lib.sjf_set_graph(self.h, int(chunk)), "sjf_set_graph")
+
+
+class Batch:
+    """Engines of uneven grid shapes advanced together: one fused launch per time step over all
+    of them (sjf_batch_* in include/sjf.h)."""
+
+    def __init__(self, engines):
+        self.lib = _lib.load()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = self.lib.sjf_batch_create(arr, len(self.engines), C.byref(h))
+        if rc != 0:
+            raise _lib.EngineError(f"sjf_batch_create failed ({_lib.STATUS.get(rc, rc)}): "
+                                   f"{self.lib.sjf_batch_last_error(None).decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise _lib.EngineError(f"{what} failed ({_lib.STATUS.get(rc, rc)}): "
+                                   f"{self.lib.sjf_batch_last_error(self.h).decode()}")
+
+    def step(self, n: int = 1):
+        self._chk(self.lib.sjf_batch_step(self.h, int(n)), "sjf_batch_step")
+
+    def synchronize(self):
+        self._chk(self.lib.sjf_batch_synchronize(self.h), "sjf_batch_synchronize")
+
+    def timed_steps(self, n: int):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.sjf_batch_timed_steps(self.h, int(n), C.byref(a), C.byref(b)),
+                  "sjf_batch_timed_steps")
+        return a.value, b.value
+
+    def info(self):
+        c, b = C.c_long(), C.c_double()
+        self._chk(self.lib.sjf_batch_info(self.h, C.byref(c), C.byref(b)), "sjf_batch_info")
+        return c.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sjf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -208,3 +208,70 @@ def bowtie_junction(nx=512, ny=512, nz=256, steps=6000, precision="fp32", pml=12
         lut=lut, pml=([pml] * 6, 1e-15, 2.0), sources=[plane_source(shape, dx, src_z, pulse)],
         dfts=dfts, monitors=[("Ex", np.array([[(cx + 0.5) * dx, cy * dx, gz * dx]]))],
         save_span=20, steps=steps, meta=dict(src_z=src_z, points=pts))
+
+
+def gap_junction(shape, gap, kind="box", dx=0.005, pml=12, fwhm_fs=10.0, cep=0.0,
+                 steps=3000, precision="fp32", thick=None, nfreq=8, name=None):
+    """One junction instance of the batched sweep.  kind "box": two Au (Drude + 2 Lorentz)
+    electrodes of the junctions/Au_SiO2_box layout -- slabs filling x outside a gap of `gap`
+    cells, SiO2 (1-pole Lorentz) filling the gap and the substrate below; kind "sidetie": the
+    junctions/Au_SiO2_sidetie layout -- 45-degree Au wedges in the x-z plane pointing at each
+    other across the gap, on a SiO2 substrate.  x-polarised Gaussian plane wave from the top,
+    one DFT point at the gap centre."""
+    nx, ny, nz = shape
+    mat = np.zeros(tuple(shape) + (3,), np.uint8)
+    cx, cy = nx / 2, ny / 2
+    zc = nz / 2
+    h = thick if thick is not None else max(4, nz // 8)
+    zt, zb = zc - h / 2, zc + h / 2
+    xl, xr = cx - gap / 2, cx + gap / 2
+    # substrate below the junction layer (to the far edge, as the reference's inf_thick)
+    paint(mat, shape, box_inside((-1, -1, zb), (nx + 1, ny + 1, nz + 1)), 1)
+    if kind == "box":
+        paint(mat, shape, box_inside((xl, -1, zt), (xr, ny + 1, zb)), 1)
+        paint(mat, shape, box_inside((-1, -1, zt), (xl, ny + 1, zb)), 2)
+        paint(mat, shape, box_inside((xr, -1, zt), (nx + 1, ny + 1, zb)), 2)
+    else:
+        def wedge(x, y, z):
+            u = np.maximum(xl - x, x - xr)  # distance behind the tip plane of either wedge
+            return (u >= 0) & (np.abs(z - zc) <= np.minimum(u, h / 2 + 0 * u)) & (y > -1)
+        paint(mat, shape, wedge, 2)
+    lut = [M.VACUUM, M.SIO2.scaled(1.0), M.AU.scaled(1.0)]
+    pulse = gaussian_pulse(0.8, fwhm_fs, cep)
+    src_z = pml + 2
+    f0 = pulse["freq"]
+    p = (int(cx), int(cy), int(zc))
+    freqs = list(np.linspace(0.8, 1.2, nfreq) * f0)
+    return Scenario(
+        name=name or f"{kind}_{nx}x{ny}x{nz}_g{gap}", shape=tuple(shape), dx=dx,
+        precision=precision, matw=pack(mat), lut=lut, pml=([pml] * 6, 1e-15, 2.0),
+        sources=[plane_source(shape, dx, src_z, pulse)],
+        dfts=[(p, (p[0] + 1, p[1] + 1, p[2] + 1), ("Ex", "Ey", "Ez"), freqs)],
+        monitors=[("Ex", np.array([[(p[0] + 0.5) * dx, p[1] * dx, p[2] * dx]]))], save_span=20,
+        steps=steps, meta=dict(kind=kind, gap=gap, cep=cep, fwhm=fwhm_fs))
+
+
+def batched_sweep(n=64, seed=3, steps=3000, precision="fp32", scale=1.0):
+    """Batched sweep: `n` instances with grid sizes drawn from the seed between 64x64x64 and
+    192x192x128 (times `scale` for reduced cases), box or side-tie Au/SiO2 junctions with gap
+    widths uniform in 10-40 cells, CEP uniform in [0, 2 pi), pulse FWHM uniform in 5-15 fs,
+    12-cell PML (8 below 96 cells), per-instance DFT point at the gap centre."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx, ny = (int(v) for v in rng.integers(64, 193, size=2))
+        nz = int(rng.integers(64, 129))
+        kind = "box" if rng.random() < 0.5 else "sidetie"
+        gap = int(rng.integers(10, 41))
+        cep = float(rng.uniform(0, 2 * math.pi))
+        fwhm = float(rng.uniform(5, 15))
+        if scale != 1.0:
+            nx, ny, nz = (max(16, int(round(v * scale))) for v in (nx, ny, nz))
+            gap = max(2, int(round(gap * scale)))
+        pml = 12 if min(nx, ny, nz) >= 96 else 8
+        if scale != 1.0:
+            pml = max(3, int(round(pml * scale)))
+        gap = min(gap, nx - 2 * pml - 4)
+        out.append(gap_junction((nx, ny, nz), gap, kind, pml=pml, fwhm_fs=fwhm, cep=cep,
+                                steps=steps, precision=precision, name=f"sweep{i:02d}_{kind}"))
+    return out
