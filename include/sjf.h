@@ -192,6 +192,18 @@ int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int
 /* Override the autotuned tile shape (ty in {1,2,4,8}, xchunk >= 1). */
 int sjf_set_tile(sjf_engine* e, int ty, int xchunk);
 
+/* ---- adjoint -----------------------------------------------------------------------------
+ * New (the reference has no gradients).  Runs `nsteps` steps from step 0 and returns the
+ * objective J = dt * sum_{n=1..nsteps} |E^n(gap)|^2 (E of the cell `gap`, all three components)
+ * and its exact gradient with respect to every design-region permittivity (sjf_set_design_eps;
+ * grad in the same z-fastest order, cap >= number of design voxels).  The gradient is the
+ * discrete adjoint of the update the engine runs: the forward pass stores the full state every
+ * `checkpoint_every` steps, the backward pass recomputes each segment from its checkpoint and
+ * applies the transposed step (Yee curls, CPML and pole recursions) in reverse.  Monitors and
+ * DFT regions see the forward pass once; the engine ends in its final forward state. */
+int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
+                    double* grad, size_t cap);
+
 /* ---- batches ----------------------------------------------------------------------------
  * New (the reference sweeps parameters with one meep process per SLURM task,
  * scripts/run.sh:47-78): engines of any grid shapes, same device, precision, step count and

@@ -33,7 +33,7 @@ EXPORTS = [
     "sjf_timed_steps", "sjf_read_field", "sjf_write_field", "sjf_step_bytes",
     "sjf_launch_info", "sjf_set_tile", "sjf_copy_probe", "sjf_set_graph",
     "sjf_batch_create", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
-    "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy",
+    "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
 ]
 
 
@@ -119,6 +119,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_batch_info": (C.c_int, [vp, lp, dp]),
         "sjf_batch_last_error": (C.c_char_p, [vp]),
         "sjf_batch_destroy": (C.c_int, [vp]),
+        "sjf_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, dp, C.c_size_t]),
     }
     for name, (res, args) in proto.items():
         f = getattr(lib, name)

@@ -543,8 +543,8 @@ int launch_step_t(LaunchSet* s) {
 
 // One time step of every engine in the set.  `sample` launches the monitor gather first.
 template <typename T>
-int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_b) {
-    if (sample && s->nmon) {
+int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_b, bool bare = false) {
+    if (sample && s->nmon && !bare) {
         hipLaunchKernelGGL((sjf::k_monitor<T>), dim3((s->nmon + 255) / 256), dim3(256), 0, s->stream,
                            (const sjf::MonDev<T>*)s->d_mon, s->nmon);
         SETCHK(s, hipGetLastError());
@@ -559,7 +559,7 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
                            s->d_src_idx, s->d_src_coef);
         SETCHK(s, hipGetLastError());
     }
-    if (s->ndft) {
+    if (s->ndft && !bare) {
         hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256)), dim3(256), 0,
                            s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total);
         SETCHK(s, hipGetLastError());
@@ -572,6 +572,12 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
 
 int launch_one_step(LaunchSet* s, bool sample, hipEvent_t a = nullptr, hipEvent_t b = nullptr) {
     return s->prec == 4 ? launch_one_step_t<float>(s, sample, a, b) : launch_one_step_t<double>(s, sample, a, b);
+}
+
+// step kernel + sources + counter only (forward recomputation inside the adjoint)
+int launch_bare_step(LaunchSet* s) {
+    return s->prec == 4 ? launch_one_step_t<float>(s, false, nullptr, nullptr, true)
+                        : launch_one_step_t<double>(s, false, nullptr, nullptr, true);
 }
 
 bool samples_at(const LaunchSet* s, long step) {
@@ -888,6 +894,204 @@ double engine_step_bytes(const sjf_engine* e) {
 }
 
 }  // namespace
+
+
+// ======================================================================== adjoint =========
+
+namespace {
+
+// Everything that evolves with the time step (the current field set, polarisation state and all
+// PML auxiliaries), in bytes.
+struct StateLayout {
+    size_t fields = 0, q = 0, p = 0, psi[3] = {0, 0, 0}, total = 0;
+};
+
+StateLayout state_layout(const sjf_engine* e) {
+    StateLayout L;
+    L.fields = (size_t)e->prec * 6 * e->N;
+    L.q = e->Qbuf ? (size_t)e->prec * 3 * e->maxp * e->N : 0;
+    L.p = e->Pbuf ? (size_t)e->prec * 3 * e->maxp * e->N : 0;
+    L.total = L.fields + L.q + L.p;
+    for (int d = 0; d < 3; ++d) {
+        L.psi[d] = e->psibuf[d] ? (size_t)e->prec * 6 * e->slab[d] : 0;
+        L.total += L.psi[d];
+    }
+    return L;
+}
+
+int copy_state(sjf_engine* e, char* ck, const StateLayout& L, bool save) {
+    auto cp = [&](void* dev, size_t off, size_t bytes) -> hipError_t {
+        if (!bytes) return hipSuccess;
+        return save ? hipMemcpyAsync(ck + off, dev, bytes, hipMemcpyDeviceToDevice, e->stream)
+                    : hipMemcpyAsync(dev, ck + off, bytes, hipMemcpyDeviceToDevice, e->stream);
+    };
+    size_t off = 0;
+    HIPCHK(e, cp(e->Fbuf[e->step & 1], off, L.fields));
+    off += L.fields;
+    HIPCHK(e, cp(e->Qbuf, off, L.q));
+    off += L.q;
+    HIPCHK(e, cp(e->Pbuf, off, L.p));
+    off += L.p;
+    for (int d = 0; d < 3; ++d) {
+        HIPCHK(e, cp(e->psibuf[d], off, L.psi[d]));
+        off += L.psi[d];
+    }
+    return SJF_OK;
+}
+
+int set_step(sjf_engine* e, long n) {
+    e->step = n;
+    HIPCHK(e, hipMemcpyAsync(e->d_step, &e->step, sizeof(long), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SJF_OK;
+}
+
+template <typename T>
+int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, double* grad, long nd) {
+    LaunchSet* s = own_set(e);
+    int rc = prepare_set(s, nsteps);
+    if (rc) return rc;
+    const StateLayout L = state_layout(e);
+    const long nck = (nsteps + K - 1) / K;
+    char* cks = nullptr;
+    if (hipMalloc((void**)&cks, L.total * (nck + 1)) != hipSuccess)
+        return fail(e, SJF_E_NOMEM, "checkpoint storage allocation failed (" +
+                                        std::to_string((double)L.total * (nck + 1) / 1e9) + " GB)");
+    double* dJ = nullptr;
+    double* gacc = nullptr;
+    T* ring = nullptr;
+    T* A = nullptr;
+    T* Aq = nullptr;
+    T* Ap = nullptr;
+    T* Apsi[3] = {nullptr, nullptr, nullptr};
+    T** dA = nullptr;
+    auto cleanup = [&]() {
+        dev_free(cks);
+        dev_free(dJ);
+        dev_free(gacc);
+        dev_free(ring);
+        dev_free(A);
+        dev_free(Aq);
+        dev_free(Ap);
+        for (int d = 0; d < 3; ++d) dev_free(Apsi[d]);
+        dev_free(dA);
+    };
+    auto bail = [&](int r) {
+        cleanup();
+        return r;
+    };
+    const long slot = 3 * nd + 3;
+    if (dev_alloc(e, (void**)&dJ, sizeof(double)) || dev_alloc(e, (void**)&gacc, sizeof(double) * nd) ||
+        dev_alloc(e, (void**)&ring, sizeof(T) * slot * (K + 1)) || dev_alloc(e, (void**)&A, sizeof(T) * 6 * e->N) ||
+        (L.q && dev_alloc(e, (void**)&Aq, L.q)) || (L.p && dev_alloc(e, (void**)&Ap, L.p)) ||
+        dev_alloc(e, (void**)&dA, sizeof(T*) * 6))
+        return bail(SJF_E_NOMEM);
+    for (int d = 0; d < 3; ++d)
+        if (L.psi[d] && dev_alloc(e, (void**)&Apsi[d], L.psi[d])) return bail(SJF_E_NOMEM);
+    T* Ah[6];
+    for (int c = 0; c < 6; ++c) Ah[c] = A + (size_t)c * e->N;
+    if ((rc = h2d(e, dA, Ah, sizeof(Ah)))) return bail(rc);
+
+    const double dt = e->dt;
+    // ---- forward pass: checkpoints every K steps, objective after every step
+    for (long n = 0; n < nsteps; ++n) {
+        if (n % K == 0) {
+            if ((rc = copy_state(e, cks + L.total * (n / K), L, true))) return bail(rc);
+        }
+        if ((rc = launch_one_step(s, samples_at(s, e->step)))) return bail(rc);
+        advance_hosts(s, 1);
+        const T* E = (const T*)e->Fbuf[e->step & 1];
+        hipLaunchKernelGGL((sjf::k_adj_objective<T>), dim3(1), dim3(64), 0, e->stream, E, e->N, gap, dt, dJ);
+        HIPCHK(e, hipGetLastError());
+    }
+    if ((rc = copy_state(e, cks + L.total * nck, L, true))) return bail(rc);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if ((rc = d2h(e, J_out, dJ, sizeof(double)))) return bail(rc);
+
+    // ---- backward pass, segment by segment
+    sjf::AdjArgs<T> g;
+    std::memset(&g, 0, sizeof(g));
+    g.f = step_args<T>(e, e->ty);
+    for (int c = 0; c < 6; ++c) g.A[c] = Ah[c];
+    g.Aq = Aq;
+    g.Ap = Ap;
+    for (int d = 0; d < 3; ++d) g.Apsi[d] = Apsi[d];
+    const unsigned gcell = (unsigned)((e->N + 255) / 256);
+    const unsigned gdes = (unsigned)((nd + 3 + 255) / 256);
+    const int de0 = e->dhi[0] - e->dlo[0], de1 = e->dhi[1] - e->dlo[1], de2 = e->dhi[2] - e->dlo[2];
+    auto record = [&](long sl) -> int {
+        const T* E = (const T*)e->Fbuf[e->step & 1];
+        hipLaunchKernelGGL((sjf::k_adj_record<T>), dim3(gdes), dim3(256), 0, e->stream, E, e->N, e->g.ny, e->g.nz,
+                           e->dlo[0], e->dlo[1], e->dlo[2], de0, de1, de2, gap, ring, sl);
+        HIPCHK(e, hipGetLastError());
+        return SJF_OK;
+    };
+    const int npv = np_variant_of(e->maxp, e->uniform);
+    for (long k = nck - 1; k >= 0; --k) {
+        const long n0 = k * K, n1 = std::min(n0 + K, nsteps);
+        if ((rc = set_step(e, n0))) return bail(rc);
+        if ((rc = copy_state(e, cks + L.total * k, L, false))) return bail(rc);
+        if ((rc = record(0))) return bail(rc);
+        for (long n = n0; n < n1; ++n) {
+            if ((rc = launch_bare_step(s))) return bail(rc);
+            e->step++;
+            if ((rc = record(n + 1 - n0))) return bail(rc);
+        }
+        if (k == nck - 1) {
+            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, dA, ring, nd, n1 - n0, gap, dt);
+            HIPCHK(e, hipGetLastError());
+        }
+        for (long n = n1 - 1; n >= n0; --n) {
+            hipLaunchKernelGGL((sjf::k_adj_grad<T>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, e->stream, g,
+                               ring, n + 1 - n0, n - n0, gacc);
+            HIPCHK(e, hipGetLastError());
+            g.hr = (n & 1) ? 2 : 4;
+            g.hw = (n & 1) ? 4 : 2;
+            hipLaunchKernelGGL((sjf::k_adj_h<T>), dim3(gcell), dim3(256), 0, e->stream, g);
+            HIPCHK(e, hipGetLastError());
+            switch (npv) {
+                case 0: hipLaunchKernelGGL((sjf::k_adj_e<T, 0>), dim3(gcell), dim3(256), 0, e->stream, g); break;
+                case 1: hipLaunchKernelGGL((sjf::k_adj_e<T, 1>), dim3(gcell), dim3(256), 0, e->stream, g); break;
+                case 3: hipLaunchKernelGGL((sjf::k_adj_e<T, 3>), dim3(gcell), dim3(256), 0, e->stream, g); break;
+                default: hipLaunchKernelGGL((sjf::k_adj_e<T, 4>), dim3(gcell), dim3(256), 0, e->stream, g); break;
+            }
+            HIPCHK(e, hipGetLastError());
+            if (n >= 1) {
+                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, dA, ring, nd, n - n0, gap, dt);
+                HIPCHK(e, hipGetLastError());
+            }
+        }
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::vector<double> gh(nd);
+    if ((rc = d2h(e, gh.data(), gacc, sizeof(double) * nd))) return bail(rc);
+    for (long v = 0; v < nd; ++v) grad[v] = -gh[v] / e->deps_h[v];
+    // leave the engine in its final forward state
+    if ((rc = set_step(e, nsteps))) return bail(rc);
+    if ((rc = copy_state(e, cks + L.total * nck, L, false))) return bail(rc);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    cleanup();
+    return SJF_OK;
+}
+
+}  // namespace
+
+extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
+                               double* grad, size_t cap) {
+    if (!e || nsteps < 1 || checkpoint_every < 1 || !gap || !J || !grad) return SJF_E_ARG;
+    if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
+    if (e->step != 0) return fail(e, SJF_E_STATE, "the adjoint run starts from step 0");
+    if (e->deps_h.empty()) return fail(e, SJF_E_STATE, "no design region (sjf_set_design_eps)");
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    for (int d = 0; d < 3; ++d)
+        if (gap[d] < 0 || gap[d] >= n[d]) return fail(e, SJF_E_ARG, "objective cell outside the grid");
+    const long nd = (long)e->deps_h.size();
+    if (cap < (size_t)nd) return fail(e, SJF_E_ARG, "gradient buffer smaller than the design region");
+    HIPCHK(e, hipSetDevice(e->g.device));
+    const long q = cell_index(e, gap[0], gap[1], gap[2]);
+    return e->prec == 4 ? adjoint_run_t<float>(e, nsteps, checkpoint_every, q, J, grad, nd)
+                        : adjoint_run_t<double>(e, nsteps, checkpoint_every, q, J, grad, nd);
+}
 
 // ============================================================================ batches ====
 

@@ -661,6 +661,295 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
     }
 }
 
+// ===================================================== adjoint (reverse-mode) kernels ======
+// Exact transpose of one k_step update (Yee curls, CPML recursions, (P, Q) pole recursions),
+// applied backwards in time to the adjoint state (Ebar, Hbar, Qbar, Pbar, psiEbar, psiHbar).
+// With b the CPML decay of a derivative term and a = b - 1, the forward term
+//   d~ = b (d + psi),  psi' = b psi + (b - 1) d
+// transposes to  dbar = b dbar~ + (b - 1) psibar',  psibar = b (dbar~ + psibar').
+// Non-PML terms have b = 1.  The adjoint of a backward difference e(r) = H(r) - H(r-1) adds
+// ebar(q) - ebar(q+1) to Hbar(q); the adjoint of a forward difference d(r) = E(r+1) - E(r)
+// adds dbar(q-1) - dbar(q) to Ebar(q); positions outside a metallic edge contribute zero.
+// Step 1 (k_adj_h) produces Hbar at the old time and the new H auxiliaries; step 2 (k_adj_e)
+// produces Ebar, Qbar, Pbar and the E auxiliaries.
+
+template <typename T>
+struct AdjArgs {
+    StepArgs<T> f;     // forward description (geometry, materials, PML tables)
+    T* A[6];           // adjoint fields Ebar x,y,z and Hbar x,y,z
+    T* Aq;             // Qbar [slot][comp][N]
+    T* Ap;             // Pbar [slot][comp][N]
+    T* Apsi[3];        // per axis [6][S]: E aux bar x2, H aux bar pair A x2, pair B x2
+    int hr, hw;        // H aux bar pair read (2 or 4) / written (4 or 2)
+};
+
+__device__ __forceinline__ void cell_of(long q, int ny, int nz, int& i, int& j, int& k) {
+    k = (int)(q % nz);
+    const long r = q / nz;
+    j = (int)(r % ny);
+    i = (int)(r / ny);
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_eps_of(const StepArgs<T>& a, long q, int i, int j, int k, int c) {
+    if (a.uniform) return a.lut[0].inv_eps;
+    const int m = (a.matw[q] >> (8 * c)) & 0xff;
+    if (m == kDesignMat) {
+        const long dq = ((long)(i - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
+                            (a.dhi[2] - a.dlo[2]) + (k - a.dlo[2]);
+        return a.dinv[dq];
+    }
+    return a.lut[m].inv_eps;
+}
+
+template <typename T>
+__device__ __forceinline__ long neighbour(const StepArgs<T>& a, int i, int j, int k, int d, int dir) {
+    int c[3] = {i, j, k};
+    const int n[3] = {a.nx, a.ny, a.nz};
+    const int p[3] = {a.px, a.py, a.pz};
+    c[d] += dir;
+    c[d] = wrap_or_ghost(c[d], n[d], p[d]);
+    if (c[d] < 0) return -1;
+    return ((long)c[0] * a.ny + c[1]) * a.nz + c[2];
+}
+
+template <typename T>
+__device__ __forceinline__ long slab_idx(const StepArgs<T>& a, int d, int i, int j, int k) {
+    return d == 0 ? slab_x(a, i, j, k) : (d == 1 ? slab_y(a, i, j, k) : slab_z(a, i, j, k));
+}
+
+template <typename T>
+__device__ __forceinline__ bool pml_at(const StepArgs<T>& a, int d, int i, int j, int k) {
+    const int c = d == 0 ? i : (d == 1 ? j : k);
+    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
+    return in_pml(a, d, c, n);
+}
+
+// E-located curl terms: component c, axis d, sign of (dH) in the curl, H component used,
+// slot of the E auxiliary for (axis d).  Ex: +dy Hz - dz Hy; Ey: +dz Hx - dx Hz; Ez: +dx Hy - dy Hx.
+__constant__ const int kEterm[3][2][4] = {
+    {{1, +1, 5, 0}, {2, -1, 4, 0}},   // Ex: (axis y, +, Hz, slot Ex on y = 0), (axis z, -, Hy, slot Ex on z = 0)
+    {{2, +1, 3, 1}, {0, -1, 5, 0}},   // Ey: (z, +, Hx, slot Ey on z = 1), (x, -, Hz, slot Ey on x = 0)
+    {{0, +1, 4, 1}, {1, -1, 3, 1}},   // Ez: (x, +, Hy, slot Ez on x = 1), (y, -, Hx, slot Ez on y = 1)
+};
+// H-located curl terms: H component h (0..2 = Hx..Hz), axis d, sign, E component used, slot.
+// Hx: -(dy Ez - dz Ey); Hy: -(dz Ex - dx Ez); Hz: -(dx Ey - dy Ex).
+__constant__ const int kHterm[3][2][4] = {
+    {{1, +1, 2, 0}, {2, -1, 1, 0}},   // Hx: (y, +, Ez, slot Hx on y = 0), (z, -, Ey, slot Hx on z = 0)
+    {{2, +1, 0, 1}, {0, -1, 2, 0}},   // Hy: (z, +, Ex, slot Hy on z = 1), (x, -, Ez, slot Hy on x = 0)
+    {{0, +1, 1, 1}, {1, -1, 0, 1}},   // Hz: (x, +, Ey, slot Hz on x = 1), (y, -, Ex, slot Hz on y = 1)
+};
+
+// ebar of E-term (c, t) at cell r: from Ebar', the E auxiliary bar (old value) and bE
+template <typename T>
+__device__ __forceinline__ T ebar_term(const AdjArgs<T>& g, long r, int i, int j, int k, int c, int t) {
+    if (r < 0) return T(0);
+    const StepArgs<T>& a = g.f;
+    const int d = kEterm[c][t][0], sg = kEterm[c][t][1], sl = kEterm[c][t][3];
+    const T u = inv_eps_of(a, r, i, j, k, c) * g.A[c][r];
+    const T et = T(sg) * a.cdt * u;  // adjoint of the corrected difference
+    if (!pml_at(a, d, i, j, k)) return et;
+    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
+    const int cc = d == 0 ? i : (d == 1 ? j : k);
+    const T b = a.coef[a.coff[d] + cc];
+    const T pb = g.Apsi[d][sl * a.S[d] + slab_idx(a, d, i, j, k)];
+    (void)n;
+    return b * et + (b - T(1)) * pb;
+}
+
+// dbar of H-term (h, t) at cell r: from Hbar (already the total at the old time), the H
+// auxiliary bar (old value, read pair) and bH
+template <typename T>
+__device__ __forceinline__ T dbar_term(const AdjArgs<T>& g, long r, int i, int j, int k, int h, int t) {
+    if (r < 0) return T(0);
+    const StepArgs<T>& a = g.f;
+    const int d = kHterm[h][t][0], sg = kHterm[h][t][1], sl = kHterm[h][t][3];
+    const T dt_ = -T(sg) * a.cdt * g.A[3 + h][r];
+    if (!pml_at(a, d, i, j, k)) return dt_;
+    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
+    const int cc = d == 0 ? i : (d == 1 ? j : k);
+    const T b = a.coef[a.coff[d] + 2 * n + cc];
+    const T pb = g.Apsi[d][(g.hr + sl) * a.S[d] + slab_idx(a, d, i, j, k)];
+    return b * dt_ + (b - T(1)) * pb;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_h(const AdjArgs<T> g) {
+    const StepArgs<T>& a = g.f;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.N) return;
+    int i, j, k;
+    cell_of(q, a.ny, a.nz, i, j, k);
+    const int n[3] = {a.nx, a.ny, a.nz};
+    T tot[3];
+#pragma unroll
+    for (int h = 0; h < 3; ++h) tot[h] = g.A[3 + h][q];
+    // Hbar'_total(q) += ebar(q) - ebar(q + e_d) for every E-term that differentiates H_h along d
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int d = kEterm[c][t][0], h = kEterm[c][t][2] - 3;
+            tot[h] += ebar_term(g, q, i, j, k, c, t);
+            int c3[3] = {i, j, k};
+            c3[d] += 1;
+            const long r = neighbour(a, i, j, k, d, +1);
+            if (r >= 0) {
+                c3[d] = wrap_or_ghost(c3[d], n[d], d == 0 ? a.px : (d == 1 ? a.py : a.pz));
+                tot[h] -= ebar_term(g, r, c3[0], c3[1], c3[2], c, t);
+            }
+        }
+    // new H auxiliary bars (write pair): psibar = b (dbar~ + psibar')
+#pragma unroll
+    for (int h = 0; h < 3; ++h)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int d = kHterm[h][t][0], sg = kHterm[h][t][1], sl = kHterm[h][t][3];
+            if (!pml_at(a, d, i, j, k)) continue;
+            const int cc = d == 0 ? i : (d == 1 ? j : k);
+            const T b = a.coef[a.coff[d] + 2 * n[d] + cc];
+            const long s = slab_idx(a, d, i, j, k);
+            const T dt_ = -T(sg) * a.cdt * tot[h];
+            g.Apsi[d][(g.hw + sl) * a.S[d] + s] = b * (dt_ + g.Apsi[d][(g.hr + sl) * a.S[d] + s]);
+        }
+#pragma unroll
+    for (int h = 0; h < 3; ++h) g.A[3 + h][q] = tot[h];
+}
+
+template <typename T, int NP>
+__global__ void __launch_bounds__(256) k_adj_e(const AdjArgs<T> g) {
+    const StepArgs<T>& a = g.f;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.N) return;
+    int i, j, k;
+    cell_of(q, a.ny, a.nz, i, j, k);
+    const int n[3] = {a.nx, a.ny, a.nz};
+    T out[3], u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        u[c] = inv_eps_of(a, q, i, j, k, c) * g.A[c][q];
+        out[c] = g.A[c][q];
+    }
+    // Ebar(q) += dbar(q - e_d) - dbar(q) for every H-term that differentiates E_c along d
+#pragma unroll
+    for (int h = 0; h < 3; ++h)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int d = kHterm[h][t][0], c = kHterm[h][t][2];
+            out[c] -= dbar_term(g, q, i, j, k, h, t);
+            const long r = neighbour(a, i, j, k, d, -1);
+            if (r >= 0) {
+                int c3[3] = {i, j, k};
+                c3[d] = wrap_or_ghost(c3[d] - 1, n[d], d == 0 ? a.px : (d == 1 ? a.py : a.pz));
+                out[c] += dbar_term(g, r, c3[0], c3[1], c3[2], h, t);
+            }
+        }
+    // pole recursions: qbar = Qbar' + Pbar' - u; Qbar = alpha qbar; Pbar = Pbar' + beta qbar;
+    // Ebar += c3 qbar
+    if (NP > 0 && !a.uniform) {
+        const uint32_t mw = a.matw[q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int m = (mw >> (8 * c)) & 0xff;
+            if (m == kDesignMat) continue;
+            const MatLut<T>& L = a.lut[m];
+#pragma unroll
+            for (int s = 0; s < NP; ++s) {
+                if (s >= L.np) continue;
+                const long o = (long)(s * 3 + c) * a.N + q;
+                const bool lor = (L.lorentz_mask >> s) & 1;
+                const T pb = lor ? g.Ap[o] : T(0);
+                const T qb = g.Aq[o] + pb - u[c];
+                g.Aq[o] = L.alpha[s] * qb;
+                if (lor) g.Ap[o] = pb + L.beta[s] * qb;
+                out[c] += L.c3[s] * qb;
+            }
+        }
+    }
+    // new E auxiliary bars (in place): psibar = b (ebar~ + psibar')
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int d = kEterm[c][t][0], sg = kEterm[c][t][1], sl = kEterm[c][t][3];
+            if (!pml_at(a, d, i, j, k)) continue;
+            const int cc = d == 0 ? i : (d == 1 ? j : k);
+            const T b = a.coef[a.coff[d] + cc];
+            const long s = slab_idx(a, d, i, j, k);
+            T& pb = g.Apsi[d][sl * a.S[d] + s];
+            pb = b * (T(sg) * a.cdt * u[c] + pb);
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g.A[c][q] = out[c];
+}
+
+// record E of the design box and of the objective cell into slot `slot` of a ring buffer
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_record(const T* E, long N, int ny, int nz, int dlo0, int dlo1,
+                                                    int dlo2, int de0, int de1, int de2, long gap,
+                                                    T* ring, long slot) {
+    const long nd = (long)de0 * de1 * de2;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    T* out = ring + slot * (3 * nd + 3);
+    if (v < nd) {
+        const int kk = (int)(v % de2);
+        const long r = v / de2;
+        const int jj = (int)(r % de1), ii = (int)(r / de1);
+        const long q = ((long)(dlo0 + ii) * ny + (dlo1 + jj)) * nz + (dlo2 + kk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c * nd + v] = E[(long)c * N + q];
+    } else if (v < nd + 3) {
+        const int c = (int)(v - nd);
+        out[3 * nd + c] = E[(long)c * N + gap];
+    }
+}
+
+// objective J += dt * sum_c E_c(gap)^2 for the state after a step (FP64)
+template <typename T>
+__global__ void k_adj_objective(const T* E, long N, long gap, double dt, double* J) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int c = 0; c < 3; ++c) {
+            const double v = (double)E[(long)c * N + gap];
+            s += v * v;
+        }
+        *J += s * dt;
+    }
+}
+
+// gradient term of step n and injection of dg/dE^n:
+//   gacc[v] += sum_c Ebar_c^{n+1}(v) (E_c^{n+1}(v) - E_c^n(v))   (before the adjoint step)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_grad(const AdjArgs<T> g, const T* ring, long s_next, long s_cur,
+                                                  double* gacc) {
+    const StepArgs<T>& a = g.f;
+    const long nd = (long)(a.dhi[0] - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) * (a.dhi[2] - a.dlo[2]);
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nd) return;
+    const int de1 = a.dhi[1] - a.dlo[1], de2 = a.dhi[2] - a.dlo[2];
+    const int kk = (int)(v % de2);
+    const long r = v / de2;
+    const int jj = (int)(r % de1), ii = (int)(r / de1);
+    const long q = ((long)(a.dlo[0] + ii) * a.ny + (a.dlo[1] + jj)) * a.nz + (a.dlo[2] + kk);
+    const T* e1 = ring + s_next * (3 * nd + 3);
+    const T* e0 = ring + s_cur * (3 * nd + 3);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        acc += (double)g.A[c][q] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
+    gacc[v] += acc;
+}
+
+// Ebar^n(gap) += dJ/dE^n = 2 dt E^n(gap)
+template <typename T>
+__global__ void k_adj_inject(T* const* A, const T* ring, long nd, long slot, long gap, double dt) {
+    const int c = threadIdx.x;
+    if (blockIdx.x == 0 && c < 3) {
+        const double e = (double)ring[slot * (3 * nd + 3) + 3 * nd + c];
+        A[c][gap] = (T)((double)A[c][gap] + 2.0 * dt * e);
+    }
+}
+
 // ---- plain streaming copy (achievable-bandwidth probe and counter calibration) -----------
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                               long n) {

@@ -242,6 +242,15 @@ class Engine:
     def set_tile(self, ty: int, xchunk: int):
         self._chk(self.lib.sjf_set_tile(self.h, int(ty), int(xchunk)), "sjf_set_tile")
 
+    def adjoint_run(self, nsteps: int, checkpoint_every: int, gap, n_design: int):
+        """Forward run of `nsteps` steps plus the discrete adjoint: returns (J, dJ/d eps) with
+        J = dt * sum_n |E^n(gap)|^2 and the gradient over the design voxels (z fastest)."""
+        J = C.c_double()
+        grad = np.zeros(int(n_design))
+        self._chk(self.lib.sjf_adjoint_run(self.h, int(nsteps), int(checkpoint_every), _ip(gap),
+                                           C.byref(J), _dp(grad), grad.size), "sjf_adjoint_run")
+        return J.value, grad
+
     def set_graph(self, chunk: int):
         """Steps per replayed hipGraph (0: launch every step directly)."""
         self._chk(self.lib.sjf_set_graph(self.h, int(chunk)), "sjf_set_graph")

@@ -1,0 +1,109 @@
+"""Adjoint gradients (sjf_adjoint_run) against central finite differences of the FP64 oracle.
+
+J(eps) = dt * sum_{n=1..N} |E^n(gap)|^2; the engine returns dJ/d eps for every design voxel from
+the discrete adjoint.  Three cases separate the parts of the transposed step: Yee curls only,
+with CPML, and with Drude/Lorentz media.  The oracle computes J independently (D/E split, its
+own PML), so agreement pins both the adjoint algebra and the forward model it transposes.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from sim_juncs_amd import materials as M
+from sim_juncs_amd import scenario
+from sim_juncs_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def design_case(kind, precision="fp64", steps=160, seed=4):
+    rng = np.random.default_rng(seed)
+    shape = (14, 12, 20)
+    dx = 0.02
+    mat = np.zeros(shape + (3,), np.uint8)
+    lut = [M.VACUUM]
+    pml = None
+    if kind in ("pml", "poles"):
+        pml = ([3, 3, 3, 3, 3, 3], 1e-15, 2.0)
+    if kind == "poles":
+        lut = [M.VACUUM, M.AU.scaled(1.0), M.SIO2.scaled(1.0)]
+        scenario.paint(mat, shape, scenario.box_inside((-1, -1, 12.5), (5.5, 13, 15.5)), 1)
+        scenario.paint(mat, shape, scenario.box_inside((8.5, -1, 12.5), (15, 13, 15.5)), 1)
+        scenario.paint(mat, shape, scenario.box_inside((-1, -1, 15.5), (15, 13, 21)), 2)
+    lo, hi = (5, 4, 9), (9, 8, 12)
+    eps = rng.uniform(1, 4, size=tuple(h - l for l, h in zip(lo, hi)))
+    scn = scenario.Scenario(name=f"adj_{kind}", shape=shape, dx=dx, precision=precision,
+                            matw=scenario.pack(mat), lut=lut, pml=pml, steps=steps,
+                            design=(lo, hi, eps))
+    scn.sources = [scenario.plane_source(shape, dx, 4, scenario.gaussian_pulse(0.8, 1.5, 0.4, start_fs=0.0))]
+    gap = (7, 6, 13)
+    return scn, gap, eps
+
+
+def oracle_J(scn, gap, eps):
+    s = scenario.Scenario(**{**scn.__dict__})
+    s.design = (scn.design[0], scn.design[1], eps)
+    dx = scn.dx
+    o = s.make(Oracle)
+    off = scenario.COMP_OFF
+    pts = [[(gap[d] + off[c][d]) * dx for d in range(3)] for c in range(3)]
+    for c in range(3):
+        o.add_monitors(c, np.array([pts[c]]))
+    o.set_save_span(1)
+    o.step(scn.steps)
+    smp = o.read_monitors().real  # E^0 .. E^{N-1}
+    last = np.array([o.read_field(c)[gap] for c in range(3)])
+    e = np.vstack([smp[1:], last[None, :]])
+    return o.dt * float((e ** 2).sum())
+
+
+@pytest.mark.parametrize("kind", ["vacuum", "pml", "poles"])
+def test_adjoint_gradient_matches_oracle_finite_differences(kind):
+    scn, gap, eps = design_case(kind)
+    print(f"[case] adjoint {kind}", flush=True)
+    eng = scn.make(Engine)
+    J, g = eng.adjoint_run(scn.steps, 37, gap, eps.size)
+    eng.close()
+    J_o = oracle_J(scn, gap, eps)
+    assert J > 0
+    assert abs(J - J_o) / J_o < 1e-10
+    rng = np.random.default_rng(1)
+    flat = eps.reshape(-1)
+    idx = rng.choice(flat.size, size=8, replace=False)
+    for v in idx:
+        h = 1e-4 * flat[v]
+        ep, em = flat.copy(), flat.copy()
+        ep[v] += h
+        em[v] -= h
+        fd = (oracle_J(scn, gap, ep.reshape(eps.shape)) - oracle_J(scn, gap, em.reshape(eps.shape))) / (2 * h)
+        assert abs(g[v] - fd) <= 1e-6 * np.abs(g).max() + 1e-9 * abs(fd), (kind, v, g[v], fd)
+
+
+def test_adjoint_independent_of_checkpoint_interval_and_leaves_final_state():
+    scn, gap, eps = design_case("poles", steps=90)
+    res = []
+    for K in (1, 13, 90):
+        eng = scn.make(Engine)
+        J, g = eng.adjoint_run(scn.steps, K, gap, eps.size)
+        assert eng.step_count() == scn.steps
+        res.append((J, g, eng.read_field("Ex")))
+        eng.close()
+    ref = scn.make(Engine)
+    ref.step(scn.steps)
+    for J, g, ex in res:
+        assert J == res[0][0]
+        assert np.array_equal(g, res[0][1])
+        assert np.array_equal(ex, ref.read_field("Ex"))
+    ref.close()
+
+
+def test_adjoint_fp32_close_to_fp64():
+    scn, gap, eps = design_case("poles")
+    e64 = scn.make(Engine, precision="fp64")
+    J64, g64 = e64.adjoint_run(scn.steps, 40, gap, eps.size)
+    e32 = scn.make(Engine, precision="fp32")
+    J32, g32 = e32.adjoint_run(scn.steps, 40, gap, eps.size)
+    e64.close()
+    e32.close()
+    assert abs(J32 - J64) / J64 < 1e-4
+    assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 1e-3
