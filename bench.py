@@ -1,13 +1,17 @@
-"""Throughput of the fused FDTD step on one MI355X.
+"""Throughput of the FDTD time step on one MI355X.
 
-Workload (default): the single large junction instance -- 512x512x256 cells of 5 nm, 12-cell
+Default workload: the single large junction instance -- 512x512x256 cells of 5 nm, 12-cell
 PML, Au bowtie (Drude + 2 Lorentz) on a SiO2 Lorentz substrate with a graphene Drude sheet,
-FP32 fields, FP64 DFT accumulators at 4 points x 8 frequencies, Gaussian 800 nm pulse.
-A "step" is one FDTD time step over the whole grid (fused update kernel + source + DFT +
-monitor sampling every 20 steps).  Fields are resident in HBM before timing starts.
+FP32 fields, FP64 compensated DFT accumulators at 4 points x 8 frequencies, 800 nm Gaussian
+pulse, 6000 steps.  A "step" is one time step over the whole grid (fused update kernel +
+source + DFT + monitor sampling every 20 steps).  Fields are resident in HBM before timing.
 
-Prints one JSON line (metric: Yee cell-updates/s, cells including PML x steps / wall time),
-with the dominant kernel's HBM roofline and the FP64 CPU oracle timed on this host's cores.
+Other workloads (--workload): plain (96^3 FP64), sweep (64 uneven instances batched),
+adjoint (256x256x128 with a 64x64x16 design region: forward + discrete adjoint), large
+(1024x768x512 FP32 plain run), variants (16 adjoint design variants).
+
+Prints one JSON line: metric Yee cell-updates/s (cells including PML x steps / wall time),
+the dominant kernel's HBM roofline, and the FP64 CPU oracle timed on this host's cores.
 """
 from __future__ import annotations
 
@@ -26,88 +30,178 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=6000)
+    p.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     p.add_argument("--warmup", type=int, default=50)
-    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain"])
-    p.add_argument("--cpu-steps", type=int, default=6)
+    p.add_argument("--workload", default="bowtie",
+                   choices=["bowtie", "plain", "sweep", "adjoint", "large", "variants"])
+    p.add_argument("--cpu-steps", type=int, default=None,
+                   help="oracle steps timed for the CPU baseline (default: about 10 s of work)")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--kernel-steps", type=int, default=200,
+    p.add_argument("--kernel-steps", type=int, default=100,
                    help="steps timed with per-launch HIP events for the kernel duration")
     return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scns, steps):
+    """FP64 oracle on this host's cores over `steps` steps of the given instances."""
+    from oracle import oracle as orc
+
+    cells = 0
+    secs = 0.0
+    for s in scns:
+        o = s.make(orc.Oracle)
+        o.step(1)
+        t = time.perf_counter()
+        o.step(steps)
+        secs += time.perf_counter() - t
+        cells += s.cells
+        o.close()
+    return {"value": cells * steps / secs, "unit": "cell-updates/s", "cores": orc.num_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle (build's CPU stepper, not meep) on {len(scns)} instance(s) of the "
+                      f"same workload, {steps} steps each after 1 warm-up step, {secs:.1f} s"}
+
+
+def roofline(bytes_step, ms_kernel):
+    achieved = bytes_step / (ms_kernel * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step",
+            "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step}
+
+
+def single(a, scn, tag):
+    from sim_juncs_amd import autotune
+    from sim_juncs_amd.engine import Engine
+
+    steps = a.steps if a.steps is not None else scn.steps
+    eng = scn.make(Engine)
+    autotune.apply_cached(eng)
+    log(f"engine ready {eng.launch_info()}")
+    eng.step(a.warmup)
+    eng.synchronize()
+    t = time.perf_counter()
+    eng.step(steps)
+    eng.synchronize()
+    wall = time.perf_counter() - t
+    log(f"timed {steps} steps in {wall:.3f}s")
+    _, ms_kernel = eng.timed_steps(a.kernel_steps)
+    rl = roofline(eng.step_bytes(), ms_kernel)
+    info = eng.launch_info()
+    eng.close()
+    cpu = None
+    if not a.no_cpu:
+        n_cpu = a.cpu_steps or max(3, min(steps, int(7e9 / scn.cells)))
+        cpu = cpu_baseline([scn], n_cpu)
+    return dict(value=scn.cells * steps / wall, steps=steps, wall=wall,
+                config={"workload": scn.name, "cells": scn.cells, "precision": scn.precision,
+                        "dft": "fp64 compensated", "tile": info},
+                roofline=rl, cpu=cpu, dtype="f32" if scn.precision == "fp32" else "f64")
+
+
+def sweep(a):
+    from sim_juncs_amd import scenario
+    from sim_juncs_amd.engine import Batch, Engine
+
+    scns = scenario.batched_sweep()
+    steps = a.steps if a.steps is not None else scns[0].steps
+    engs = [s.make(Engine) for s in scns]
+    b = Batch(engs)
+    cells, bytes_step = b.info()
+    log(f"batch of {len(engs)} engines, {cells} cells")
+    b.step(a.warmup)
+    b.synchronize()
+    t = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t
+    _, ms_kernel = b.timed_steps(a.kernel_steps)
+    b.close()
+    for e in engs:
+        e.close()
+    cpu = None if a.no_cpu else cpu_baseline(scns[:4], a.cpu_steps or 20)
+    return dict(value=cells * steps / wall, steps=steps, wall=wall,
+                config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
+                        "launch": "one fused launch per step over all instances, hipGraph replay"},
+                roofline=roofline(bytes_step, ms_kernel), cpu=cpu, dtype="f32")
+
+
+def adjoint(a, variants=False):
+    from sim_juncs_amd import scenario
+    from sim_juncs_amd.engine import Engine
+
+    scns = scenario.adjoint_variants() if variants else [scenario.adjoint_junction()]
+    steps = a.steps if a.steps is not None else scns[0].steps
+    cells = 0
+    wall = 0.0
+    for s in scns:
+        eng = s.make(Engine)
+        nd = s.design[2].size
+        t = time.perf_counter()
+        J, g = eng.adjoint_run(steps, 100, s.meta["gap"], nd)
+        wall += time.perf_counter() - t
+        cells += s.cells
+        log(f"{s.name}: J={J:.4e} |grad|={float((g * g).sum()) ** 0.5:.4e}")
+        eng.close()
+    s0 = scns[0]
+    eng = s0.make(Engine)
+    eng.step(a.warmup)
+    _, ms_kernel = eng.timed_steps(a.kernel_steps)
+    rl = roofline(eng.step_bytes(), ms_kernel)
+    eng.close()
+    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or 60)
+    if cpu:
+        # the adjoint costs a forward, a recomputation and a reverse sweep per step
+        cpu["value"] /= 3.0
+        cpu["sample"] += "; divided by 3 for forward + recompute + reverse sweeps"
+    return dict(value=cells * steps / wall, steps=steps, wall=wall,
+                config={"workload": ("variants_" if variants else "") + s0.name, "cells": cells,
+                        "precision": "fp32", "checkpoint_every": 100,
+                        "note": "value counts forward steps per second of a full gradient computation"},
+                roofline=rl, cpu=cpu, dtype="f32")
 
 
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
-                                   "(instances shard across GPUs without exchange, see DESIGN)"}))
+                                   "(instances shard across GPUs without exchange)"}))
         sys.exit(2)
-    import numpy as np  # noqa: F401
-
     from sim_juncs_amd import scenario
-    from sim_juncs_amd.engine import Engine
 
     t0 = time.time()
     if a.workload == "bowtie":
-        scn = scenario.bowtie_junction(steps=a.steps)
+        r = single(a, scenario.bowtie_junction(), "bowtie")
+    elif a.workload == "plain":
+        r = single(a, scenario.plain_box(), "plain")
+    elif a.workload == "large":
+        r = single(a, scenario.large_junction(), "large")
+    elif a.workload == "sweep":
+        r = sweep(a)
+    elif a.workload == "adjoint":
+        r = adjoint(a)
     else:
-        scn = scenario.plain_box(steps=a.steps, precision="fp64")
-    print(f"[bench] scenario {scn.name} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    eng = scn.make(Engine)
-    print(f"[bench] engine ready {eng.launch_info()}", file=sys.stderr, flush=True)
-    eng.step(a.warmup)
-    eng.synchronize()
-    t1 = time.perf_counter()
-    eng.step(a.steps)
-    eng.synchronize()
-    wall = time.perf_counter() - t1
-    cells = scn.cells
-    value = cells * a.steps / wall
-    print(f"[bench] timed {a.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
-
-    ms_tot, ms_kernel = eng.timed_steps(a.kernel_steps)
-    bytes_step = eng.step_bytes()
-    achieved = bytes_step / (ms_kernel * 1e-3) / 1e9
-    info = eng.launch_info()
-    eng.close()
-
-    cpu = None
-    if not a.no_cpu:
-        from oracle import oracle as orc
-
-        t2 = time.time()
-        o = scn.make(orc.Oracle)
-        print(f"[bench] oracle set up in {time.time() - t2:.1f}s", file=sys.stderr, flush=True)
-        o.step(1)
-        t3 = time.perf_counter()
-        o.step(a.cpu_steps)
-        cpu_s = time.perf_counter() - t3
-        o.close()
-        cpu = {"value": cells * a.cpu_steps / cpu_s, "unit": "cell-updates/s",
-               "cores": orc.num_threads(), "kind": "port",
-               "sample": f"FP64 oracle, same {scn.shape} grid and materials, {a.cpu_steps} steps "
-                         f"after 1 warm-up step ({cpu_s:.1f}s)"}
-
+        r = adjoint(a, variants=True)
+    log(f"done in {time.time() - t0:.1f}s")
     out = {
         "metric": "Yee cell-updates/s",
-        "value": value,
+        "value": r["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": r["steps"],
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": r["wall"] / r["steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if scn.precision == "fp32" else "f64",
+        "dtype": r["dtype"],
         "data": "synthetic",
-        "config": {"workload": scn.name, "cells": cells, "precision": scn.precision,
-                   "dft": "fp64 compensated", "tile": info},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step", "kernel_ms": ms_kernel,
-                     "algorithmic_bytes_per_launch": bytes_step},
-        "cpu_baseline": cpu,
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu"],
     }
     print(json.dumps(out))
 

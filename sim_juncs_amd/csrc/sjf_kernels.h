@@ -74,8 +74,10 @@ struct StepArgs {
     const T* dinv;
 };
 
+#define GLOBAL __attribute__((address_space(1)))
+
 template <typename T>
-__device__ __forceinline__ T ldg(const T* p, long q, bool ok) {
+__device__ __forceinline__ T ldg(GLOBAL const T* p, long q, bool ok) {
     return ok ? p[q] : T(0);
 }
 
@@ -133,11 +135,25 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     T* sh = reinterpret_cast<T*>(smem_raw);  // [2 parity][2 comp][TY][B]
     MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sh + 4 * TY * B);
 
-    for (int q = threadIdx.x; q < a.nmat; q += B) slut[q] = a.lut[q];
+    // The argument table lives in global memory, so its pointers arrive as generic pointers;
+    // casting them to the global address space keeps every access a global_load/store (flat
+    // accesses would also count against the LDS wait counter and stall the barrier).
+    {
+        GLOBAL const uint32_t* src = (GLOBAL const uint32_t*)a.lut;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(slut);
+        const int words = (int)(sizeof(MatLut<T>) / 4) * a.nmat;
+        for (int q = threadIdx.x; q < words; q += B) dst[q] = src[q];
+    }
 
-    const long step = *a.d_step;
-    const T* F0 = a.F[step & 1];
-    T* F1 = a.F[(step + 1) & 1];
+    const long step = *(GLOBAL const long*)a.d_step;
+    GLOBAL const T* F0 = (GLOBAL const T*)a.F[step & 1];
+    GLOBAL T* F1 = (GLOBAL T*)a.F[(step + 1) & 1];
+    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
+    GLOBAL T* Qg = (GLOBAL T*)a.Q;
+    GLOBAL T* Pg = (GLOBAL T*)a.P;
+    GLOBAL T* psig[3] = {(GLOBAL T*)a.psi[0], (GLOBAL T*)a.psi[1], (GLOBAL T*)a.psi[2]};
+    GLOBAL const T* coef = (GLOBAL const T*)a.coef;
+    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
     const int hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const int hw = (step & 1) ? 2 : 4;  // and written
     const int tyi = t % a.nty, xci = t / a.nty;
@@ -164,7 +180,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     for (int r = 0; r < TY + 1; ++r) ypml[r] = gj[r] >= 0 && in_pml(a, 1, gj[r], a.ny);
 
     const bool zpml = kin && in_pml(a, 2, k, a.nz);
-    const T* cz = a.coef + a.coff[2];
+    GLOBAL const T* cz = coef + a.coff[2];
     const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[a.nz + k] : T(0);
     const T bHz = zpml ? cz[2 * a.nz + k] : T(0), aHz = zpml ? cz[3 * a.nz + k] : T(0);
 
@@ -195,7 +211,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         for (int r = 0; r < TY; ++r) {
             hp[r][0] = hp[r][1] = T(0);
             const long q0 = (long)p0 * NYZ + (long)gj[r + 1] * NZ + k;
-            mw[r] = (!uniform && own_row[r] && istart >= x0) ? a.matw[q0] : 0u;
+            mw[r] = (!uniform && own_row[r] && istart >= x0) ? matw[q0] : 0u;
         }
     }
 
@@ -206,7 +222,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         const int pn = wrap_or_ghost(i + 1, a.nx, a.px);
         const int pnn = wrap_or_ghost(i + 2, a.nx, a.px);
         const bool xpml = in_pml(a, 0, pi, a.nx);
-        const T* cx = a.coef + a.coff[0];
+        GLOBAL const T* cx = coef + a.coff[0];
         const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[a.nx + pi] : T(0);
         const T bHx = xpml ? cx[2 * a.nx + pi] : T(0), aHx = xpml ? cx[3 * a.nx + pi] : T(0);
 
@@ -224,7 +240,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
             const long q = (long)pn * NYZ + (long)gj[r + 1] * NZ + k;
-            mwn[r] = (!uniform && has_next && own_row[r] && pn >= 0) ? a.matw[q] : 0u;
+            mwn[r] = (!uniform && has_next && own_row[r] && pn >= 0) ? matw[q] : 0u;
         }
         T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
 #pragma unroll
@@ -243,18 +259,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
             if (ok && xpml) {
                 const long s = slab_x(a, pi, j, k);
-                pH[r][0][0] = a.psi[0][(hr + 0) * a.S[0] + s];
-                pH[r][0][1] = a.psi[0][(hr + 1) * a.S[0] + s];
+                pH[r][0][0] = psig[0][(hr + 0) * a.S[0] + s];
+                pH[r][0][1] = psig[0][(hr + 1) * a.S[0] + s];
             }
             if (ok && ypml[r]) {
                 const long s = slab_y(a, pi, j, k);
-                pH[r][1][0] = a.psi[1][(hr + 0) * a.S[1] + s];
-                pH[r][1][1] = a.psi[1][(hr + 1) * a.S[1] + s];
+                pH[r][1][0] = psig[1][(hr + 0) * a.S[1] + s];
+                pH[r][1][1] = psig[1][(hr + 1) * a.S[1] + s];
             }
             if (ok && zpml) {
                 const long s = slab_z(a, pi, j, k);
-                pH[r][2][0] = a.psi[2][(hr + 0) * a.S[2] + s];
-                pH[r][2][1] = a.psi[2][(hr + 1) * a.S[2] + s];
+                pH[r][2][0] = psig[2][(hr + 0) * a.S[2] + s];
+                pH[r][2][1] = psig[2][(hr + 1) * a.S[2] + s];
             }
         }
         T pE[TY][3][2];
@@ -272,18 +288,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             const long q = (long)pi * NYZ + (long)j * NZ + k;
             if (xpml) {
                 const long s = slab_x(a, pi, j, k);
-                pE[r][0][0] = a.psi[0][0 * a.S[0] + s];
-                pE[r][0][1] = a.psi[0][1 * a.S[0] + s];
+                pE[r][0][0] = psig[0][0 * a.S[0] + s];
+                pE[r][0][1] = psig[0][1 * a.S[0] + s];
             }
             if (ypml[r + 1]) {
                 const long s = slab_y(a, pi, j, k);
-                pE[r][1][0] = a.psi[1][0 * a.S[1] + s];
-                pE[r][1][1] = a.psi[1][1 * a.S[1] + s];
+                pE[r][1][0] = psig[1][0 * a.S[1] + s];
+                pE[r][1][1] = psig[1][1 * a.S[1] + s];
             }
             if (zpml) {
                 const long s = slab_z(a, pi, j, k);
-                pE[r][2][0] = a.psi[2][0 * a.S[2] + s];
-                pE[r][2][1] = a.psi[2][1 * a.S[2] + s];
+                pE[r][2][0] = psig[2][0 * a.S[2] + s];
+                pE[r][2][1] = psig[2][1 * a.S[2] + s];
             }
             if (NP > 0 && !uniform) {
 #pragma unroll
@@ -294,8 +310,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
                         if (s < np) {
-                            Qv[r][c][s] = a.Q[(s * 3 + c) * a.N + q];
-                            if ((lm >> s) & 1) Pv[r][c][s] = a.P[(s * 3 + c) * a.N + q];
+                            Qv[r][c][s] = Qg[(s * 3 + c) * a.N + q];
+                            if ((lm >> s) & 1) Pv[r][c][s] = Pg[(s * 3 + c) * a.N + q];
                         }
                     }
                 }
@@ -317,14 +333,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             T dEz_x = en[r][2] - ec[r][2];
             T dEy_x = en[r][1] - ec[r][1];
             if (ok && ypml[r]) {
-                const T* cy = a.coef + a.coff[1];
+                GLOBAL const T* cy = coef + a.coff[1];
                 const T b = cy[2 * a.ny + j], c = cy[3 * a.ny + j];
                 dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
                 dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
                 if (wr) {
                     const long s = slab_y(a, pi, j, k);
-                    a.psi[1][(hw + 0) * a.S[1] + s] = pH[r][1][0];
-                    a.psi[1][(hw + 1) * a.S[1] + s] = pH[r][1][1];
+                    psig[1][(hw + 0) * a.S[1] + s] = pH[r][1][0];
+                    psig[1][(hw + 1) * a.S[1] + s] = pH[r][1][1];
                 }
             }
             if (ok && zpml) {
@@ -332,8 +348,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                 if (wr) {
                     const long s = slab_z(a, pi, j, k);
-                    a.psi[2][(hw + 0) * a.S[2] + s] = pH[r][2][0];
-                    a.psi[2][(hw + 1) * a.S[2] + s] = pH[r][2][1];
+                    psig[2][(hw + 0) * a.S[2] + s] = pH[r][2][0];
+                    psig[2][(hw + 1) * a.S[2] + s] = pH[r][2][1];
                 }
             }
             if (ok && xpml) {
@@ -341,8 +357,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
                 if (wr) {
                     const long s = slab_x(a, pi, j, k);
-                    a.psi[0][(hw + 0) * a.S[0] + s] = pH[r][0][0];
-                    a.psi[0][(hw + 1) * a.S[0] + s] = pH[r][0][1];
+                    psig[0][(hw + 0) * a.S[0] + s] = pH[r][0][0];
+                    psig[0][(hw + 1) * a.S[0] + s] = pH[r][0][1];
                 }
             }
             // outside a metallic edge H is identically zero (ghost row of the halo)
@@ -380,27 +396,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 T dHz_x = h[r + 1][2] - hp[r][1];
                 T dHy_x = h[r + 1][1] - hp[r][0];
                 if (ypml[r + 1]) {
-                    const T* cy = a.coef + a.coff[1];
+                    GLOBAL const T* cy = coef + a.coff[1];
                     const T b = cy[j], c = cy[a.ny + j];
                     dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
                     dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
                     const long s = slab_y(a, pi, j, k);
-                    a.psi[1][0 * a.S[1] + s] = pE[r][1][0];
-                    a.psi[1][1 * a.S[1] + s] = pE[r][1][1];
+                    psig[1][0 * a.S[1] + s] = pE[r][1][0];
+                    psig[1][1 * a.S[1] + s] = pE[r][1][1];
                 }
                 if (zpml) {
                     dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
                     dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
                     const long s = slab_z(a, pi, j, k);
-                    a.psi[2][0 * a.S[2] + s] = pE[r][2][0];
-                    a.psi[2][1 * a.S[2] + s] = pE[r][2][1];
+                    psig[2][0 * a.S[2] + s] = pE[r][2][0];
+                    psig[2][1 * a.S[2] + s] = pE[r][2][1];
                 }
                 if (xpml) {
                     dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
                     dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
                     const long s = slab_x(a, pi, j, k);
-                    a.psi[0][0 * a.S[0] + s] = pE[r][0][0];
-                    a.psi[0][1 * a.S[0] + s] = pE[r][0][1];
+                    psig[0][0 * a.S[0] + s] = pE[r][0][0];
+                    psig[0][1 * a.S[0] + s] = pE[r][0][1];
                 }
                 const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
                 if (uniform) {
@@ -417,7 +433,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                             const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) +
                                              (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
                                             (k - a.dlo[2]);
-                            inv = a.dinv[dq];
+                            inv = dinv[dq];
                         } else {
                             const MatLut<T>& L = slut[m];
                             inv = L.inv_eps;
@@ -427,9 +443,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                                     T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * e0;
                                     if ((L.lorentz_mask >> s) & 1) {
                                         qn += L.beta[s] * Pv[r][c][s];
-                                        a.P[(s * 3 + c) * a.N + q] = Pv[r][c][s] + qn;
+                                        Pg[(s * 3 + c) * a.N + q] = Pv[r][c][s] + qn;
                                     }
-                                    a.Q[(s * 3 + c) * a.N + q] = qn;
+                                    Qg[(s * 3 + c) * a.N + q] = qn;
                                     sq += qn;
                                 }
                             }

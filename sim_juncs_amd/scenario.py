@@ -275,3 +275,53 @@ def batched_sweep(n=64, seed=3, steps=3000, precision="fp32", scale=1.0):
         out.append(gap_junction((nx, ny, nz), gap, kind, pml=pml, fwhm_fs=fwhm, cep=cep,
                                 steps=steps, precision=precision, name=f"sweep{i:02d}_{kind}"))
     return out
+
+
+def adjoint_junction(shape=(256, 256, 128), design=(64, 64, 16), steps=4000, seed=4,
+                     precision="fp32", pml=12, dx=0.005, fwhm_fs=10.0, name=None):
+    """Adjoint workload: a design box of per-voxel permittivity, uniform in [1, 4] (seeded),
+    between two Au (Drude + 2 Lorentz) electrodes of the same thickness on a SiO2 substrate;
+    objective J = time-integrated |E|^2 at the centre of the design box.  Full size:
+    256x256x128 cells, 64x64x16 design voxels, 4000 steps."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    mat = np.zeros(tuple(shape) + (3,), np.uint8)
+    cx, cy, cz = nx // 2, ny // 2, nz // 2
+    lo = (cx - design[0] // 2, cy - design[1] // 2, cz - design[2] // 2)
+    hi = (lo[0] + design[0], lo[1] + design[1], lo[2] + design[2])
+    paint(mat, shape, box_inside((-1, -1, hi[2] - 0.5), (nx + 1, ny + 1, nz + 1)), 1)
+    paint(mat, shape, box_inside((-1, -1, lo[2] - 0.5), (lo[0] - 0.5, ny + 1, hi[2] - 0.5)), 2)
+    paint(mat, shape, box_inside((hi[0] - 0.5, -1, lo[2] - 0.5), (nx + 1, ny + 1, hi[2] - 0.5)), 2)
+    eps = rng.uniform(1.0, 4.0, size=design)
+    lut = [M.VACUUM, M.SIO2.scaled(1.0), M.AU.scaled(1.0)]
+    pulse = gaussian_pulse(0.8, fwhm_fs, 0.0)
+    gap = (cx, cy, cz)
+    return Scenario(
+        name=name or f"adjoint_{nx}x{ny}x{nz}_d{design[0]}x{design[1]}x{design[2]}", shape=tuple(shape),
+        dx=dx, precision=precision, matw=pack(mat), lut=lut, pml=([pml] * 6, 1e-15, 2.0),
+        sources=[plane_source(shape, dx, pml + 2, pulse)], design=(lo, hi, eps), steps=steps,
+        meta=dict(gap=gap))
+
+
+def adjoint_variants(n=16, seed=5, shape=(192, 192, 96), design=(48, 48, 12), steps=3000,
+                     precision="fp32"):
+    """Batched adjoint workload: `n` design variants (eps uniform in [1, 4] on the design box,
+    CEP uniform in [0, 2 pi)) of the adjoint junction, each with its gradient."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = adjoint_junction(shape=shape, design=design, steps=steps, seed=int(rng.integers(1 << 30)),
+                             precision=precision, name=f"variant{i:02d}")
+        cep = float(rng.uniform(0, 2 * math.pi))
+        s.sources[0]["phase"] = cep
+        s.meta["cep"] = cep
+        out.append(s)
+    return out
+
+
+def large_junction(steps=10000, precision="fp32"):
+    """The large plain run: 1024x768x512 cells of the bowtie junction (Au/SiO2/graphene), sized
+    to the MI355X's 288 GB of HBM."""
+    s = bowtie_junction(nx=1024, ny=768, nz=512, steps=steps, precision=precision, pml=12)
+    s.name = "bowtie_1024x768x512"
+    return s

@@ -107,3 +107,17 @@ def test_adjoint_fp32_close_to_fp64():
     e32.close()
     assert abs(J32 - J64) / J64 < 1e-4
     assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 1e-3
+
+
+def test_torch_autograd_binding():
+    import torch
+
+    from sim_juncs_amd.adjoint import GapFieldObjective, run_adjoint
+
+    scn, gap, eps = design_case("poles", precision="fp32", steps=80)
+    t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    J = GapFieldObjective.apply(t, scn, scn.steps, 20, gap)
+    J.backward()
+    J_ref, g_ref = run_adjoint(scn, eps, scn.steps, 20, gap)
+    assert abs(J.item() - J_ref) <= 1e-12 * abs(J_ref)
+    assert np.allclose(t.grad.numpy(), g_ref, rtol=1e-12, atol=0)
