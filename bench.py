@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-steps", type=int, default=100,
                    help="steps timed with per-launch HIP events for the kernel duration")
+    p.add_argument("--tile", default=None,
+                   help="fused-kernel tile as form,rows,xchunk (default: tuned cache or engine default)")
     return p.parse_args()
 
 
@@ -79,7 +81,11 @@ def single(a, scn, tag):
 
     steps = a.steps if a.steps is not None else scn.steps
     eng = scn.make(Engine)
-    autotune.apply_cached(eng)
+    if a.tile:
+        form, rows, xc = a.tile.split(",")
+        eng.set_tile(form, int(rows), int(xc))
+    else:
+        autotune.apply_cached(eng)
     log(f"engine ready {eng.launch_info()}")
     eng.step(a.warmup)
     eng.synchronize()

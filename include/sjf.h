@@ -186,11 +186,18 @@ int sjf_write_field(sjf_engine* e, int comp, const double* in);
 /* Algorithmic HBM bytes one launch of the fused update kernel must move (E/H read+write,
  * material word, polarisation state, PML auxiliaries), computed from the actual masks. */
 int sjf_step_bytes(const sjf_engine* e, double* bytes);
-/* Launch configuration of the fused update kernel: rows per thread, x-chunk length,
- * threads per workgroup, workgroups. */
-int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int* groups);
-/* Override the autotuned tile shape (ty in {1,2,4,8}, xchunk >= 1). */
-int sjf_set_tile(sjf_engine* e, int ty, int xchunk);
+/* Forms of the fused update kernel.  REGISTER_ROWS: one workgroup per z line, each thread
+ * updates `rows` (1, 2 or 4) consecutive y rows and recomputes H on the row below them.
+ * THREAD_ROWS: a workgroup holds rows + 1 rows of nz threads and updates `rows` y rows,
+ * exchanging y and z neighbours through LDS; needs (rows + 1) * nz <= 1024 and every array
+ * addressable with 32-bit byte offsets (cells * precision < 2^32). */
+#define SJF_TILE_REGISTER_ROWS 1
+#define SJF_TILE_THREAD_ROWS 2
+/* Launch configuration of the fused update kernel: form, y rows (per thread or per
+ * workgroup), x-chunk length, threads per workgroup, workgroups. */
+int sjf_launch_info(const sjf_engine* e, int* form, int* rows, int* xchunk, int* threads, int* groups);
+/* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x). */
+int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
 
 /* ---- adjoint -----------------------------------------------------------------------------
  * New (the reference has no gradients).  Runs `nsteps` steps from step 0 and returns the

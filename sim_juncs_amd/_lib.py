@@ -108,8 +108,8 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_read_field": (C.c_int, [vp, C.c_int, dp]),
         "sjf_write_field": (C.c_int, [vp, C.c_int, dp]),
         "sjf_step_bytes": (C.c_int, [vp, dp]),
-        "sjf_launch_info": (C.c_int, [vp, ip, ip, ip, ip]),
-        "sjf_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
+        "sjf_launch_info": (C.c_int, [vp, ip, ip, ip, ip, ip]),
+        "sjf_set_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "sjf_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
         "sjf_set_graph": (C.c_int, [vp, C.c_int]),
         "sjf_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),

@@ -1,9 +1,10 @@
 """Tile-shape autotuner for the fused step kernel.
 
-The fused kernel's launch shape has two free parameters: rows per thread (register blocking
-in y, 1/2/4/8) and the x-chunk each workgroup marches over.  `tune()` times candidates on a
-live engine with HIP events (the simulation keeps advancing; timing does not depend on the
-field values) and returns the fastest.  Results are cached per (shape, precision, material
+The fused kernel's launch shape has three free parameters: its form (register rows: each
+thread updates 1/2/4 y rows; thread rows: a workgroup of rows + 1 thread rows exchanges y and z
+neighbours through LDS), the y rows, and the x-chunk each workgroup marches over.  `tune()`
+times candidates on a live engine with HIP events (the simulation keeps advancing; timing does
+not depend on the field values) and returns the fastest.  Results are cached per (shape, precision, material
 mix) in a JSON file next to this module so later runs reuse them.
 
     python -m sim_juncs_amd.autotune --workload bowtie
@@ -18,8 +19,23 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned_tiles.json")
 
-ROWS = (1, 2, 4)
-XCHUNKS = (2, 4, 8, 16, 32, 64)
+REGISTER_ROWS = (1, 2)
+XCHUNKS = (4, 8, 16, 32, 64)
+
+
+def candidates(shape, xchunks=XCHUNKS):
+    """(form, rows, xchunk) triples worth timing for a grid shape."""
+    nx, ny, nz = shape
+    out = []
+    for rows in REGISTER_ROWS:
+        if rows == 1 or rows <= ny:
+            out += [("register_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
+    max_rows = 1024 // nz - 1
+    for rows in sorted({r for r in (1, 2, 3, 5, 7, 11, 15) if r <= max_rows} | ({max_rows} if max_rows >= 1 else set())):
+        if rows > ny and rows != 1:
+            continue
+        out += [("thread_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
+    return out
 
 
 def key(shape, precision, tag="") -> str:
@@ -39,41 +55,37 @@ def lookup(shape, precision, tag=""):
 
 def apply_cached(eng, tag="") -> bool:
     t = lookup(eng.shape, eng.precision, tag)
-    if t:
-        eng.set_tile(t["ty"], t["xchunk"])
+    if t and "form" in t:
+        eng.set_tile(t["form"], t["rows"], t["xchunk"])
         return True
     return False
 
 
-def tune(eng, steps: int = 20, rows=ROWS, xchunks=XCHUNKS, verbose: bool = True):
+def tune(eng, steps: int = 20, xchunks=XCHUNKS, verbose: bool = True):
     """Time every candidate tile on `eng`; returns (best dict, all results)."""
     results = []
     eng.step(2)
-    for ty in rows:
-        if ty > eng.shape[1] and ty != 1:
-            continue
-        for xc in xchunks:
-            if xc > eng.shape[0] and xc != xchunks[0]:
-                continue
-            try:
-                eng.set_tile(ty, xc)
-                eng.timed_steps(2)
-                _, ms = eng.timed_steps(steps)
-            except Exception as exc:  # tile rejected (e.g. LDS budget)
-                if verbose:
-                    print(f"[tune] ty={ty} xc={xc}: {exc}", file=sys.stderr, flush=True)
-                continue
-            results.append(dict(ty=ty, xchunk=xc, ms=ms))
+    for form, rows, xc in candidates(eng.shape, xchunks):
+        try:
+            eng.set_tile(form, rows, xc)
+            eng.timed_steps(2)
+            _, ms = eng.timed_steps(steps)
+        except Exception as exc:  # tile rejected (e.g. LDS budget)
             if verbose:
-                print(f"[tune] ty={ty} xc={xc}: {ms:.4f} ms", file=sys.stderr, flush=True)
+                print(f"[tune] {form} rows={rows} xc={xc}: {exc}", file=sys.stderr, flush=True)
+            continue
+        results.append(dict(form=form, rows=rows, xchunk=xc, ms=ms))
+        if verbose:
+            print(f"[tune] {form} rows={rows} xc={xc}: {ms:.4f} ms", file=sys.stderr, flush=True)
     best = min(results, key=lambda r: r["ms"])
-    eng.set_tile(best["ty"], best["xchunk"])
+    eng.set_tile(best["form"], best["rows"], best["xchunk"])
     return best, results
 
 
 def store(shape, precision, best, tag=""):
     c = load_cache()
-    c[key(shape, precision, tag)] = dict(ty=best["ty"], xchunk=best["xchunk"], ms=best["ms"])
+    c[key(shape, precision, tag)] = dict(form=best["form"], rows=best["rows"], xchunk=best["xchunk"],
+                                         ms=best["ms"])
     with open(CACHE, "w") as f:
         json.dump(c, f, indent=1, sort_keys=True)
 
@@ -84,11 +96,12 @@ def main():
     from sim_juncs_amd.engine import Engine
 
     p = argparse.ArgumentParser()
-    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain"])
+    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain", "adjoint", "large"])
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--store", action="store_true")
     a = p.parse_args()
-    scn = scenario.bowtie_junction() if a.workload == "bowtie" else scenario.plain_box()
+    scn = {"bowtie": scenario.bowtie_junction, "plain": scenario.plain_box,
+           "adjoint": scenario.adjoint_junction, "large": scenario.large_junction}[a.workload]()
     eng = scn.make(Engine)
     best, res = tune(eng, steps=a.steps)
     bytes_step = eng.step_bytes()

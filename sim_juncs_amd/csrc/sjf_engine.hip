@@ -100,7 +100,8 @@ struct sjf_engine {
     double* d_samples = nullptr;
     std::vector<Dft> dft;
 
-    int ty = 1, xc = 16;
+    int form = SJF_TILE_REGISTER_ROWS;  // fused-kernel form
+    int ty = 1, xc = 16;                // y rows per thread (register form) or per workgroup
     bool tile_set = false;
     int graph_chunk = 32;  // steps per captured graph (0: no graphs)
 
@@ -221,7 +222,19 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
     return 1.0 / e->lut_h[m].eps_inf;
 }
 
-int block_threads(const sjf_engine* e) { return ((e->g.nz + 63) / 64) * 64; }
+// Row-of-threads form: (rows + 1) thread rows of nz threads per workgroup, 32-bit offsets.
+bool thread_rows_fit(const sjf_engine* e, int rows) {
+    return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024 && (double)e->N * e->prec < 4294967296.0;
+}
+
+int default_thread_rows(const sjf_engine* e) { return std::max(1, 1024 / e->g.nz - 1); }
+
+int block_threads_for(const sjf_engine* e, int form, int ty) {
+    const int t = form == SJF_TILE_THREAD_ROWS ? (ty + 1) * e->g.nz : e->g.nz;
+    return ((t + 63) / 64) * 64;
+}
+
+int block_threads(const sjf_engine* e) { return block_threads_for(e, e->form, e->ty); }
 
 int tiles_of(const sjf_engine* e, int ty) {
     const int nty = (e->g.ny + ty - 1) / ty;
@@ -242,6 +255,7 @@ template <typename T>
 sjf::StepArgs<T> step_args(const sjf_engine* e, int ty) {
     sjf::StepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
+    a.rt = ty + 1;
     a.nx = e->g.nx;
     a.ny = e->g.ny;
     a.nz = e->g.nz;
@@ -283,6 +297,7 @@ struct LaunchSet {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int prec = 4, ty = 1, npv = 0, block = 64;
+    int form = SJF_TILE_REGISTER_ROWS;
     size_t lds = 0;
     std::vector<long> versions;
     bool built = false;
@@ -357,12 +372,15 @@ int build_set_t(LaunchSet* s) {
     bool all_uniform = true;
     int t = 0;
     for (sjf_engine* e : s->eng) {
-        args.push_back(step_args<T>(e, s->ty));
+        // the engine's own set runs its tuned rows; a batch picks rows per engine for its form
+        const int rows = (s == e->own) ? e->ty
+                                       : (s->form == SJF_TILE_THREAD_ROWS ? default_thread_rows(e) : s->ty);
+        args.push_back(step_args<T>(e, rows));
         first.push_back(t);
-        t += tiles_of(e, s->ty);
+        t += tiles_of(e, rows);
         maxp = std::max(maxp, e->maxp);
         all_uniform = all_uniform && e->uniform;
-        block = std::max(block, block_threads(e));
+        block = std::max(block, block_threads_for(e, s->form, rows));
     }
     first.push_back(t);
     s->ntiles = t;
@@ -370,9 +388,10 @@ int build_set_t(LaunchSet* s) {
     s->npv = np_variant_of(maxp, all_uniform);
     int nmat = 1;
     for (sjf_engine* e : s->eng) nmat = std::max(nmat, e->nmat);
-    s->lds = sizeof(T) * 4 * s->ty * block + sizeof(sjf::MatLut<T>) * nmat;
+    s->lds = (s->form == SJF_TILE_THREAD_ROWS ? sizeof(T) * 6 * block : sizeof(T) * 4 * s->ty * block) +
+             sizeof(sjf::MatLut<T>) * nmat;
     if (s->lds > 65536)
-        return fail(e0, SJF_E_UNSUPPORTED, "tile needs more than 64 KiB of LDS; lower the rows per thread");
+        return fail(e0, SJF_E_UNSUPPORTED, "tile needs more than 64 KiB of LDS; use fewer rows");
     int rc = upload_vec(e0, s->stream, &s->d_args, args);
     if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_tile_first, first);
     if (rc) return rc;
@@ -514,6 +533,11 @@ int launch_step_np(LaunchSet* s) {
     const dim3 grid(s->ntiles), block(s->block);
     const auto* args = (const sjf::StepArgs<T>*)s->d_args;
     const int n = (int)s->eng.size();
+    if (s->form == SJF_TILE_THREAD_ROWS) {
+        hipLaunchKernelGGL((sjf::k_step_rows<T, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+        SETCHK(s, hipGetLastError());
+        return SJF_OK;
+    }
     switch (s->ty) {
         case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
         case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
@@ -710,8 +734,9 @@ LaunchSet* own_set(sjf_engine* e) {
         e->own->stream = e->stream;
         e->own->prec = e->prec;
     }
-    if (e->own->ty != e->ty) {
+    if (e->own->ty != e->ty || e->own->form != e->form) {
         e->own->ty = e->ty;
+        e->own->form = e->form;
         e->own->built = false;
     }
     return e->own;
@@ -761,11 +786,19 @@ int timed_set(LaunchSet* s, long n, double* ms_total, double* ms_kernel_mean) {
 
 void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
-    // Defaults measured on MI355X: one row per thread and short x-chunks (many workgroups)
-    // beat deeper register blocking; the autotuner refines per workload.
+    // Defaults measured on MI355X: for short z lines (nz <= 128) the row-of-threads form wins
+    // (up to ~1024 threads per workgroup, few halo rows); for longer lines one row per thread
+    // keeps more waves resident.  The autotuner refines per workload.
+    long target_groups = 16384;
+    e->form = SJF_TILE_REGISTER_ROWS;
     e->ty = 1;
-    const int nty = e->g.ny;
-    const long target_groups = 16384;
+    const int rows = e->g.nz > 96 ? default_thread_rows(e) : std::min(3, default_thread_rows(e));
+    if (e->g.nz <= 128 && thread_rows_fit(e, rows)) {
+        e->form = SJF_TILE_THREAD_ROWS;
+        e->ty = rows;
+        target_groups = 2048;
+    }
+    const int nty = (e->g.ny + e->ty - 1) / e->ty;
     long nxc = std::max(1L, (target_groups + nty - 1) / nty);
     nxc = std::min<long>(nxc, e->g.nx);
     const int xc = (int)((e->g.nx + nxc - 1) / nxc);
@@ -1599,8 +1632,9 @@ int sjf_step_bytes(const sjf_engine* e, double* bytes) {
     return SJF_OK;
 }
 
-int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int* groups) {
+int sjf_launch_info(const sjf_engine* e, int* form, int* ty, int* xchunk, int* threads, int* groups) {
     if (!e) return SJF_E_ARG;
+    if (form) *form = e->form;
     if (ty) *ty = e->ty;
     if (xchunk) *xchunk = e->xc;
     if (threads) *threads = block_threads(e);
@@ -1608,10 +1642,18 @@ int sjf_launch_info(const sjf_engine* e, int* ty, int* xchunk, int* threads, int
     return SJF_OK;
 }
 
-int sjf_set_tile(sjf_engine* e, int ty, int xchunk) {
+int sjf_set_tile(sjf_engine* e, int form, int ty, int xchunk) {
     if (!e) return SJF_E_ARG;
-    if (ty != 1 && ty != 2 && ty != 4) return fail(e, SJF_E_ARG, "ty must be 1, 2 or 4");
+    if (form == SJF_TILE_REGISTER_ROWS) {
+        if (ty != 1 && ty != 2 && ty != 4) return fail(e, SJF_E_ARG, "rows per thread must be 1, 2 or 4");
+    } else if (form == SJF_TILE_THREAD_ROWS) {
+        if (!thread_rows_fit(e, ty))
+            return fail(e, SJF_E_ARG, "row-of-threads tile needs (rows + 1) * nz <= 1024 and 4-byte-addressable arrays");
+    } else {
+        return fail(e, SJF_E_ARG, "unknown tile form");
+    }
     if (xchunk < 1) return fail(e, SJF_E_ARG, "xchunk must be >= 1");
+    e->form = form;
     e->ty = ty;
     e->xc = std::min(xchunk, e->g.nx);
     e->tile_set = true;
@@ -1669,6 +1711,9 @@ int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out) {
     s->own_stream = true;
     s->prec = e0->prec;
     s->ty = 1;
+    s->form = SJF_TILE_THREAD_ROWS;
+    for (int i = 0; i < n; ++i)
+        if (!thread_rows_fit(engines[i], default_thread_rows(engines[i]))) s->form = SJF_TILE_REGISTER_ROWS;
     for (int i = 0; i < n; ++i) {
         s->eng.push_back(engines[i]);
         engines[i]->batch = b;

@@ -54,6 +54,7 @@ struct StepArgs {
     int nx, ny, nz;
     int px, py, pz;
     int nty, xc;
+    int rt;       // row-of-threads form: thread rows per workgroup (owned rows + 1 halo row)
     long N;       // cells
     T cdt;        // dt / dx
     T* F[2];              // field sets [Ex Ey Ez Hx Hy Hz][N]; step n reads F[n&1], writes F[~n&1]
@@ -473,22 +474,385 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     }
 }
 
-// Fused step over all instances of a launch set: logical tile t (after the XCD-aware remap)
-// belongs to instance b with tile_first[b] <= t < tile_first[b+1].
-template <typename T, int TY, int NP, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T>* __restrict__ args,
-                                               const int* __restrict__ tile_first, int ninst) {
-    // XCD-aware tile order: hardware places workgroup w on XCD w % 8; map so that each XCD
-    // receives a contiguous run of logical tiles (y-tile fastest, then x-chunk).
+// Loads and stores at a 32-bit byte offset from a wave-uniform base: the base stays in scalar
+// registers and the offset in one vector register (global_load ... v_off, s[base]), so an access
+// costs no 64-bit vector address arithmetic.  The caller guarantees offsets below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ldb(GLOBAL const T* base, uint32_t off) {
+    return *(GLOBAL const T*)((GLOBAL const char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void stb(GLOBAL T* base, uint32_t off, T v) {
+    *(GLOBAL T*)((GLOBAL char*)base + off) = v;
+}
+
+// Fused step, row-of-threads form.  A workgroup holds a.rt rows of nz threads (one cell of a
+// y row per thread); thread row 0 recomputes H on the row below the tile (the y halo) and rows
+// 1..rt-1 own rows j0..j0+rt-2.  The y and z neighbours of E (for H) and of H (for E) are
+// exchanged through LDS, so each field value is loaded from global memory once per tile and
+// the y halo costs one extra row per rt-1 owned rows instead of one per row.  The tile
+// marches along x; E of the next two planes and H of the next plane are prefetched into
+// registers one iteration ahead.  The top thread row loads E of the row above the tile
+// (Ex, Ez) itself.  Requires N * sizeof(T) < 4 GiB (32-bit byte offsets).
+template <typename T, int NP>
+__device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int B = blockDim.x;
+    T* sE = reinterpret_cast<T*>(smem_raw);  // [3][B] E^n of the current plane
+    T* sH = sE + 3 * B;                       // [3][B] H^{n+1/2} of the current plane
+    MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sH + 3 * B);
+    {
+        GLOBAL const uint32_t* src = (GLOBAL const uint32_t*)a.lut;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(slut);
+        const int words = (int)(sizeof(MatLut<T>) / 4) * a.nmat;
+        for (int q = threadIdx.x; q < words; q += B) dst[q] = src[q];
+    }
+
+    const long step = *(GLOBAL const long*)a.d_step;
+    GLOBAL const T* F0 = (GLOBAL const T*)a.F[step & 1];
+    GLOBAL T* F1 = (GLOBAL T*)a.F[(step + 1) & 1];
+    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
+    GLOBAL T* Qg = (GLOBAL T*)a.Q;
+    GLOBAL T* Pg = (GLOBAL T*)a.P;
+    GLOBAL T* psig[3] = {(GLOBAL T*)a.psi[0], (GLOBAL T*)a.psi[1], (GLOBAL T*)a.psi[2]};
+    GLOBAL const T* coef = (GLOBAL const T*)a.coef;
+    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
+    const long N = a.N;
+    const int hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
+    const int hw = (step & 1) ? 2 : 4;  // and written
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int RT = a.rt, R = RT - 1;
+    const int tid = threadIdx.x;
+    const bool act = tid < RT * nz;
+    const int r = act ? tid / nz : 0;
+    const int k = act ? tid - r * nz : 0;
+    const int tyi = t % a.nty, xci = t / a.nty;
+    const int jr = tyi * R + r - 1;  // unwrapped global row of this thread row
+    const int j = act ? wrap_or_ghost(jr, ny, a.py) : -1;
+    const bool valid = j >= 0;
+    const bool owned = act && r >= 1 && jr < ny;
+    const bool top = act && r == RT - 1;
+    const int jt = top ? wrap_or_ghost(jr + 1, ny, a.py) : -1;
+    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
+    const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
+    const T cdt = a.cdt;
+    const bool uniform = a.uniform != 0;
+    const uint32_t plane = (uint32_t)ny * (uint32_t)nz;
+    const uint32_t rowc = valid ? (uint32_t)j * nz + k : 0u;
+    const uint32_t rowt = jt >= 0 ? (uint32_t)jt * nz + k : 0u;
+    constexpr uint32_t W = sizeof(T);
+
+    const bool ypml = valid && in_pml(a, 1, j, ny);
+    const bool zpml = act && in_pml(a, 2, k, nz);
+    T bEy = T(0), aEy = T(0), bHy = T(0), aHy = T(0);
+    if (ypml) {
+        GLOBAL const T* cy = coef + a.coff[1];
+        bEy = cy[j];
+        aEy = cy[ny + j];
+        bHy = cy[2 * ny + j];
+        aHy = cy[3 * ny + j];
+    }
+    T bEz = T(0), aEz = T(0), bHz = T(0), aHz = T(0);
+    if (zpml) {
+        GLOBAL const T* cz = coef + a.coff[2];
+        bEz = cz[k];
+        aEz = cz[nz + k];
+        bHz = cz[2 * nz + k];
+        aHz = cz[3 * nz + k];
+    }
+
+    const int x0 = xci * a.xc;
+    const int x1 = min(x0 + a.xc, nx);
+    const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
+
+    T ec[3], en[3], te[2], ten[2], hc[3], hp[2] = {T(0), T(0)};
+    uint32_t mw = 0;
+    {
+        const int p0 = wrap_or_ghost(istart, nx, a.px);
+        const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
+        const uint32_t o0 = ((uint32_t)p0 * plane + rowc) * W;
+        const uint32_t o1 = ((uint32_t)(p1 < 0 ? 0 : p1) * plane + rowc) * W;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ec[c] = valid ? ldb(F0 + c * N, o0) : T(0);
+            en[c] = (valid && p1 >= 0) ? ldb(F0 + c * N, o1) : T(0);
+            hc[c] = valid ? ldb(F0 + (3 + c) * N, o0) : T(0);
+        }
+        const uint32_t t0 = ((uint32_t)p0 * plane + rowt) * W;
+        const uint32_t t1 = ((uint32_t)(p1 < 0 ? 0 : p1) * plane + rowt) * W;
+        te[0] = jt >= 0 ? ldb(F0, t0) : T(0);
+        te[1] = jt >= 0 ? ldb(F0 + 2 * N, t0) : T(0);
+        ten[0] = (jt >= 0 && p1 >= 0) ? ldb(F0, t1) : T(0);
+        ten[1] = (jt >= 0 && p1 >= 0) ? ldb(F0 + 2 * N, t1) : T(0);
+        mw = (!uniform && owned && istart >= x0) ? ldb(matw, ((uint32_t)p0 * plane + rowc) * 4u) : 0u;
+    }
+
+    for (int i = istart; i < x1; ++i) {
+        const bool own_plane = i >= x0;
+        const bool has_next = i + 1 < x1;
+        const int pi = wrap_or_ghost(i, nx, a.px);
+        const uint32_t qc = (uint32_t)pi * plane + rowc;
+
+        // ------------------------------------------------------------ loads -----------------
+        T enn[3] = {T(0), T(0), T(0)}, tenn[2] = {T(0), T(0)}, hn[3] = {T(0), T(0), T(0)};
+        uint32_t mwn = 0;
+        if (has_next) {
+            const int pn = wrap_or_ghost(i + 1, nx, a.px);
+            const int pnn = wrap_or_ghost(i + 2, nx, a.px);
+            const uint32_t on = (uint32_t)pn * plane + rowc;
+            const uint32_t onn = ((uint32_t)(pnn < 0 ? 0 : pnn) * plane + rowc) * W;
+            if (valid && pnn >= 0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) enn[c] = ldb(F0 + c * N, onn);
+            }
+            if (jt >= 0 && pnn >= 0) {
+                const uint32_t ot = ((uint32_t)pnn * plane + rowt) * W;
+                tenn[0] = ldb(F0, ot);
+                tenn[1] = ldb(F0 + 2 * N, ot);
+            }
+            if (valid) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hn[c] = ldb(F0 + (3 + c) * N, on * W);
+            }
+            if (!uniform && owned) mwn = ldb(matw, on * 4u);
+        }
+        const bool xpml = in_pml(a, 0, pi, nx);
+        T bEx = T(0), aEx = T(0), bHx = T(0), aHx = T(0);
+        if (xpml) {
+            GLOBAL const T* cx = coef + a.coff[0];
+            bEx = cx[pi];
+            aEx = cx[nx + pi];
+            bHx = cx[2 * nx + pi];
+            aHx = cx[3 * nx + pi];
+        }
+        const bool wr = owned && own_plane;
+        uint32_t sx = 0, sy = 0, sz = 0;
+        T pH[3][2], pE[3][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pH[d][0] = pH[d][1] = pE[d][0] = pE[d][1] = T(0);
+        if (valid && xpml) {
+            const int s = pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0];
+            sx = (((uint32_t)s * ny + j) * nz + k) * W;
+            pH[0][0] = ldb(psig[0] + hr * a.S[0], sx);
+            pH[0][1] = ldb(psig[0] + (hr + 1) * a.S[0], sx);
+            if (wr) {
+                pE[0][0] = ldb(psig[0], sx);
+                pE[0][1] = ldb(psig[0] + a.S[0], sx);
+            }
+        }
+        if (ypml) {
+            const int s = j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1];
+            sy = (((uint32_t)pi * (a.plo[1] + a.phi[1]) + s) * nz + k) * W;
+            pH[1][0] = ldb(psig[1] + hr * a.S[1], sy);
+            pH[1][1] = ldb(psig[1] + (hr + 1) * a.S[1], sy);
+            if (wr) {
+                pE[1][0] = ldb(psig[1], sy);
+                pE[1][1] = ldb(psig[1] + a.S[1], sy);
+            }
+        }
+        if (valid && zpml) {
+            const int s = k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2];
+            sz = (((uint32_t)pi * ny + j) * (a.plo[2] + a.phi[2]) + s) * W;
+            pH[2][0] = ldb(psig[2] + hr * a.S[2], sz);
+            pH[2][1] = ldb(psig[2] + (hr + 1) * a.S[2], sz);
+            if (wr) {
+                pE[2][0] = ldb(psig[2], sz);
+                pE[2][1] = ldb(psig[2] + a.S[2], sz);
+            }
+        }
+        T Qv[3][NP > 0 ? NP : 1], Pv[3][NP > 0 ? NP : 1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[c][s] = Pv[c][s] = T(0);
+        if (NP > 0 && !uniform && wr) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int m = (mw >> (8 * c)) & 0xff;
+                const int np = m == kDesignMat ? 0 : slut[m].np;
+                const int lm = m == kDesignMat ? 0 : slut[m].lorentz_mask;
+#pragma unroll
+                for (int s = 0; s < NP; ++s) {
+                    if (s < np) {
+                        Qv[c][s] = ldb(Qg + (s * 3 + c) * N, qc * W);
+                        if ((lm >> s) & 1) Pv[c][s] = ldb(Pg + (s * 3 + c) * N, qc * W);
+                    }
+                }
+            }
+        }
+
+        // ------------------------------------------------------------ H^{n+1/2} on plane i --
+        sE[tid] = ec[0];
+        sE[B + tid] = ec[1];
+        sE[2 * B + tid] = ec[2];
+        lds_barrier();
+        T h[3] = {T(0), T(0), T(0)};
+        if (valid) {
+            const T exj = top ? te[0] : sE[tid + nz];
+            const T ezj = top ? te[1] : sE[2 * B + tid + nz];
+            const T exk = kp1 >= 0 ? sE[r * nz + kp1] : T(0);
+            const T eyk = kp1 >= 0 ? sE[B + r * nz + kp1] : T(0);
+            T dEz_y = ezj - ec[2];
+            T dEx_y = exj - ec[0];
+            T dEy_z = eyk - ec[1];
+            T dEx_z = exk - ec[0];
+            T dEz_x = en[2] - ec[2];
+            T dEy_x = en[1] - ec[1];
+            if (ypml) {
+                dEz_y = cpml(dEz_y, bHy, aHy, pH[1][0]);
+                dEx_y = cpml(dEx_y, bHy, aHy, pH[1][1]);
+            }
+            if (zpml) {
+                dEy_z = cpml(dEy_z, bHz, aHz, pH[2][0]);
+                dEx_z = cpml(dEx_z, bHz, aHz, pH[2][1]);
+            }
+            if (xpml) {
+                dEz_x = cpml(dEz_x, bHx, aHx, pH[0][0]);
+                dEy_x = cpml(dEy_x, bHx, aHx, pH[0][1]);
+            }
+            h[0] = hc[0] - cdt * (dEz_y - dEy_z);
+            h[1] = hc[1] - cdt * (dEx_z - dEz_x);
+            h[2] = hc[2] - cdt * (dEy_x - dEx_y);
+            if (wr) {
+                const uint32_t o = qc * W;
+                stb(F1 + 3 * N, o, h[0]);
+                stb(F1 + 4 * N, o, h[1]);
+                stb(F1 + 5 * N, o, h[2]);
+                if (xpml) {
+                    stb(psig[0] + hw * a.S[0], sx, pH[0][0]);
+                    stb(psig[0] + (hw + 1) * a.S[0], sx, pH[0][1]);
+                }
+                if (ypml) {
+                    stb(psig[1] + hw * a.S[1], sy, pH[1][0]);
+                    stb(psig[1] + (hw + 1) * a.S[1], sy, pH[1][1]);
+                }
+                if (zpml) {
+                    stb(psig[2] + hw * a.S[2], sz, pH[2][0]);
+                    stb(psig[2] + (hw + 1) * a.S[2], sz, pH[2][1]);
+                }
+            }
+        }
+        sH[tid] = h[0];
+        sH[B + tid] = h[1];
+        sH[2 * B + tid] = h[2];
+        lds_barrier();
+
+        // ------------------------------------------------------------ E^{n+1} on plane i ----
+        if (wr) {
+            const T hxj = sH[tid - nz];
+            const T hzj = sH[2 * B + tid - nz];
+            const T hxk = km1 >= 0 ? sH[r * nz + km1] : T(0);
+            const T hyk = km1 >= 0 ? sH[B + r * nz + km1] : T(0);
+            T dHz_y = h[2] - hzj;
+            T dHx_y = h[0] - hxj;
+            T dHy_z = h[1] - hyk;
+            T dHx_z = h[0] - hxk;
+            T dHz_x = h[2] - hp[1];
+            T dHy_x = h[1] - hp[0];
+            if (ypml) {
+                dHz_y = cpml(dHz_y, bEy, aEy, pE[1][0]);
+                dHx_y = cpml(dHx_y, bEy, aEy, pE[1][1]);
+                stb(psig[1], sy, pE[1][0]);
+                stb(psig[1] + a.S[1], sy, pE[1][1]);
+            }
+            if (zpml) {
+                dHy_z = cpml(dHy_z, bEz, aEz, pE[2][0]);
+                dHx_z = cpml(dHx_z, bEz, aEz, pE[2][1]);
+                stb(psig[2], sz, pE[2][0]);
+                stb(psig[2] + a.S[2], sz, pE[2][1]);
+            }
+            if (xpml) {
+                dHz_x = cpml(dHz_x, bEx, aEx, pE[0][0]);
+                dHy_x = cpml(dHy_x, bEx, aEx, pE[0][1]);
+                stb(psig[0], sx, pE[0][0]);
+                stb(psig[0] + a.S[0], sx, pE[0][1]);
+            }
+            const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
+            const uint32_t o = qc * W;
+            if (uniform) {
+                const T inv = slut[0].inv_eps;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) stb(F1 + c * N, o, ec[c] + inv * (cdt * curl[c]));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int m = (mw >> (8 * c)) & 0xff;
+                    const T e0 = ec[c];
+                    T inv, sq = T(0);
+                    if (m == kDesignMat) {
+                        const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
+                                            (a.dhi[2] - a.dlo[2]) +
+                                        (k - a.dlo[2]);
+                        inv = dinv[dq];
+                    } else {
+                        const MatLut<T>& L = slut[m];
+                        inv = L.inv_eps;
+#pragma unroll
+                        for (int s = 0; s < NP; ++s) {
+                            if (s < L.np) {
+                                T qn = L.alpha[s] * Qv[c][s] + L.c3[s] * e0;
+                                if ((L.lorentz_mask >> s) & 1) {
+                                    qn += L.beta[s] * Pv[c][s];
+                                    stb(Pg + (s * 3 + c) * N, o, Pv[c][s] + qn);
+                                }
+                                stb(Qg + (s * 3 + c) * N, o, qn);
+                                sq += qn;
+                            }
+                        }
+                    }
+                    stb(F1 + c * N, o, e0 + inv * (cdt * curl[c] - sq));
+                }
+            }
+        }
+
+        hp[0] = h[1];
+        hp[1] = h[2];
+        mw = mwn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ec[c] = en[c];
+            en[c] = enn[c];
+            hc[c] = hn[c];
+        }
+        te[0] = ten[0];
+        te[1] = ten[1];
+        ten[0] = tenn[0];
+        ten[1] = tenn[1];
+    }
+}
+
+// XCD-aware tile order: hardware places workgroup w on XCD w % 8; map so that each XCD
+// receives a contiguous run of logical tiles (y-tile fastest, then x-chunk), then find the
+// instance that owns the tile: instance b owns tile_first[b] <= t < tile_first[b+1].
+__device__ __forceinline__ int tile_instance(const int* __restrict__ tile_first, int ninst, int& t) {
     const int w = blockIdx.x, nw = gridDim.x;
-    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) / 2;
         if (tile_first[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
-    step_tile<T, TY, NP>(args[lo], t - tile_first[lo]);
+    t -= tile_first[lo];
+    return lo;
+}
+
+// Fused step over all instances of a launch set (rows-per-thread form).
+template <typename T, int TY, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T>* __restrict__ args,
+                                               const int* __restrict__ tile_first, int ninst) {
+    int t;
+    const int b = tile_instance(tile_first, ninst, t);
+    step_tile<T, TY, NP>(args[b], t);
+}
+
+// Fused step over all instances of a launch set (row-of-threads form).
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_step_rows(const StepArgs<T>* __restrict__ args,
+                                                    const int* __restrict__ tile_first, int ninst) {
+    int t;
+    const int b = tile_instance(tile_first, ninst, t);
+    step_rows<T, NP>(args[b], t);
 }
 
 // ---- per-step scalars live on the device --------------------------------------------------

@@ -14,6 +14,7 @@ from . import _lib
 from .materials import MaterialSpec
 
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+TILE_FORMS = {1: "register_rows", 2: "thread_rows"}
 
 
 def comp_index(c) -> int:
@@ -235,12 +236,18 @@ class Engine:
         return b.value
 
     def launch_info(self) -> dict:
-        v = [C.c_int() for _ in range(4)]
+        """Fused-kernel launch shape: form ("register_rows" | "thread_rows"), y rows per thread
+        or per workgroup, x-chunk length, threads per workgroup, workgroups."""
+        v = [C.c_int() for _ in range(5)]
         self._chk(self.lib.sjf_launch_info(self.h, *[C.byref(x) for x in v]), "sjf_launch_info")
-        return dict(ty=v[0].value, xchunk=v[1].value, threads=v[2].value, groups=v[3].value)
+        return dict(form=TILE_FORMS[v[0].value], rows=v[1].value, xchunk=v[2].value,
+                    threads=v[3].value, groups=v[4].value)
 
-    def set_tile(self, ty: int, xchunk: int):
-        self._chk(self.lib.sjf_set_tile(self.h, int(ty), int(xchunk)), "sjf_set_tile")
+    def set_tile(self, form: str, rows: int, xchunk: int):
+        """Override the default tile: form "register_rows" (rows per thread in {1, 2, 4}) or
+        "thread_rows" (rows per workgroup, (rows + 1) * nz <= 1024)."""
+        code = {v: k for k, v in TILE_FORMS.items()}[form]
+        self._chk(self.lib.sjf_set_tile(self.h, code, int(rows), int(xchunk)), "sjf_set_tile")
 
     def adjoint_run(self, nsteps: int, checkpoint_every: int, gap, n_design: int):
         """Forward run of `nsteps` steps plus the discrete adjoint: returns (J, dJ/d eps) with

@@ -87,9 +87,14 @@ def test_plain_box_fp32_within_tolerance():
     eng.close()
 
 
+R, T = "register_rows", "thread_rows"
+
+
 @pytest.mark.parametrize("shape,tile", [
-    ((5, 7, 9), (1, 1)), ((5, 7, 9), (4, 2)), ((9, 10, 70), (2, 3)), ((9, 10, 70), (4, 64)),
-    ((12, 3, 65), (4, 5)), ((1, 1, 130), (1, 1)), ((6, 13, 128), (4, 4)),
+    ((5, 7, 9), (R, 1, 1)), ((5, 7, 9), (R, 4, 2)), ((9, 10, 70), (R, 2, 3)), ((9, 10, 70), (R, 4, 64)),
+    ((12, 3, 65), (R, 4, 5)), ((1, 1, 130), (R, 1, 1)), ((6, 13, 128), (R, 4, 4)),
+    ((5, 7, 9), (T, 1, 1)), ((5, 7, 9), (T, 3, 2)), ((9, 10, 70), (T, 2, 3)), ((9, 10, 70), (T, 13, 64)),
+    ((12, 3, 65), (T, 4, 5)), ((1, 1, 130), (T, 1, 1)), ((6, 13, 128), (T, 7, 4)), ((7, 9, 100), (T, 9, 3)),
 ])
 def test_random_media_tiles_fp64(shape, tile):
     scn, rng = random_scenario(shape, seed=sum(shape), pml=[1, 2, 1, 0, 2, 3] if min(shape) > 5 else None)
@@ -107,13 +112,14 @@ def test_random_media_tiles_fp64(shape, tile):
     eng.close()
 
 
+@pytest.mark.parametrize("tile", [("register_rows", 2, 3), ("thread_rows", 4, 3)])
 @pytest.mark.parametrize("periodic", [(1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)])
-def test_periodic_axes_fp64(periodic):
+def test_periodic_axes_fp64(periodic, tile):
     shape = (8, 6, 66)
     pml = [0 if periodic[d] else 2 for d in range(3) for _ in range(2)]
     scn, rng = random_scenario(shape, seed=7, pml=pml, periodic=periodic)
     eng = scn.make(Engine)
-    eng.set_tile(2, 3)
+    eng.set_tile(*tile)
     orc = scn.make(Oracle)
     for c in range(6):
         f = rng.standard_normal(shape)
