@@ -1,4 +1,5 @@
-"""Compile the engine (libsjf.so, hipcc for gfx950) in-tree.
+"""Compile the engine (libsjf.so, hipcc for gfx950) and the host layer (libsjf_host.so and
+the sjf_geom driver, g++) in-tree.
 
     python -m sim_juncs_amd.build
 """
@@ -35,5 +36,33 @@ def build_engine(verbose: bool = False) -> str:
     return OUT
 
 
+HOST_SRC = [os.path.join(HERE, "host", f) for f in
+            ("csg.cpp", "settings.cpp", "materials.cpp", "fft.cpp", "h5out.cpp", "bound_geom.cpp", "capi.cpp")]
+HOST_OUT = os.path.join(HERE, "libsjf_host.so")
+GEOM_OUT = os.path.join(HERE, "sjf_geom")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fopenmp", "-fPIC", "-Wall", "-Wno-unused-result"]
+
+
+def build_host(verbose: bool = False) -> str:
+    """libsjf_host.so (links libsjf.so, found next to it at run time) and the sjf_geom binary."""
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    link = ["-L", HERE, "-lsjf", "-ldl", "-Wl,-rpath,$ORIGIN"]
+    steps = [
+        ["g++", *CXX_FLAGS, "-shared", *inc, "-o", HOST_OUT + ".tmp", *HOST_SRC, *link],
+        ["g++", *CXX_FLAGS, *inc, "-o", GEOM_OUT + ".tmp", os.path.join(HERE, "host", "sjf_geom.cpp"),
+         "-L", HERE, "-lsjf_host", "-lsjf", "-ldl", "-Wl,-rpath,$ORIGIN"],
+    ]
+    for cmd, out in zip(steps, (HOST_OUT, GEOM_OUT)):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building {os.path.basename(out)}")
+        os.replace(out + ".tmp", out)
+    return HOST_OUT
+
+
 if __name__ == "__main__":
     print(build_engine(verbose=True))
+    print(build_host(verbose=True))

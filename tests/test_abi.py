@@ -29,6 +29,18 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == names
 
 
+def test_host_library_exports_every_declared_symbol():
+    from sim_juncs_amd import host
+
+    src = open(os.path.join(ROOT, "include", "sjf_host.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(sjfh_[a-z_0-9]+)\s*\(", src)))
+    assert len(names) >= 12
+    lib = host.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+
+
 def test_struct_layouts_match_header():
     # sizes implied by include/sjf.h on x86-64 (int = 4, double = 8, natural alignment)
     assert C.sizeof(_lib.Grid) == 3 * 4 + 4 + 8 + 8 + 4 + 3 * 4 + 4 + 4  # 4 bytes pad before dx

@@ -1,0 +1,95 @@
+"""The C++ driver (BoundGeom over the engine) end to end on the GPU.
+
+* The reference's small end-to-end case (src/main_test.cpp:1884-2028: tests/run.conf with
+  length 2, PML 1, resolution 5.25 here on a 21^3 grid): field_samples.h5 layout, metadata and
+  monitor series; the series themselves match the FP64 oracle driven with the same voxelised
+  materials, source and monitors (real and imaginary parts as two real runs).
+* The Au/SiO2 bowtie junction scene through the sjf_geom command-line binary: 40 monitor
+  clusters of 40 points, Drude Au and Lorentz SiO2 from the voxeliser, finite series.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from sim_juncs_amd import host
+from sim_juncs_amd.materials import MaterialSpec, Pole
+
+pytestmark = pytest.mark.gpu
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def lut_specs(lut):
+    return [MaterialSpec(m["eps_inf"], tuple(Pole(m["f0"][q], m["gamma"][q], m["sigma"][q], bool(m["is_drude"][q]))
+                                             for q in range(m["npoles"]))) for m in lut]
+
+
+def test_driver_small_case_layout_and_oracle(tmp_path):
+    conf = os.path.join(G, "conf", "run.conf")
+    scene = os.path.join(G, "scene_run.json")
+    argv = ["--conf-file", conf, "--length", "2", "--grid-num", "21", "--out-dir", str(tmp_path)]
+    print("[case] driver run.conf 21^3 fp64 complex", flush=True)
+    s = host.run(conf, scene, argv, precision=8, complex_fields=True)
+    assert s["grid_num"] == 21 and s["n_monitors"] == 2 and s["n_clusters"] == 1
+    h5 = str(tmp_path / "field_samples.h5")
+    assert set(host.h5_list(h5, "/")) == {"info", "cluster_0", "cluster_1"}
+    tb = host.h5_read(h5, "info/time_bounds")
+    assert tb[0] == 0.0 and tb[1] > 0 and 0 < tb[2] < tb[1]
+    assert host.h5_read(h5, "info/n_clusters")[0] == 1
+    for k, v in (("l_per_um", 2.0), ("sim_length", 2.0), ("pml_thickness", 1.0), ("tot_len", 4.0)):
+        assert host.h5_read(h5, f"info/cgs_params/{k}")[0] == v
+    locs = host.h5_read(h5, "cluster_0/locations", ["x", "y", "z"])
+    assert locs.tolist() == [[1.0, 1.0, 1.0], [2.0, 4.0, 1.1]]
+    n_t = s["n_t_pts"]  # save_span 1
+    series = [host.h5_read(h5, f"cluster_0/point_{p}/time", ["Re", "Im"]) for p in range(2)]
+    assert all(t.shape == (n_t, 2) for t in series)
+    eps = host.h5_read(str(tmp_path / "eps-000000.00.h5"), "eps")
+    assert eps.size == 21 ** 3 and np.all(eps == 1.0)
+
+    # the same simulation on the FP64 oracle, one run per part
+    L = 4.0
+    dx = L / 21
+    words, lut = host.voxelize(scene, 21, dx, ambient_eps=1.0, um_scale=2.0, resolution=21 / L)
+    info, p = host.source_info(scene, 0, um_scale=2.0)
+    ref = []
+    for part in (0, 1):
+        o = Oracle((21, 21, 21), dx, precision="fp64", part=part)
+        o.set_materials(words, lut_specs(lut))
+        o.set_pml([5] * 6, 1e-15, 2.0)
+        o.add_source(info["component"], [0, 0, 1, 4, 4, 1], kind="gaussian", freq=p["freq"], width=p["width"],
+                     phase=p["phase"], t_start=p["t_start"], t_end=p["t_end"], amp=info["amplitude"])
+        o.add_monitors("Ex", locs)
+        o.set_save_span(1)
+        assert abs(o.dt - s["dt"]) < 1e-15
+        o.step(n_t)
+        ref.append(o.read_monitors().real[:n_t])
+        o.close()
+    for pt in range(2):
+        for part in (0, 1):
+            a, b = series[pt][:, part], ref[part][:, pt]
+            scale = max(np.abs(ref[0]).max(), np.abs(ref[1]).max())
+            assert np.abs(a - b).max() <= 1e-10 * scale, (pt, part)
+    assert np.abs(ref[0]).max() > 0 and np.abs(ref[1]).max() > 0
+
+
+def test_geom_binary_bowtie_junction(tmp_path):
+    exe = host.GEOM_BIN
+    assert os.path.exists(exe), "sjf_geom is not built"
+    conf = os.path.join(G, "conf", "bowtie_params.conf")
+    scene = os.path.join(G, "scene_bowtie.json")
+    print("[case] sjf_geom bowtie 45^3 fp32 real", flush=True)
+    r = subprocess.run([exe, "--conf-file", conf, "--geom-file", scene, "--grid-num", "45", "--out-dir",
+                        str(tmp_path), "--fp32", "--real"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "simulation completed" in r.stdout
+    h5 = str(tmp_path / "field_samples.h5")
+    names = host.h5_list(h5, "/")
+    assert len(names) == 42 and "cluster_00" in names and "cluster_40" in names
+    n_t = int(host.h5_read(h5, "info/n_time_points")[0])
+    assert n_t > 20
+    t = host.h5_read(h5, "cluster_05/point_0200/time", ["Re", "Im"])
+    assert t.shape == (n_t, 2) and np.all(np.isfinite(t)) and np.abs(t[:, 0]).max() > 0
+    assert np.all(t[:, 1] == 0)  # real run: no imaginary part
