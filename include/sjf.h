@@ -196,7 +196,8 @@ int sjf_step_bytes(const sjf_engine* e, double* bytes);
 /* Launch configuration of the fused update kernel: form, y rows (per thread or per
  * workgroup), x-chunk length, threads per workgroup, workgroups. */
 int sjf_launch_info(const sjf_engine* e, int* form, int* rows, int* xchunk, int* threads, int* groups);
-/* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x). */
+/* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x). The
+ * x-chunk also applies to the reverse step of sjf_adjoint_run (one y row per workgroup). */
 int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
 
 /* ---- adjoint -----------------------------------------------------------------------------

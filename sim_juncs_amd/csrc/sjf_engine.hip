@@ -979,6 +979,35 @@ int set_step(sjf_engine* e, long n) {
     return SJF_OK;
 }
 
+// x-chunk of the fused reverse step: the one sjf_set_tile fixed, else enough workgroups to
+// fill the device (one y row per workgroup)
+int adj_xchunk(const sjf_engine* e) {
+    if (e->tile_set) return std::min(e->xc, e->g.nx);
+    const long target = 8192;
+    long nxc = std::max(1L, (target + e->g.ny - 1) / e->g.ny);
+    nxc = std::min<long>(nxc, e->g.nx);
+    return std::max(1, (int)((e->g.nx + nxc - 1) / nxc));
+}
+
+template <typename T, int NP>
+void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
+    if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 256>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else if (block <= 512) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 512>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 1024>), dim3(tiles), dim3(block), lds, e->stream, g);
+}
+
+template <typename T>
+int launch_adj_step(sjf_engine* e, const sjf::AdjArgs<T>& g, int npv, int block, unsigned tiles, size_t lds) {
+    switch (npv) {
+        case 0: launch_adj_np<T, 0>(e, g, block, tiles, lds); break;
+        case 1: launch_adj_np<T, 1>(e, g, block, tiles, lds); break;
+        case 3: launch_adj_np<T, 3>(e, g, block, tiles, lds); break;
+        default: launch_adj_np<T, 4>(e, g, block, tiles, lds); break;
+    }
+    HIPCHK(e, hipGetLastError());
+    return SJF_OK;
+}
+
 template <typename T>
 int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, double* grad, long nd) {
     LaunchSet* s = own_set(e);
@@ -997,7 +1026,6 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     T* Aq = nullptr;
     T* Ap = nullptr;
     T* Apsi[3] = {nullptr, nullptr, nullptr};
-    T** dA = nullptr;
     auto cleanup = [&]() {
         dev_free(cks);
         dev_free(dJ);
@@ -1007,23 +1035,20 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         dev_free(Aq);
         dev_free(Ap);
         for (int d = 0; d < 3; ++d) dev_free(Apsi[d]);
-        dev_free(dA);
     };
     auto bail = [&](int r) {
         cleanup();
         return r;
     };
     const long slot = 3 * nd + 3;
+    // adjoint fields: two sets of six components (read / written per reverse step)
     if (dev_alloc(e, (void**)&dJ, sizeof(double)) || dev_alloc(e, (void**)&gacc, sizeof(double) * nd) ||
-        dev_alloc(e, (void**)&ring, sizeof(T) * slot * (K + 1)) || dev_alloc(e, (void**)&A, sizeof(T) * 6 * e->N) ||
-        (L.q && dev_alloc(e, (void**)&Aq, L.q)) || (L.p && dev_alloc(e, (void**)&Ap, L.p)) ||
-        dev_alloc(e, (void**)&dA, sizeof(T*) * 6))
+        dev_alloc(e, (void**)&ring, sizeof(T) * slot * (K + 1)) || dev_alloc(e, (void**)&A, sizeof(T) * 12 * e->N) ||
+        (L.q && dev_alloc(e, (void**)&Aq, L.q)) || (L.p && dev_alloc(e, (void**)&Ap, L.p)))
         return bail(SJF_E_NOMEM);
+    // auxiliary bars: per axis 8 slots (E pair, two H pairs, second E pair)
     for (int d = 0; d < 3; ++d)
-        if (L.psi[d] && dev_alloc(e, (void**)&Apsi[d], L.psi[d])) return bail(SJF_E_NOMEM);
-    T* Ah[6];
-    for (int c = 0; c < 6; ++c) Ah[c] = A + (size_t)c * e->N;
-    if ((rc = h2d(e, dA, Ah, sizeof(Ah)))) return bail(rc);
+        if (L.psi[d] && dev_alloc(e, (void**)&Apsi[d], L.psi[d] / 6 * 8)) return bail(SJF_E_NOMEM);
 
     const double dt = e->dt;
     // ---- forward pass: checkpoints every K steps, objective after every step
@@ -1044,12 +1069,19 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     // ---- backward pass, segment by segment
     sjf::AdjArgs<T> g;
     std::memset(&g, 0, sizeof(g));
-    g.f = step_args<T>(e, e->ty);
-    for (int c = 0; c < 6; ++c) g.A[c] = Ah[c];
+    g.f = step_args<T>(e, 1);  // one row per thread; x-chunks of adj_xchunk planes
+    g.f.xc = adj_xchunk(e);
+    g.f.nty = e->g.ny;
+    g.A[0] = A;
+    g.A[1] = A + (size_t)6 * e->N;
+    g.rd = 0;
     g.Aq = Aq;
     g.Ap = Ap;
     for (int d = 0; d < 3; ++d) g.Apsi[d] = Apsi[d];
-    const unsigned gcell = (unsigned)((e->N + 255) / 256);
+    const int block = ((e->g.nz + 63) / 64) * 64;
+    const unsigned tiles = (unsigned)((long)e->g.ny * ((e->g.nx + g.f.xc - 1) / g.f.xc));
+    const size_t lds = sizeof(T) * 12 * block + sizeof(sjf::MatLut<T>) * e->nmat;
+    if (lds > 65536) return bail(fail(e, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
     const unsigned gdes = (unsigned)((nd + 3 + 255) / 256);
     const int de0 = e->dhi[0] - e->dlo[0], de1 = e->dhi[1] - e->dlo[1], de2 = e->dhi[2] - e->dlo[2];
     auto record = [&](long sl) -> int {
@@ -1060,6 +1092,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         return SJF_OK;
     };
     const int npv = np_variant_of(e->maxp, e->uniform);
+    auto ebar = [&]() { return g.A[g.rd]; };
     for (long k = nck - 1; k >= 0; --k) {
         const long n0 = k * K, n1 = std::min(n0 + K, nsteps);
         if ((rc = set_step(e, n0))) return bail(rc);
@@ -1071,26 +1104,23 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
             if ((rc = record(n + 1 - n0))) return bail(rc);
         }
         if (k == nck - 1) {
-            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, dA, ring, nd, n1 - n0, gap, dt);
+            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), e->N, ring, nd,
+                               n1 - n0, gap, dt);
             HIPCHK(e, hipGetLastError());
         }
         for (long n = n1 - 1; n >= n0; --n) {
-            hipLaunchKernelGGL((sjf::k_adj_grad<T>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, e->stream, g,
-                               ring, n + 1 - n0, n - n0, gacc);
+            hipLaunchKernelGGL((sjf::k_adj_grad<T>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, e->stream,
+                               g.f, ebar(), ring, n + 1 - n0, n - n0, gacc);
             HIPCHK(e, hipGetLastError());
             g.hr = (n & 1) ? 2 : 4;
             g.hw = (n & 1) ? 4 : 2;
-            hipLaunchKernelGGL((sjf::k_adj_h<T>), dim3(gcell), dim3(256), 0, e->stream, g);
-            HIPCHK(e, hipGetLastError());
-            switch (npv) {
-                case 0: hipLaunchKernelGGL((sjf::k_adj_e<T, 0>), dim3(gcell), dim3(256), 0, e->stream, g); break;
-                case 1: hipLaunchKernelGGL((sjf::k_adj_e<T, 1>), dim3(gcell), dim3(256), 0, e->stream, g); break;
-                case 3: hipLaunchKernelGGL((sjf::k_adj_e<T, 3>), dim3(gcell), dim3(256), 0, e->stream, g); break;
-                default: hipLaunchKernelGGL((sjf::k_adj_e<T, 4>), dim3(gcell), dim3(256), 0, e->stream, g); break;
-            }
-            HIPCHK(e, hipGetLastError());
+            g.er = (n & 1) ? 0 : 6;
+            g.ew = (n & 1) ? 6 : 0;
+            if ((rc = launch_adj_step<T>(e, g, npv, block, tiles, lds))) return bail(rc);
+            g.rd = 1 - g.rd;
             if (n >= 1) {
-                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, dA, ring, nd, n - n0, gap, dt);
+                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), e->N, ring, nd,
+                                   n - n0, gap, dt);
                 HIPCHK(e, hipGetLastError());
             }
         }

@@ -1050,217 +1050,363 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
 // Non-PML terms have b = 1.  The adjoint of a backward difference e(r) = H(r) - H(r-1) adds
 // ebar(q) - ebar(q+1) to Hbar(q); the adjoint of a forward difference d(r) = E(r+1) - E(r)
 // adds dbar(q-1) - dbar(q) to Ebar(q); positions outside a metallic edge contribute zero.
-// Step 1 (k_adj_h) produces Hbar at the old time and the new H auxiliaries; step 2 (k_adj_e)
-// produces Ebar, Qbar, Pbar and the E auxiliaries.
+// One fused launch per reverse step (adj_tile) does both halves:
+//   phase 1, Hbar_total(q) = Hbar'(q) + sum over E-terms [ebar(q) - ebar(q + e_d)], and the new
+//            H auxiliary bars psiHbar = bH (-s cdt Hbar_total + psiHbar');
+//   phase 2, Ebar(q) = Ebar'(q) + sum over H-terms [dbar(q - e_d) - dbar(q)] + the pole
+//            recursions, and the new E auxiliary bars psiEbar = bE (s cdt u + psiEbar');
+// with u = inv_eps Ebar', ebar = s cdt u (PML: bE s cdt u + (bE - 1) psiEbar'), dbar = -s cdt
+// Hbar_total (PML: bH (-s cdt Hbar_total) + (bH - 1) psiHbar').  The terms, as (E comp, axis,
+// sign, H comp, aux slot on that axis):
+//   E-terms: (Ex, y, +, Hz, 0) (Ex, z, -, Hy, 0) (Ey, z, +, Hx, 1) (Ey, x, -, Hz, 0)
+//            (Ez, x, +, Hy, 1) (Ez, y, -, Hx, 1)
+//   H-terms: (Hx, y, +, Ez, 0) (Hx, z, -, Ey, 0) (Hy, z, +, Ex, 1) (Hy, x, -, Ez, 0)
+//            (Hz, x, +, Ey, 1) (Hz, y, -, Ex, 1)
+// This has the dependency pattern of the forward step (phase 1 uses forward differences,
+// phase 2 backward ones), so the same tiling applies: a workgroup spans a z line, each thread
+// one y row plus a halo row below it where phase 1 is recomputed, z neighbours through LDS,
+// and the tile marches along x carrying u of the next planes.  Adjoint fields and E auxiliary
+// bars are double-buffered (neighbouring workgroups read the old values of the halo row);
+// Qbar and Pbar are only touched by their owner and are updated in place.
 
 template <typename T>
 struct AdjArgs {
-    StepArgs<T> f;     // forward description (geometry, materials, PML tables)
-    T* A[6];           // adjoint fields Ebar x,y,z and Hbar x,y,z
+    StepArgs<T> f;     // forward description (geometry, materials, PML tables, tiles)
+    T* A[2];           // adjoint field sets [Ebar x,y,z, Hbar x,y,z][N]: read A[rd], write A[1 - rd]
+    int rd;
     T* Aq;             // Qbar [slot][comp][N]
     T* Ap;             // Pbar [slot][comp][N]
-    T* Apsi[3];        // per axis [6][S]: E aux bar x2, H aux bar pair A x2, pair B x2
+    T* Apsi[3];        // per axis [8][S]: E aux bar pair (0,1), H aux bar pairs (2,3) and (4,5),
+                       // E aux bar pair (6,7)
+    int er, ew;        // E aux bar pair read (0 or 6) / written (6 or 0)
     int hr, hw;        // H aux bar pair read (2 or 4) / written (4 or 2)
 };
 
-__device__ __forceinline__ void cell_of(long q, int ny, int nz, int& i, int& j, int& k) {
-    k = (int)(q % nz);
-    const long r = q / nz;
-    j = (int)(r % ny);
-    i = (int)(r / ny);
-}
-
 template <typename T>
-__device__ __forceinline__ T inv_eps_of(const StepArgs<T>& a, long q, int i, int j, int k, int c) {
-    if (a.uniform) return a.lut[0].inv_eps;
-    const int m = (a.matw[q] >> (8 * c)) & 0xff;
+__device__ __forceinline__ T inv_eps_cell(const StepArgs<T>& a, const MatLut<T>* slut, GLOBAL const T* dinv,
+                                          uint32_t mw, int c, int i, int j, int k) {
+    if (a.uniform) return slut[0].inv_eps;
+    const int m = (mw >> (8 * c)) & 0xff;
     if (m == kDesignMat) {
-        const long dq = ((long)(i - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
-                            (a.dhi[2] - a.dlo[2]) + (k - a.dlo[2]);
-        return a.dinv[dq];
+        const long dq = ((long)(i - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
+                        (k - a.dlo[2]);
+        return dinv[dq];
     }
-    return a.lut[m].inv_eps;
-}
-
-template <typename T>
-__device__ __forceinline__ long neighbour(const StepArgs<T>& a, int i, int j, int k, int d, int dir) {
-    int c[3] = {i, j, k};
-    const int n[3] = {a.nx, a.ny, a.nz};
-    const int p[3] = {a.px, a.py, a.pz};
-    c[d] += dir;
-    c[d] = wrap_or_ghost(c[d], n[d], p[d]);
-    if (c[d] < 0) return -1;
-    return ((long)c[0] * a.ny + c[1]) * a.nz + c[2];
-}
-
-template <typename T>
-__device__ __forceinline__ long slab_idx(const StepArgs<T>& a, int d, int i, int j, int k) {
-    return d == 0 ? slab_x(a, i, j, k) : (d == 1 ? slab_y(a, i, j, k) : slab_z(a, i, j, k));
-}
-
-template <typename T>
-__device__ __forceinline__ bool pml_at(const StepArgs<T>& a, int d, int i, int j, int k) {
-    const int c = d == 0 ? i : (d == 1 ? j : k);
-    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
-    return in_pml(a, d, c, n);
-}
-
-// E-located curl terms: component c, axis d, sign of (dH) in the curl, H component used,
-// slot of the E auxiliary for (axis d).  Ex: +dy Hz - dz Hy; Ey: +dz Hx - dx Hz; Ez: +dx Hy - dy Hx.
-__constant__ const int kEterm[3][2][4] = {
-    {{1, +1, 5, 0}, {2, -1, 4, 0}},   // Ex: (axis y, +, Hz, slot Ex on y = 0), (axis z, -, Hy, slot Ex on z = 0)
-    {{2, +1, 3, 1}, {0, -1, 5, 0}},   // Ey: (z, +, Hx, slot Ey on z = 1), (x, -, Hz, slot Ey on x = 0)
-    {{0, +1, 4, 1}, {1, -1, 3, 1}},   // Ez: (x, +, Hy, slot Ez on x = 1), (y, -, Hx, slot Ez on y = 1)
-};
-// H-located curl terms: H component h (0..2 = Hx..Hz), axis d, sign, E component used, slot.
-// Hx: -(dy Ez - dz Ey); Hy: -(dz Ex - dx Ez); Hz: -(dx Ey - dy Ex).
-__constant__ const int kHterm[3][2][4] = {
-    {{1, +1, 2, 0}, {2, -1, 1, 0}},   // Hx: (y, +, Ez, slot Hx on y = 0), (z, -, Ey, slot Hx on z = 0)
-    {{2, +1, 0, 1}, {0, -1, 2, 0}},   // Hy: (z, +, Ex, slot Hy on z = 1), (x, -, Ez, slot Hy on x = 0)
-    {{0, +1, 1, 1}, {1, -1, 0, 1}},   // Hz: (x, +, Ey, slot Hz on x = 1), (y, -, Ex, slot Hz on y = 1)
-};
-
-// ebar of E-term (c, t) at cell r: from Ebar', the E auxiliary bar (old value) and bE
-template <typename T>
-__device__ __forceinline__ T ebar_term(const AdjArgs<T>& g, long r, int i, int j, int k, int c, int t) {
-    if (r < 0) return T(0);
-    const StepArgs<T>& a = g.f;
-    const int d = kEterm[c][t][0], sg = kEterm[c][t][1], sl = kEterm[c][t][3];
-    const T u = inv_eps_of(a, r, i, j, k, c) * g.A[c][r];
-    const T et = T(sg) * a.cdt * u;  // adjoint of the corrected difference
-    if (!pml_at(a, d, i, j, k)) return et;
-    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
-    const int cc = d == 0 ? i : (d == 1 ? j : k);
-    const T b = a.coef[a.coff[d] + cc];
-    const T pb = g.Apsi[d][sl * a.S[d] + slab_idx(a, d, i, j, k)];
-    (void)n;
-    return b * et + (b - T(1)) * pb;
-}
-
-// dbar of H-term (h, t) at cell r: from Hbar (already the total at the old time), the H
-// auxiliary bar (old value, read pair) and bH
-template <typename T>
-__device__ __forceinline__ T dbar_term(const AdjArgs<T>& g, long r, int i, int j, int k, int h, int t) {
-    if (r < 0) return T(0);
-    const StepArgs<T>& a = g.f;
-    const int d = kHterm[h][t][0], sg = kHterm[h][t][1], sl = kHterm[h][t][3];
-    const T dt_ = -T(sg) * a.cdt * g.A[3 + h][r];
-    if (!pml_at(a, d, i, j, k)) return dt_;
-    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
-    const int cc = d == 0 ? i : (d == 1 ? j : k);
-    const T b = a.coef[a.coff[d] + 2 * n + cc];
-    const T pb = g.Apsi[d][(g.hr + sl) * a.S[d] + slab_idx(a, d, i, j, k)];
-    return b * dt_ + (b - T(1)) * pb;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_h(const AdjArgs<T> g) {
-    const StepArgs<T>& a = g.f;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.N) return;
-    int i, j, k;
-    cell_of(q, a.ny, a.nz, i, j, k);
-    const int n[3] = {a.nx, a.ny, a.nz};
-    T tot[3];
-#pragma unroll
-    for (int h = 0; h < 3; ++h) tot[h] = g.A[3 + h][q];
-    // Hbar'_total(q) += ebar(q) - ebar(q + e_d) for every E-term that differentiates H_h along d
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int d = kEterm[c][t][0], h = kEterm[c][t][2] - 3;
-            tot[h] += ebar_term(g, q, i, j, k, c, t);
-            int c3[3] = {i, j, k};
-            c3[d] += 1;
-            const long r = neighbour(a, i, j, k, d, +1);
-            if (r >= 0) {
-                c3[d] = wrap_or_ghost(c3[d], n[d], d == 0 ? a.px : (d == 1 ? a.py : a.pz));
-                tot[h] -= ebar_term(g, r, c3[0], c3[1], c3[2], c, t);
-            }
-        }
-    // new H auxiliary bars (write pair): psibar = b (dbar~ + psibar')
-#pragma unroll
-    for (int h = 0; h < 3; ++h)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int d = kHterm[h][t][0], sg = kHterm[h][t][1], sl = kHterm[h][t][3];
-            if (!pml_at(a, d, i, j, k)) continue;
-            const int cc = d == 0 ? i : (d == 1 ? j : k);
-            const T b = a.coef[a.coff[d] + 2 * n[d] + cc];
-            const long s = slab_idx(a, d, i, j, k);
-            const T dt_ = -T(sg) * a.cdt * tot[h];
-            g.Apsi[d][(g.hw + sl) * a.S[d] + s] = b * (dt_ + g.Apsi[d][(g.hr + sl) * a.S[d] + s]);
-        }
-#pragma unroll
-    for (int h = 0; h < 3; ++h) g.A[3 + h][q] = tot[h];
+    return slut[m].inv_eps;
 }
 
 template <typename T, int NP>
-__global__ void __launch_bounds__(256) k_adj_e(const AdjArgs<T> g) {
+__device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     const StepArgs<T>& a = g.f;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.N) return;
-    int i, j, k;
-    cell_of(q, a.ny, a.nz, i, j, k);
-    const int n[3] = {a.nx, a.ny, a.nz};
-    T out[3], u[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        u[c] = inv_eps_of(a, q, i, j, k, c) * g.A[c][q];
-        out[c] = g.A[c][q];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int B = blockDim.x;
+    T* se = reinterpret_cast<T*>(smem_raw);  // [2 parity][4][B]: z E-terms of rows -1 and 0
+    T* sd = se + 8 * B;                      // [2 parity][2][B]: z H-terms of row 0
+    MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sd + 4 * B);
+    {
+        GLOBAL const uint32_t* src = (GLOBAL const uint32_t*)a.lut;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(slut);
+        const int words = (int)(sizeof(MatLut<T>) / 4) * a.nmat;
+        for (int q = threadIdx.x; q < words; q += B) dst[q] = src[q];
     }
-    // Ebar(q) += dbar(q - e_d) - dbar(q) for every H-term that differentiates E_c along d
+    GLOBAL const T* AR = (GLOBAL const T*)g.A[g.rd];
+    GLOBAL T* AW = (GLOBAL T*)g.A[1 - g.rd];
+    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
+    GLOBAL T* Qb = (GLOBAL T*)g.Aq;
+    GLOBAL T* Pb = (GLOBAL T*)g.Ap;
+    GLOBAL T* psib[3] = {(GLOBAL T*)g.Apsi[0], (GLOBAL T*)g.Apsi[1], (GLOBAL T*)g.Apsi[2]};
+    GLOBAL const T* coef = (GLOBAL const T*)a.coef;
+    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
+    const long N = a.N, NZ = a.nz, NYZ = (long)a.ny * a.nz;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int j0 = t % a.nty, xci = t / a.nty;
+    const int x0 = xci * a.xc, x1 = min(x0 + a.xc, nx);
+    const int k = threadIdx.x;
+    const bool kin = k < nz;
+    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz), km1 = wrap_or_ghost(k - 1, nz, a.pz);
+    const T cdt = a.cdt;
+    const int er = g.er, ew = g.ew, hr = g.hr, hw = g.hw;
+    // rows: 0 = j0 - 1 (halo), 1 = j0 (owned), 2 = j0 + 1 (its +y neighbour)
+    int gj[3];
+    bool rok[3], ypml[3];
 #pragma unroll
-    for (int h = 0; h < 3; ++h)
+    for (int r = 0; r < 3; ++r) {
+        gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
+        rok[r] = kin && gj[r] >= 0;
+        ypml[r] = rok[r] && in_pml(a, 1, gj[r], ny);
+    }
+    const bool zpml = kin && in_pml(a, 2, k, nz);
+    GLOBAL const T* cy = coef + a.coff[1];
+    GLOBAL const T* cz = coef + a.coff[2];
+    GLOBAL const T* cx = coef + a.coff[0];
+    const T bEz = zpml ? cz[k] : T(1), bHz = zpml ? cz[2 * nz + k] : T(1);
+
+    const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
+    // u = inv_eps * Ebar' on planes i and i + 1 (rows 0..2), raw Ebar' and material words of
+    // plane i + 2 in flight
+    T uc[3][3], un[3][3], rn[3][3];
+    uint32_t wn[3];
+    auto load_raw = [&](int p, T (&raw)[3][3], uint32_t (&w)[3]) {
+        const int pp = wrap_or_ghost(p, nx, a.px);
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int d = kHterm[h][t][0], c = kHterm[h][t][2];
-            out[c] -= dbar_term(g, q, i, j, k, h, t);
-            const long r = neighbour(a, i, j, k, d, -1);
-            if (r >= 0) {
-                int c3[3] = {i, j, k};
-                c3[d] = wrap_or_ghost(c3[d] - 1, n[d], d == 0 ? a.px : (d == 1 ? a.py : a.pz));
-                out[c] += dbar_term(g, r, c3[0], c3[1], c3[2], h, t);
+        for (int r = 0; r < 3; ++r) {
+            const bool ok = rok[r] && pp >= 0;
+            const long q = (long)(ok ? pp : 0) * NYZ + (long)(ok ? gj[r] : 0) * NZ + k;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) raw[r][c] = ok ? AR[c * N + q] : T(0);
+            w[r] = (ok && !a.uniform) ? matw[q] : 0u;
+        }
+    };
+    auto to_u = [&](int p, const T (&raw)[3][3], const uint32_t (&w)[3], T (&u)[3][3]) {
+        const int pp = wrap_or_ghost(p, nx, a.px);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                u[r][c] = (rok[r] && pp >= 0) ? inv_eps_cell(a, slut, dinv, w[r], c, pp, gj[r], k) * raw[r][c] : T(0);
+    };
+    lds_barrier();  // LUT in LDS
+    {
+        T raw[3][3];
+        uint32_t w[3];
+        load_raw(istart, raw, w);
+        to_u(istart, raw, w, uc);
+        load_raw(istart + 1, raw, w);
+        to_u(istart + 1, raw, w, un);
+    }
+    T dbx_prev[2] = {T(0), T(0)};  // x H-terms of row 0 on plane i - 1: (Hz -> Ey), (Hy -> Ez)
+
+    for (int i = istart; i < x1; ++i) {
+        const bool own_plane = i >= x0;
+        const bool has_next = i + 1 < x1;
+        const int pi = wrap_or_ghost(i, nx, a.px);
+        const int pn = wrap_or_ghost(i + 1, nx, a.px);
+        if (has_next) load_raw(i + 2, rn, wn);
+        const bool xpml = in_pml(a, 0, pi, nx);
+        const bool xpmln = pn >= 0 && in_pml(a, 0, pn, nx);
+        const T bEx = xpml ? cx[pi] : T(1), bHx = xpml ? cx[2 * nx + pi] : T(1);
+        const T bExn = xpmln ? cx[pn] : T(1);
+
+        // ---- phase 1: E-terms and Hbar_total on rows -1 and 0 of plane i ----------------
+        // E-terms on rows r = 0, 1 (labels: y (Ex->Hz), z (Ex->Hy), z (Ey->Hx), x (Ey->Hz),
+        // x (Ez->Hy), y (Ez->Hx)) and the y-terms of row 2
+        T eb[3][6];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int j = gj[r];
+            const bool ok = rok[r];
+            const long q = (long)pi * NYZ + (long)(ok ? j : 0) * NZ + k;
+            const T ux = uc[r][0], uy = uc[r][1], uz = uc[r][2];
+            T t_ex_y = cdt * ux, t_ex_z = -cdt * ux, t_ey_z = cdt * uy, t_ey_x = -cdt * uy, t_ez_x = cdt * uz,
+              t_ez_y = -cdt * uz;
+            if (ok && ypml[r]) {
+                const T b = cy[j];
+                const long s = slab_y(a, pi, j, k);
+                t_ex_y = b * t_ex_y + (b - T(1)) * psib[1][(er + 0) * a.S[1] + s];
+                t_ez_y = b * t_ez_y + (b - T(1)) * psib[1][(er + 1) * a.S[1] + s];
+            }
+            if (r < 2) {
+                if (ok && zpml) {
+                    const long s = slab_z(a, pi, j, k);
+                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * psib[2][(er + 0) * a.S[2] + s];
+                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * psib[2][(er + 1) * a.S[2] + s];
+                }
+                if (ok && xpml) {
+                    const long s = slab_x(a, pi, j, k);
+                    t_ey_x = bEx * t_ey_x + (bEx - T(1)) * psib[0][(er + 0) * a.S[0] + s];
+                    t_ez_x = bEx * t_ez_x + (bEx - T(1)) * psib[0][(er + 1) * a.S[0] + s];
+                }
+            }
+            eb[r][0] = t_ex_y;
+            eb[r][1] = t_ex_z;
+            eb[r][2] = t_ey_z;
+            eb[r][3] = t_ey_x;
+            eb[r][4] = t_ez_x;
+            eb[r][5] = t_ez_y;
+            (void)q;
+        }
+        // x-terms of plane i + 1, rows 0 and 1
+        T ebn[2][2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int j = gj[r];
+            T a1 = -cdt * un[r][1], a2 = cdt * un[r][2];  // (Ey -> Hz), (Ez -> Hy)
+            if (rok[r] && xpmln) {
+                const long s = slab_x(a, pn, j, k);
+                a1 = bExn * a1 + (bExn - T(1)) * psib[0][(er + 0) * a.S[0] + s];
+                a2 = bExn * a2 + (bExn - T(1)) * psib[0][(er + 1) * a.S[0] + s];
+            }
+            ebn[r][0] = (rok[r] && pn >= 0) ? a1 : T(0);
+            ebn[r][1] = (rok[r] && pn >= 0) ? a2 : T(0);
+        }
+        // z-terms through LDS
+        T* bufe = se + (i & 1) * 4 * B;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            bufe[(2 * r + 0) * B + k] = eb[r][1];  // (Ex -> Hy)
+            bufe[(2 * r + 1) * B + k] = eb[r][2];  // (Ey -> Hx)
+        }
+        lds_barrier();
+        T tot[2][3];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int j = gj[r];
+            const bool ok = rok[r];
+            const long q = (long)pi * NYZ + (long)(ok ? j : 0) * NZ + k;
+            const T exz_k = kp1 >= 0 ? bufe[(2 * r + 0) * B + kp1] : T(0);
+            const T eyz_k = kp1 >= 0 ? bufe[(2 * r + 1) * B + kp1] : T(0);
+            // Hx: (Ey, z) and (Ez, y); Hy: (Ex, z) and (Ez, x); Hz: (Ex, y) and (Ey, x)
+            const T hx = ok ? AR[3 * N + q] : T(0);
+            const T hy = ok ? AR[4 * N + q] : T(0);
+            const T hz = ok ? AR[5 * N + q] : T(0);
+            tot[r][0] = ok ? hx + (eb[r][2] - eyz_k) + (eb[r][5] - eb[r + 1][5]) : T(0);
+            tot[r][1] = ok ? hy + (eb[r][1] - exz_k) + (eb[r][4] - ebn[r][1]) : T(0);
+            tot[r][2] = ok ? hz + (eb[r][0] - eb[r + 1][0]) + (eb[r][3] - ebn[r][0]) : T(0);
+        }
+        // own row: Hbar at the old time and the new H auxiliary bars
+        const bool wr = own_plane && rok[1];
+        if (wr) {
+            const int j = gj[1];
+            const long q = (long)pi * NYZ + (long)j * NZ + k;
+            AW[3 * N + q] = tot[1][0];
+            AW[4 * N + q] = tot[1][1];
+            AW[5 * N + q] = tot[1][2];
+            if (ypml[1]) {  // (Hx, y, +, slot 0), (Hz, y, -, slot 1)
+                const T b = cy[2 * ny + j];
+                const long s = slab_y(a, pi, j, k);
+                psib[1][(hw + 0) * a.S[1] + s] = b * (-cdt * tot[1][0] + psib[1][(hr + 0) * a.S[1] + s]);
+                psib[1][(hw + 1) * a.S[1] + s] = b * (cdt * tot[1][2] + psib[1][(hr + 1) * a.S[1] + s]);
+            }
+            if (zpml) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
+                const long s = slab_z(a, pi, j, k);
+                psib[2][(hw + 0) * a.S[2] + s] = bHz * (cdt * tot[1][0] + psib[2][(hr + 0) * a.S[2] + s]);
+                psib[2][(hw + 1) * a.S[2] + s] = bHz * (-cdt * tot[1][1] + psib[2][(hr + 1) * a.S[2] + s]);
+            }
+            if (xpml) {  // (Hy, x, -, slot 0), (Hz, x, +, slot 1)
+                const long s = slab_x(a, pi, j, k);
+                psib[0][(hw + 0) * a.S[0] + s] = bHx * (cdt * tot[1][1] + psib[0][(hr + 0) * a.S[0] + s]);
+                psib[0][(hw + 1) * a.S[0] + s] = bHx * (-cdt * tot[1][2] + psib[0][(hr + 1) * a.S[0] + s]);
             }
         }
-    // pole recursions: qbar = Qbar' + Pbar' - u; Qbar = alpha qbar; Pbar = Pbar' + beta qbar;
-    // Ebar += c3 qbar
-    if (NP > 0 && !a.uniform) {
-        const uint32_t mw = a.matw[q];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int m = (mw >> (8 * c)) & 0xff;
-            if (m == kDesignMat) continue;
-            const MatLut<T>& L = a.lut[m];
-#pragma unroll
-            for (int s = 0; s < NP; ++s) {
-                if (s >= L.np) continue;
-                const long o = (long)(s * 3 + c) * a.N + q;
-                const bool lor = (L.lorentz_mask >> s) & 1;
-                const T pb = lor ? g.Ap[o] : T(0);
-                const T qb = g.Aq[o] + pb - u[c];
-                g.Aq[o] = L.alpha[s] * qb;
-                if (lor) g.Ap[o] = pb + L.beta[s] * qb;
-                out[c] += L.c3[s] * qb;
+
+        // ---- phase 2: H-terms and Ebar on row 0 of plane i --------------------------------
+        // H-terms of row 0: (Hx, y) (Hx, z) (Hy, z) (Hy, x) (Hz, x) (Hz, y); y-terms of row -1
+        T db[6], dbh[2];
+        {
+            const int j = gj[1];
+            T d_hx_y = -cdt * tot[1][0], d_hx_z = cdt * tot[1][0], d_hy_z = -cdt * tot[1][1], d_hy_x = cdt * tot[1][1],
+              d_hz_x = -cdt * tot[1][2], d_hz_y = cdt * tot[1][2];
+            if (rok[1] && ypml[1]) {
+                const T b = cy[2 * ny + j];
+                const long s = slab_y(a, pi, j, k);
+                d_hx_y = b * d_hx_y + (b - T(1)) * psib[1][(hr + 0) * a.S[1] + s];
+                d_hz_y = b * d_hz_y + (b - T(1)) * psib[1][(hr + 1) * a.S[1] + s];
             }
+            if (rok[1] && zpml) {
+                const long s = slab_z(a, pi, j, k);
+                d_hx_z = bHz * d_hx_z + (bHz - T(1)) * psib[2][(hr + 0) * a.S[2] + s];
+                d_hy_z = bHz * d_hy_z + (bHz - T(1)) * psib[2][(hr + 1) * a.S[2] + s];
+            }
+            if (rok[1] && xpml) {
+                const long s = slab_x(a, pi, j, k);
+                d_hy_x = bHx * d_hy_x + (bHx - T(1)) * psib[0][(hr + 0) * a.S[0] + s];
+                d_hz_x = bHx * d_hz_x + (bHx - T(1)) * psib[0][(hr + 1) * a.S[0] + s];
+            }
+            db[0] = d_hx_y;
+            db[1] = d_hx_z;
+            db[2] = d_hy_z;
+            db[3] = d_hy_x;
+            db[4] = d_hz_x;
+            db[5] = d_hz_y;
         }
+        {
+            const int j = gj[0];
+            T d_hx_y = -cdt * tot[0][0], d_hz_y = cdt * tot[0][2];
+            if (rok[0] && ypml[0]) {
+                const T b = cy[2 * ny + j];
+                const long s = slab_y(a, pi, j, k);
+                d_hx_y = b * d_hx_y + (b - T(1)) * psib[1][(hr + 0) * a.S[1] + s];
+                d_hz_y = b * d_hz_y + (b - T(1)) * psib[1][(hr + 1) * a.S[1] + s];
+            }
+            dbh[0] = rok[0] ? d_hx_y : T(0);
+            dbh[1] = rok[0] ? d_hz_y : T(0);
+        }
+        if (own_plane) {
+            T* bufd = sd + (i & 1) * 2 * B;
+            bufd[0 * B + k] = db[2];  // (Hy -> Ex)
+            bufd[1 * B + k] = db[1];  // (Hx -> Ey)
+            lds_barrier();
+            if (wr) {
+                const int j = gj[1];
+                const long q = (long)pi * NYZ + (long)j * NZ + k;
+                const T hyz_k = km1 >= 0 ? bufd[0 * B + km1] : T(0);
+                const T hxz_k = km1 >= 0 ? bufd[1 * B + km1] : T(0);
+                // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
+                T out[3];
+                out[0] = AR[q] + (hyz_k - db[2]) + (dbh[1] - db[5]);
+                out[1] = AR[N + q] + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
+                out[2] = AR[2 * N + q] + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
+                const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
+                if (NP > 0 && !a.uniform) {
+                    const uint32_t mw = matw[q];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const int m = (mw >> (8 * c)) & 0xff;
+                        if (m == kDesignMat) continue;
+                        const MatLut<T>& L = slut[m];
+#pragma unroll
+                        for (int s = 0; s < NP; ++s) {
+                            if (s >= L.np) continue;
+                            const long o = (long)(s * 3 + c) * N + q;
+                            const bool lor = (L.lorentz_mask >> s) & 1;
+                            const T pb = lor ? Pb[o] : T(0);
+                            const T qb = Qb[o] + pb - u0[c];
+                            Qb[o] = L.alpha[s] * qb;
+                            if (lor) Pb[o] = pb + L.beta[s] * qb;
+                            out[c] += L.c3[s] * qb;
+                        }
+                    }
+                }
+                // new E auxiliary bars: E-terms of row 0 with their signs
+                if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
+                    const T b = cy[j];
+                    const long s = slab_y(a, pi, j, k);
+                    psib[1][(ew + 0) * a.S[1] + s] = b * (cdt * u0[0] + psib[1][(er + 0) * a.S[1] + s]);
+                    psib[1][(ew + 1) * a.S[1] + s] = b * (-cdt * u0[2] + psib[1][(er + 1) * a.S[1] + s]);
+                }
+                if (zpml) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
+                    const long s = slab_z(a, pi, j, k);
+                    psib[2][(ew + 0) * a.S[2] + s] = bEz * (-cdt * u0[0] + psib[2][(er + 0) * a.S[2] + s]);
+                    psib[2][(ew + 1) * a.S[2] + s] = bEz * (cdt * u0[1] + psib[2][(er + 1) * a.S[2] + s]);
+                }
+                if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
+                    const long s = slab_x(a, pi, j, k);
+                    psib[0][(ew + 0) * a.S[0] + s] = bEx * (-cdt * u0[1] + psib[0][(er + 0) * a.S[0] + s]);
+                    psib[0][(ew + 1) * a.S[0] + s] = bEx * (cdt * u0[2] + psib[0][(er + 1) * a.S[0] + s]);
+                }
+                AW[q] = out[0];
+                AW[N + q] = out[1];
+                AW[2 * N + q] = out[2];
+            }
+        } else {
+            lds_barrier();  // keep the barrier count uniform across iterations
+        }
+        dbx_prev[0] = rok[1] ? db[4] : T(0);
+        dbx_prev[1] = rok[1] ? db[3] : T(0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) uc[r][c] = un[r][c];
+        if (has_next) to_u(i + 2, rn, wn, un);
     }
-    // new E auxiliary bars (in place): psibar = b (ebar~ + psibar')
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int d = kEterm[c][t][0], sg = kEterm[c][t][1], sl = kEterm[c][t][3];
-            if (!pml_at(a, d, i, j, k)) continue;
-            const int cc = d == 0 ? i : (d == 1 ? j : k);
-            const T b = a.coef[a.coff[d] + cc];
-            const long s = slab_idx(a, d, i, j, k);
-            T& pb = g.Apsi[d][sl * a.S[d] + s];
-            pb = b * (T(sg) * a.cdt * u[c] + pb);
-        }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) g.A[c][q] = out[c];
+}
+
+// Fused reverse step over one engine: workgroup w handles tile w after the XCD-aware remap.
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_step(const AdjArgs<T> g) {
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    adj_tile<T, NP>(g, t);
 }
 
 // record E of the design box and of the objective cell into slot `slot` of a ring buffer
@@ -1297,12 +1443,12 @@ __global__ void k_adj_objective(const T* E, long N, long gap, double dt, double*
     }
 }
 
-// gradient term of step n and injection of dg/dE^n:
+// gradient term of step n:
 //   gacc[v] += sum_c Ebar_c^{n+1}(v) (E_c^{n+1}(v) - E_c^n(v))   (before the adjoint step)
+// Ebar: the adjoint field set holding Ebar^{n+1} ([comp][N]).
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_grad(const AdjArgs<T> g, const T* ring, long s_next, long s_cur,
-                                                  double* gacc) {
-    const StepArgs<T>& a = g.f;
+__global__ void __launch_bounds__(256) k_adj_grad(const StepArgs<T> a, const T* Ebar, const T* ring, long s_next,
+                                                  long s_cur, double* gacc) {
     const long nd = (long)(a.dhi[0] - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) * (a.dhi[2] - a.dlo[2]);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nd) return;
@@ -1316,17 +1462,17 @@ __global__ void __launch_bounds__(256) k_adj_grad(const AdjArgs<T> g, const T* r
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        acc += (double)g.A[c][q] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
+        acc += (double)Ebar[(long)c * a.N + q] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
     gacc[v] += acc;
 }
 
 // Ebar^n(gap) += dJ/dE^n = 2 dt E^n(gap)
 template <typename T>
-__global__ void k_adj_inject(T* const* A, const T* ring, long nd, long slot, long gap, double dt) {
+__global__ void k_adj_inject(T* Ebar, long N, const T* ring, long nd, long slot, long gap, double dt) {
     const int c = threadIdx.x;
     if (blockIdx.x == 0 && c < 3) {
         const double e = (double)ring[slot * (3 * nd + 3) + 3 * nd + c];
-        A[c][gap] = (T)((double)A[c][gap] + 2.0 * dt * e);
+        Ebar[(long)c * N + gap] = (T)((double)Ebar[(long)c * N + gap] + 2.0 * dt * e);
     }
 }
 

@@ -1,8 +1,9 @@
 """Adjoint gradients (sjf_adjoint_run) against central finite differences of the FP64 oracle.
 
 J(eps) = dt * sum_{n=1..N} |E^n(gap)|^2; the engine returns dJ/d eps for every design voxel from
-the discrete adjoint.  Three cases separate the parts of the transposed step: Yee curls only,
-with CPML, and with Drude/Lorentz media.  The oracle computes J independently (D/E split, its
+the discrete adjoint.  The cases separate the parts of the transposed step (Yee curls only, with
+CPML, with Drude/Lorentz media, periodic axes) and the tiling of the fused reverse kernel
+(several x planes per workgroup, a z line longer than one wavefront).  The oracle computes J independently (D/E split, its
 own PML), so agreement pins both the adjoint algebra and the forward model it transposes.
 """
 import numpy as np
@@ -16,16 +17,28 @@ from sim_juncs_amd.engine import Engine
 pytestmark = pytest.mark.gpu
 
 
+# kind -> (shape, periodic axes, PML cells, media, adjoint x-chunk override)
+CASES = {
+    "vacuum": ((14, 12, 20), (0, 0, 0), None, False, None),
+    "pml": ((14, 12, 20), (0, 0, 0), [3] * 6, False, None),
+    "poles": ((14, 12, 20), (0, 0, 0), [3] * 6, True, None),
+    # periodic x and y, PML on the z faces only
+    "periodic": ((14, 12, 20), (1, 1, 0), [0, 0, 0, 0, 3, 3], True, None),
+    # several x planes per workgroup, ragged last chunk (14 = 4 + 4 + 4 + 2)
+    "poles_xchunk4": ((14, 12, 20), (0, 0, 0), [3] * 6, True, 4),
+    # z line of two wavefronts (70 cells in a 128-thread workgroup), chunks of 3 planes
+    "poles_long_z": ((10, 8, 70), (0, 0, 0), [3] * 6, True, 3),
+}
+
+
 def design_case(kind, precision="fp64", steps=160, seed=4):
     rng = np.random.default_rng(seed)
-    shape = (14, 12, 20)
+    shape, periodic, cells, media, _ = CASES[kind]
     dx = 0.02
     mat = np.zeros(shape + (3,), np.uint8)
     lut = [M.VACUUM]
-    pml = None
-    if kind in ("pml", "poles"):
-        pml = ([3, 3, 3, 3, 3, 3], 1e-15, 2.0)
-    if kind == "poles":
+    pml = (cells, 1e-15, 2.0) if cells else None
+    if media:
         lut = [M.VACUUM, M.AU.scaled(1.0), M.SIO2.scaled(1.0)]
         scenario.paint(mat, shape, scenario.box_inside((-1, -1, 12.5), (5.5, 13, 15.5)), 1)
         scenario.paint(mat, shape, scenario.box_inside((8.5, -1, 12.5), (15, 13, 15.5)), 1)
@@ -33,7 +46,7 @@ def design_case(kind, precision="fp64", steps=160, seed=4):
     lo, hi = (5, 4, 9), (9, 8, 12)
     eps = rng.uniform(1, 4, size=tuple(h - l for l, h in zip(lo, hi)))
     scn = scenario.Scenario(name=f"adj_{kind}", shape=shape, dx=dx, precision=precision,
-                            matw=scenario.pack(mat), lut=lut, pml=pml, steps=steps,
+                            periodic=periodic, matw=scenario.pack(mat), lut=lut, pml=pml, steps=steps,
                             design=(lo, hi, eps))
     scn.sources = [scenario.plane_source(shape, dx, 4, scenario.gaussian_pulse(0.8, 1.5, 0.4, start_fs=0.0))]
     gap = (7, 6, 13)
@@ -57,11 +70,13 @@ def oracle_J(scn, gap, eps):
     return o.dt * float((e ** 2).sum())
 
 
-@pytest.mark.parametrize("kind", ["vacuum", "pml", "poles"])
+@pytest.mark.parametrize("kind", list(CASES))
 def test_adjoint_gradient_matches_oracle_finite_differences(kind):
     scn, gap, eps = design_case(kind)
     print(f"[case] adjoint {kind}", flush=True)
     eng = scn.make(Engine)
+    if CASES[kind][4]:
+        eng.set_tile("register_rows", 1, CASES[kind][4])
     J, g = eng.adjoint_run(scn.steps, 37, gap, eps.size)
     eng.close()
     J_o = oracle_J(scn, gap, eps)
