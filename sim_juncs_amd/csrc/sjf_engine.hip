@@ -66,7 +66,7 @@ struct sjf_engine {
     std::string err;
     long version = 0;  // bumped by every change that invalidates compiled launch tables
 
-    void* Fbuf[2] = {};  // field sets [Ex Ey Ez Hx Hy Hz][N]; step n reads set n&1
+    void* Fbuf[2] = {};  // field sets [x][Ex Ey Ez Hx Hy Hz][y][z]; step n reads set n&1
     long* d_step = nullptr;
     long step = 0;
 
@@ -78,8 +78,8 @@ struct sjf_engine {
     void* lut = nullptr;
     int nmat = 1;
     int maxp = 0;
-    void* Qbuf = nullptr;  // [slot][comp][N]
-    void* Pbuf = nullptr;  // [slot][comp][N], present when a slot holds a Lorentz pole
+    void* Qbuf = nullptr;  // [x][slot * 3 + comp][y][z]
+    void* Pbuf = nullptr;  // the same, present when a slot holds a Lorentz pole
     bool uniform = true;
     int dlo[3] = {0, 0, 0}, dhi[3] = {0, 0, 0};
     std::vector<double> deps_h;
@@ -160,6 +160,14 @@ int d2h(sjf_engine* e, void* dst, const void* src, size_t bytes) {
     return SJF_OK;
 }
 
+// the field set read by the next step
+void* field_set(sjf_engine* e) { return e->Fbuf[e->step & 1]; }
+
+// byte offset of component comp of cell q in a field set
+size_t field_offset(const sjf_engine* e, int comp, long q) {
+    return (size_t)e->prec * sjf::field_index(q, comp, (long)e->g.ny * e->g.nz);
+}
+
 const double kCompOff[6][3] = {{0.5, 0.0, 0.0}, {0.0, 0.5, 0.0}, {0.0, 0.0, 0.5},
                                {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}};
 
@@ -223,9 +231,7 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
 }
 
 // Row-of-threads form: (rows + 1) thread rows of nz threads per workgroup, 32-bit offsets.
-bool thread_rows_fit(const sjf_engine* e, int rows) {
-    return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024 && (double)e->N * e->prec < 4294967296.0;
-}
+bool thread_rows_fit(const sjf_engine* e, int rows) { return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024; }
 
 int default_thread_rows(const sjf_engine* e) { return std::max(1, 1024 / e->g.nz - 1); }
 
@@ -275,6 +281,7 @@ sjf::StepArgs<T> step_args(const sjf_engine* e, int ty) {
     a.lut = (const sjf::MatLut<T>*)e->lut;
     a.Q = (T*)e->Qbuf;
     a.P = (T*)e->Pbuf;
+    a.nqp = e->Qbuf ? 3 * e->maxp : 0;
     for (int d = 0; d < 3; ++d) {
         a.pml_on[d] = e->pml_on[d] ? 1 : 0;
         a.plo[d] = e->pml[2 * d];
@@ -409,13 +416,13 @@ int build_set_t(LaunchSet* s) {
             d.p = src.dev;
             d.p.part = e->g.part;
             d.p.dt = e->dt;
-            for (int b = 0; b < 2; ++b) d.F[b] = (T*)e->Fbuf[b] + (size_t)src.comp * e->N;
+            for (int b = 0; b < 2; ++b) d.F[b] = (T*)e->Fbuf[b];
             d.d_step = e->d_step;
             const int sid = (int)srcs.size();
             srcs.push_back(d);
             const long base = (long)idx.size();
             for (size_t q = 0; q < src.idx.size(); ++q) {
-                idx.push_back(src.idx[q]);
+                idx.push_back(sjf::field_index(src.idx[q], src.comp, (long)e->g.ny * e->g.nz));
                 coef.push_back(e->dt * src.wt[q] * inv_eps_at(e, src.idx[q], src.comp));
             }
             const long n = (long)src.idx.size();
@@ -444,14 +451,15 @@ int build_set_t(LaunchSet* s) {
             sjf::MonDev<T> m;
             std::memset(&m, 0, sizeof(m));
             const int c = e->mon[p].comp;
-            for (int b = 0; b < 2; ++b) m.F[b] = (const T*)e->Fbuf[b] + (size_t)c * e->N;
+            for (int b = 0; b < 2; ++b) m.F[b] = (const T*)e->Fbuf[b];
             m.d_step = e->d_step;
             m.out = e->d_samples;
             m.npt = (int)e->mon.size();
             m.p = (int)p;
             m.span = e->save_span;
             for (int q = 0; q < 8; ++q) {
-                m.idx[q] = e->mon[p].idx[q];
+                const long cq = e->mon[p].idx[q];
+                m.idx[q] = cq >= 0 ? sjf::field_index(cq, c, (long)e->g.ny * e->g.nz) : -1;
                 m.w[q] = e->mon[p].w[q];
             }
             mons.push_back(m);
@@ -834,13 +842,27 @@ int convert_to_device(sjf_engine* e, void* dst, const double* src, long n) {
     return h2d(e, dst, tmp.data(), sizeof(T) * n);
 }
 
+// One component of the current field set to / from a dense z-fastest host array: the component
+// is one plane of ny * nz values in each x plane of six, a pitched copy.
 template <typename T>
-int convert_from_device(sjf_engine* e, double* dst, const void* src, long n) {
-    if (sizeof(T) == sizeof(double)) return d2h(e, dst, src, sizeof(double) * n);
-    std::vector<T> tmp(n);
-    int rc = d2h(e, tmp.data(), src, sizeof(T) * n);
-    if (rc) return rc;
-    for (long q = 0; q < n; ++q) dst[q] = (double)tmp[q];
+int write_component(sjf_engine* e, int comp, const double* src) {
+    const size_t plane = sizeof(T) * (size_t)e->g.ny * e->g.nz;
+    std::vector<T> tmp(e->N);
+    for (long q = 0; q < e->N; ++q) tmp[q] = (T)src[q];
+    char* dst = (char*)field_set(e) + plane * comp;
+    HIPCHK(e, hipMemcpy2DAsync(dst, 6 * plane, tmp.data(), plane, plane, e->g.nx, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SJF_OK;
+}
+
+template <typename T>
+int read_component(sjf_engine* e, int comp, double* dst) {
+    const size_t plane = sizeof(T) * (size_t)e->g.ny * e->g.nz;
+    std::vector<T> tmp(e->N);
+    const char* src = (const char*)field_set(e) + plane * comp;
+    HIPCHK(e, hipMemcpy2DAsync(tmp.data(), plane, src, 6 * plane, plane, e->g.nx, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (long q = 0; q < e->N; ++q) dst[q] = (double)tmp[q];
     return SJF_OK;
 }
 
@@ -900,10 +922,7 @@ void trilinear(const sjf_engine* e, int comp, const double* xyz, long* idx, doub
             }
 }
 
-// current buffer of a component (the set read by the next step)
-void* field_buf(sjf_engine* e, int comp) {
-    return (char*)e->Fbuf[e->step & 1] + (size_t)e->prec * comp * e->N;
-}
+
 
 double engine_step_bytes(const sjf_engine* e) {
     const double p = e->prec;
@@ -1059,7 +1078,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         if ((rc = launch_one_step(s, samples_at(s, e->step)))) return bail(rc);
         advance_hosts(s, 1);
         const T* E = (const T*)e->Fbuf[e->step & 1];
-        hipLaunchKernelGGL((sjf::k_adj_objective<T>), dim3(1), dim3(64), 0, e->stream, E, e->N, gap, dt, dJ);
+        hipLaunchKernelGGL((sjf::k_adj_objective<T>), dim3(1), dim3(64), 0, e->stream, E, (long)e->g.ny * e->g.nz, gap, dt, dJ);
         HIPCHK(e, hipGetLastError());
     }
     if ((rc = copy_state(e, cks + L.total * nck, L, true))) return bail(rc);
@@ -1086,7 +1105,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     const int de0 = e->dhi[0] - e->dlo[0], de1 = e->dhi[1] - e->dlo[1], de2 = e->dhi[2] - e->dlo[2];
     auto record = [&](long sl) -> int {
         const T* E = (const T*)e->Fbuf[e->step & 1];
-        hipLaunchKernelGGL((sjf::k_adj_record<T>), dim3(gdes), dim3(256), 0, e->stream, E, e->N, e->g.ny, e->g.nz,
+        hipLaunchKernelGGL((sjf::k_adj_record<T>), dim3(gdes), dim3(256), 0, e->stream, E, (long)e->g.ny * e->g.nz, e->g.ny, e->g.nz,
                            e->dlo[0], e->dlo[1], e->dlo[2], de0, de1, de2, gap, ring, sl);
         HIPCHK(e, hipGetLastError());
         return SJF_OK;
@@ -1413,6 +1432,9 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         }
         if (!e->pml_on[d]) continue;
         e->slab[d] = (long)(lo + hi) * (e->N / n);
+        // the step kernels address the six auxiliary slots of a slab with 32-bit byte offsets
+        if ((double)e->prec * 6 * e->slab[d] >= 4294967296.0)
+            return fail(e, SJF_E_UNSUPPORTED, "PML slab auxiliaries exceed 4 GiB on one axis");
         int rc = dev_alloc(e, &e->psibuf[d], (size_t)e->prec * 6 * e->slab[d]);
         if (rc) return rc;
     }
@@ -1549,7 +1571,7 @@ int sjf_get_field(sjf_engine* e, int comp, const double xyz[3], double out[2]) {
     double v = 0;
     for (int q = 0; q < 8; ++q) {
         if (idx[q] < 0) continue;
-        const char* base = (const char*)field_buf(e, comp) + (size_t)idx[q] * e->prec;
+        const char* base = (const char*)field_set(e) + field_offset(e, comp, idx[q]);
         if (e->prec == 4) {
             float f;
             int rc = d2h(e, &f, base, 4);
@@ -1645,15 +1667,13 @@ int sjf_timed_steps(sjf_engine* e, long n, double* ms_total, double* ms_kernel_m
 int sjf_read_field(sjf_engine* e, int comp, double* out) {
     if (!e || !out || comp < 0 || comp > 5) return SJF_E_ARG;
     if (int rc = sync_engine(e)) return rc;
-    void* src = field_buf(e, comp);
-    return e->prec == 4 ? convert_from_device<float>(e, out, src, e->N) : convert_from_device<double>(e, out, src, e->N);
+    return e->prec == 4 ? read_component<float>(e, comp, out) : read_component<double>(e, comp, out);
 }
 
 int sjf_write_field(sjf_engine* e, int comp, const double* in) {
     if (!e || !in || comp < 0 || comp > 5) return SJF_E_ARG;
     if (int rc = sync_engine(e)) return rc;
-    void* dst = field_buf(e, comp);
-    return e->prec == 4 ? convert_to_device<float>(e, dst, in, e->N) : convert_to_device<double>(e, dst, in, e->N);
+    return e->prec == 4 ? write_component<float>(e, comp, in) : write_component<double>(e, comp, in);
 }
 
 int sjf_step_bytes(const sjf_engine* e, double* bytes) {

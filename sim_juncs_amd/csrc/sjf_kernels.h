@@ -57,14 +57,15 @@ struct StepArgs {
     int rt;       // row-of-threads form: thread rows per workgroup (owned rows + 1 halo row)
     long N;       // cells
     T cdt;        // dt / dx
-    T* F[2];              // field sets [Ex Ey Ez Hx Hy Hz][N]; step n reads F[n&1], writes F[~n&1]
+    T* F[2];              // field sets [x][Ex Ey Ez Hx Hy Hz][y][z]; step n reads F[n&1], writes F[~n&1]
     const long* d_step;   // the engine's step counter
     int uniform;  // 1: single non-dispersive material, no material word read
     int nmat;
     const uint32_t* matw;
     const MatLut<T>* lut;
-    T* Q;  // [slot][comp][N]
-    T* P;  // [slot][comp][N] (Lorentz slots)
+    T* Q;     // [x][slot * 3 + comp][y][z]
+    T* P;     // [x][slot * 3 + comp][y][z] (Lorentz slots)
+    int nqp;  // polarisation planes per x plane (3 * slots)
     int pml_on[3], plo[3], phi[3];
     long S[3];      // PML slab sizes (cells) per axis
     T* psi[3];      // per axis [6][S]: E aux (2 comps), H aux pair A (2), H aux pair B (2);
@@ -124,21 +125,61 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// ---- raw buffer access ----------------------------------------------------------------------
+// A raw buffer descriptor over p (no stride, 4 GiB range, the gfx9 dword data format).  A load
+// or store through it names the descriptor (four scalar registers), a 32-bit per-lane offset (one
+// vector register) and a 32-bit wave-uniform offset (one scalar register), so an access needs no
+// vector address arithmetic and a loop-invariant lane offset stays a single register.  Callers
+// keep the sum of both offsets below 4 GiB.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Rsrc rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(Rsrc r, uint32_t v, uint32_t s);
+template <>
+__device__ __forceinline__ float bld<float>(Rsrc r, uint32_t v, uint32_t s) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0));
+}
+template <>
+__device__ __forceinline__ double bld<double>(Rsrc r, uint32_t v, uint32_t s) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)v, (int)s, 0));
+}
+template <>
+__device__ __forceinline__ uint32_t bld<uint32_t>(Rsrc r, uint32_t v, uint32_t s) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, float x) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, double x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, 0);
+}
+
+// Field and polarisation layout: x planes of components, [x][comp][y][z] for a field set (six
+// components) and [x][slot * 3 + comp][y][z] for Q and P.  One x plane of a set is contiguous,
+// so a kernel addresses a plane through one descriptor and a component by a 32-bit offset.
+__host__ __device__ __forceinline__ long field_index(long q, int c, long nyz) { return q + (5 * (q / nyz) + c) * nyz; }
+
 // Fused H / polarisation / E update of one time step.  TY: y rows per thread; NP: pole slots
-// compiled in (>= the largest pole count of any material); MAXT: launch bound.
-// Each plane iteration issues every global load it needs first (fields, PML auxiliaries,
-// polarisation state, and the next plane's E and material words), then computes H, exchanges
+// compiled in (>= the largest pole count of any material).
+// A workgroup covers one z line (thread k = cell k) of TY consecutive y rows and marches along
+// x.  Each plane iteration issues every global load it needs first (fields, PML auxiliaries,
+// polarisation state, and E and material words of the planes ahead), then computes H, exchanges
 // the z neighbours of H through LDS, and computes E.
+// Addressing: a thread's byte offset within a component plane depends only on its row and z cell,
+// so it is fixed for the whole march; each access is a per-plane descriptor, that lane offset
+// and a uniform component offset.  Row and plane validity are uniform across the workgroup and
+// ghost values are selected to zero after unconditional loads; lanes past the end of the z line
+// (nz not a multiple of the wavefront) read the last cell of the line and store nothing.
 template <typename T, int TY, int NP>
 __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
+    constexpr uint32_t TS = sizeof(T);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int B = blockDim.x;
     T* sh = reinterpret_cast<T*>(smem_raw);  // [2 parity][2 comp][TY][B]
     MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sh + 4 * TY * B);
-
-    // The argument table lives in global memory, so its pointers arrive as generic pointers;
-    // casting them to the global address space keeps every access a global_load/store (flat
-    // accesses would also count against the LDS wait counter and stall the barrier).
     {
         GLOBAL const uint32_t* src = (GLOBAL const uint32_t*)a.lut;
         uint32_t* dst = reinterpret_cast<uint32_t*>(slut);
@@ -147,43 +188,69 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     }
 
     const long step = *(GLOBAL const long*)a.d_step;
-    GLOBAL const T* F0 = (GLOBAL const T*)a.F[step & 1];
-    GLOBAL T* F1 = (GLOBAL T*)a.F[(step + 1) & 1];
-    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
-    GLOBAL T* Qg = (GLOBAL T*)a.Q;
-    GLOBAL T* Pg = (GLOBAL T*)a.P;
-    GLOBAL T* psig[3] = {(GLOBAL T*)a.psi[0], (GLOBAL T*)a.psi[1], (GLOBAL T*)a.psi[2]};
+    const char* F0 = (const char*)a.F[step & 1];
+    const char* F1 = (const char*)a.F[(step + 1) & 1];
+    const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
-    const int hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
-    const int hw = (step & 1) ? 2 : 4;  // and written
+    const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
+    const uint32_t hw = (step & 1) ? 2 : 4;  // and written
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tyi = t % a.nty, xci = t / a.nty;
     const int j0 = tyi * TY;
     const int x0 = xci * a.xc;
-    const int x1 = min(x0 + a.xc, a.nx);
+    const int x1 = min(x0 + a.xc, nx);
     const int k = threadIdx.x;
-    const bool kin = k < a.nz;
-    const int kp1 = wrap_or_ghost(k + 1, a.nz, a.pz);
-    const int km1 = wrap_or_ghost(k - 1, a.nz, a.pz);
-    const long NZ = a.nz;
-    const long NYZ = (long)a.ny * a.nz;
+    const bool kin = k < nz;
+    const int kc = kin ? k : nz - 1;
+    const int kp1 = wrap_or_ghost(kc + 1, nz, a.pz);
+    const int km1 = wrap_or_ghost(kc - 1, nz, a.pz);
+    const bool kp1ok = kp1 >= 0;
+    const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
+    const long FPB = 6L * PB;                             // bytes of one x plane of a field set
+    const long QPB = (long)a.nqp * PB;                    // and of the polarisation
+    const long PW = (long)ny * nz * 4;                    // and of the material words
     const T cdt = a.cdt;
     const bool uniform = a.uniform != 0;
+    const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
+    const uint32_t SB0 = (uint32_t)(a.S[0] * TS), SB1 = (uint32_t)(a.S[1] * TS), SB2 = (uint32_t)(a.S[2] * TS);
+    const Rsrc rpx = rsrc(a.psi[0]), rpy = rsrc(a.psi[1]), rpz = rsrc(a.psi[2]);
 
-    int gj[TY + 2];  // global row of local row r-1 (r = 0 is the halo row below the tile)
+    int gj[TY + 2];        // global row of local row r-1 (r = 0 is the halo row below the tile)
+    bool rv[TY + 2];       // row inside the grid (uniform)
+    uint32_t oR[TY + 2];   // byte offset of (row, kc) within a component plane
+    uint32_t oW[TY + 2];   // the same for the 4-byte material words
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+        gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
+        rv[r] = gj[r] >= 0;
+        const int jj = rv[r] ? gj[r] : 0;
+        oR[r] = (uint32_t)(jj * nz + kc) * TS;
+        oW[r] = (uint32_t)(jj * nz + kc) * 4u;
+    }
+    uint32_t oK1[TY + 1];  // byte offset of (row, kc + 1)
+#pragma unroll
+    for (int r = 0; r < TY + 1; ++r) {
+        const int jj = rv[r] ? gj[r] : 0;
+        oK1[r] = (uint32_t)(jj * nz + (kp1ok ? kp1 : kc)) * TS;
+    }
     bool own_row[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) own_row[r] = (j0 + r) < ny;
     bool ypml[TY + 1];
+    int sy[TY + 1];
 #pragma unroll
-    for (int r = 0; r < TY + 2; ++r) gj[r] = wrap_or_ghost(j0 + r - 1, a.ny, a.py);
-#pragma unroll
-    for (int r = 0; r < TY; ++r) own_row[r] = kin && (j0 + r) < a.ny;
-#pragma unroll
-    for (int r = 0; r < TY + 1; ++r) ypml[r] = gj[r] >= 0 && in_pml(a, 1, gj[r], a.ny);
+    for (int r = 0; r < TY + 1; ++r) {
+        ypml[r] = rv[r] && in_pml(a, 1, gj[r], ny);
+        sy[r] = ypml[r] ? (gj[r] < a.plo[1] ? gj[r] : gj[r] - (ny - a.phi[1]) + a.plo[1]) : 0;
+    }
 
-    const bool zpml = kin && in_pml(a, 2, k, a.nz);
+    const bool zpml = kin && in_pml(a, 2, k, nz);
+    const uint32_t oK = (uint32_t)kc * TS;
+    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : 0u;
     GLOBAL const T* cz = coef + a.coff[2];
-    const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[a.nz + k] : T(0);
-    const T bHz = zpml ? cz[2 * a.nz + k] : T(0), aHz = zpml ? cz[3 * a.nz + k] : T(0);
+    const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[nz + k] : T(0);
+    const T bHz = zpml ? cz[2 * nz + k] : T(0), aHz = zpml ? cz[3 * nz + k] : T(0);
 
     lds_barrier();
 
@@ -194,88 +261,108 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
     {
-        const int p0 = wrap_or_ghost(istart, a.nx, a.px);
-        const int p1 = wrap_or_ghost(istart + 1, a.nx, a.px);
+        const int p0 = wrap_or_ghost(istart, nx, a.px);
+        const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
+        const Rsrc r0 = rsrc(F0 + (long)(p0 >= 0 ? p0 : 0) * FPB);
+        const Rsrc r1 = rsrc(F0 + (long)(p1 >= 0 ? p1 : 0) * FPB);
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            const bool ok0 = kin && gj[r] >= 0 && p0 >= 0;
-            const bool ok1 = kin && gj[r] >= 0 && p1 >= 0;
-            const long q0 = (long)p0 * NYZ + (long)gj[r] * NZ + k;
-            const long q1 = (long)p1 * NYZ + (long)gj[r] * NZ + k;
+            const bool ok0 = rv[r] && p0 >= 0, ok1 = rv[r] && p1 >= 0;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                ec[r][c] = ldg((F0 + c * a.N), q0, ok0);
-                en[r][c] = ldg((F0 + c * a.N), q1, ok1);
+                const T v0 = bld<T>(r0, oR[r], c * PB);
+                const T v1 = bld<T>(r1, oR[r], c * PB);
+                ec[r][c] = ok0 ? v0 : T(0);
+                en[r][c] = ok1 ? v1 : T(0);
             }
         }
+        const bool mw0 = !uniform && istart >= x0;
+        const Rsrc rm = rsrc(matw + (long)(p0 >= 0 ? p0 : 0) * PW);
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
             hp[r][0] = hp[r][1] = T(0);
-            const long q0 = (long)p0 * NYZ + (long)gj[r + 1] * NZ + k;
-            mw[r] = (!uniform && own_row[r] && istart >= x0) ? matw[q0] : 0u;
+            mw[r] = 0u;
+            if (mw0 && own_row[r]) mw[r] = bld<uint32_t>(rm, oW[r + 1], 0);
         }
     }
 
     for (int i = istart; i < x1; ++i) {
         const bool own_plane = i >= x0;
         const bool has_next = i + 1 < x1;
-        const int pi = wrap_or_ghost(i, a.nx, a.px);
-        const int pn = wrap_or_ghost(i + 1, a.nx, a.px);
-        const int pnn = wrap_or_ghost(i + 2, a.nx, a.px);
-        const bool xpml = in_pml(a, 0, pi, a.nx);
+        const int pi = wrap_or_ghost(i, nx, a.px);
+        const int pn = wrap_or_ghost(i + 1, nx, a.px);
+        const int pnn = wrap_or_ghost(i + 2, nx, a.px);
+        const bool xpml = in_pml(a, 0, pi, nx);
+        const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
         GLOBAL const T* cx = coef + a.coff[0];
-        const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[a.nx + pi] : T(0);
-        const T bHx = xpml ? cx[2 * a.nx + pi] : T(0), aHx = xpml ? cx[3 * a.nx + pi] : T(0);
+        const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[nx + pi] : T(0);
+        const T bHx = xpml ? cx[2 * nx + pi] : T(0), aHx = xpml ? cx[3 * nx + pi] : T(0);
+        const Rsrc rF = rsrc(F0 + (long)pi * FPB);
+        const Rsrc rW = rsrc(F1 + (long)pi * FPB);
 
         // ---------------------------------------------------------------- loads ----------
         T enn[TY + 2][3];
         uint32_t mwn[TY];
+        {
+            const bool nn_ok = has_next && pnn >= 0;
+            const Rsrc rn = rsrc(F0 + (long)(nn_ok ? pnn : pi) * FPB);
 #pragma unroll
-        for (int r = 0; r < TY + 2; ++r) {
-            const bool ok = has_next && kin && gj[r] >= 0 && pnn >= 0;
-            const long q = (long)pnn * NYZ + (long)gj[r] * NZ + k;
-            enn[r][0] = ldg((F0 + 0 * a.N), q, ok);
-            enn[r][1] = (r < TY + 1) ? ldg((F0 + 1 * a.N), q, ok) : T(0);
-            enn[r][2] = ldg((F0 + 2 * a.N), q, ok);
-        }
+            for (int r = 0; r < TY + 2; ++r) {
+                const bool ok = nn_ok && rv[r];
+                const T v0 = bld<T>(rn, oR[r], 0);
+                const T v2 = bld<T>(rn, oR[r], 2 * PB);
+                enn[r][0] = ok ? v0 : T(0);
+                enn[r][2] = ok ? v2 : T(0);
+                if (r < TY + 1) {
+                    const T v1 = bld<T>(rn, oR[r], PB);
+                    enn[r][1] = ok ? v1 : T(0);
+                } else {
+                    enn[r][1] = T(0);
+                }
+            }
+            const bool mwn_ok = !uniform && has_next && pn >= 0;
+            const Rsrc rm = rsrc(matw + (long)(mwn_ok ? pn : pi) * PW);
 #pragma unroll
-        for (int r = 0; r < TY; ++r) {
-            const long q = (long)pn * NYZ + (long)gj[r + 1] * NZ + k;
-            mwn[r] = (!uniform && has_next && own_row[r] && pn >= 0) ? matw[q] : 0u;
+            for (int r = 0; r < TY; ++r) {
+                mwn[r] = 0u;
+                if (mwn_ok && own_row[r]) mwn[r] = bld<uint32_t>(rm, oW[r + 1], 0);
+            }
         }
         T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
 #pragma unroll
         for (int r = 0; r < TY + 1; ++r) {
-            const int j = gj[r];
-            const bool ok = kin && j >= 0;
-            const long q = (long)pi * NYZ + (long)j * NZ + k;
-            h0[r][0] = ldg((F0 + 3 * a.N), q, ok);
-            h0[r][1] = ldg((F0 + 4 * a.N), q, ok);
-            h0[r][2] = ldg((F0 + 5 * a.N), q, ok);
-            const bool okk = ok && kp1 >= 0;
-            const long qk = q - k + kp1;
-            ek1[r][0] = ldg((F0 + 0 * a.N), qk, okk);
-            ek1[r][1] = ldg((F0 + 1 * a.N), qk, okk);
+            const bool ok = rv[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T v = bld<T>(rF, oR[r], (3 + c) * PB);
+                h0[r][c] = ok ? v : T(0);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const T v = bld<T>(rF, oK1[r], c * PB);
+                ek1[r][c] = (ok && kp1ok) ? v : T(0);
+            }
 #pragma unroll
             for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
             if (ok && xpml) {
-                const long s = slab_x(a, pi, j, k);
-                pH[r][0][0] = psig[0][(hr + 0) * a.S[0] + s];
-                pH[r][0][1] = psig[0][(hr + 1) * a.S[0] + s];
+                pH[r][0][0] = bld<T>(rpx, oR[r], hr * SB0 + sxo);
+                pH[r][0][1] = bld<T>(rpx, oR[r], (hr + 1) * SB0 + sxo);
             }
-            if (ok && ypml[r]) {
-                const long s = slab_y(a, pi, j, k);
-                pH[r][1][0] = psig[1][(hr + 0) * a.S[1] + s];
-                pH[r][1][1] = psig[1][(hr + 1) * a.S[1] + s];
+            if (ypml[r]) {
+                const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
+                pH[r][1][0] = bld<T>(rpy, oK, hr * SB1 + so);
+                pH[r][1][1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
             }
             if (ok && zpml) {
-                const long s = slab_z(a, pi, j, k);
-                pH[r][2][0] = psig[2][(hr + 0) * a.S[2] + s];
-                pH[r][2][1] = psig[2][(hr + 1) * a.S[2] + s];
+                const uint32_t so = (uint32_t)((pi * ny + gj[r]) * Lz) * TS;
+                pH[r][2][0] = bld<T>(rpz, oZ, hr * SB2 + so);
+                pH[r][2][1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
             }
         }
         T pE[TY][3][2];
         T Qv[TY][3][NP > 0 ? NP : 1], Pv[TY][3][NP > 0 ? NP : 1];
+        const Rsrc rQ = rsrc((const char*)a.Q + (long)pi * QPB);
+        const Rsrc rP = rsrc((const char*)a.P + (long)pi * QPB);
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
 #pragma unroll
@@ -285,34 +372,32 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
 #pragma unroll
                 for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[r][c][s] = Pv[r][c][s] = T(0);
             if (!(own_plane && own_row[r])) continue;
-            const int j = gj[r + 1];
-            const long q = (long)pi * NYZ + (long)j * NZ + k;
+            const uint32_t o = oR[r + 1];
             if (xpml) {
-                const long s = slab_x(a, pi, j, k);
-                pE[r][0][0] = psig[0][0 * a.S[0] + s];
-                pE[r][0][1] = psig[0][1 * a.S[0] + s];
+                pE[r][0][0] = bld<T>(rpx, o, sxo);
+                pE[r][0][1] = bld<T>(rpx, o, SB0 + sxo);
             }
             if (ypml[r + 1]) {
-                const long s = slab_y(a, pi, j, k);
-                pE[r][1][0] = psig[1][0 * a.S[1] + s];
-                pE[r][1][1] = psig[1][1 * a.S[1] + s];
+                const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
+                pE[r][1][0] = bld<T>(rpy, oK, so);
+                pE[r][1][1] = bld<T>(rpy, oK, SB1 + so);
             }
             if (zpml) {
-                const long s = slab_z(a, pi, j, k);
-                pE[r][2][0] = psig[2][0 * a.S[2] + s];
-                pE[r][2][1] = psig[2][1 * a.S[2] + s];
+                const uint32_t so = (uint32_t)((pi * ny + gj[r + 1]) * Lz) * TS;
+                pE[r][2][0] = bld<T>(rpz, oZ, so);
+                pE[r][2][1] = bld<T>(rpz, oZ, SB2 + so);
             }
             if (NP > 0 && !uniform) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int m = (mw[r] >> (8 * c)) & 0xff;
-                    const int np = m == kDesignMat ? 0 : slut[m].np;
+                    const int np = (m == kDesignMat || !kin) ? 0 : slut[m].np;
                     const int lm = m == kDesignMat ? 0 : slut[m].lorentz_mask;
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
                         if (s < np) {
-                            Qv[r][c][s] = Qg[(s * 3 + c) * a.N + q];
-                            if ((lm >> s) & 1) Pv[r][c][s] = Pg[(s * 3 + c) * a.N + q];
+                            Qv[r][c][s] = bld<T>(rQ, o, (s * 3 + c) * PB);
+                            if ((lm >> s) & 1) Pv[r][c][s] = bld<T>(rP, o, (s * 3 + c) * PB);
                         }
                     }
                 }
@@ -323,9 +408,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         T h[TY + 1][3];
 #pragma unroll
         for (int r = 0; r < TY + 1; ++r) {
+            const bool ok = rv[r];
             const int j = gj[r];
-            const bool ok = kin && j >= 0;
-            const long q = (long)pi * NYZ + (long)j * NZ + k;
             const bool wr = own_plane && r >= 1 && own_row[r > 0 ? r - 1 : 0] && ok;
             T dEz_y = ec[r + 1][2] - ec[r][2];
             T dEx_y = ec[r + 1][0] - ec[r][0];
@@ -333,43 +417,42 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             T dEx_z = ek1[r][0] - ec[r][0];
             T dEz_x = en[r][2] - ec[r][2];
             T dEy_x = en[r][1] - ec[r][1];
-            if (ok && ypml[r]) {
+            if (ypml[r]) {
                 GLOBAL const T* cy = coef + a.coff[1];
-                const T b = cy[2 * a.ny + j], c = cy[3 * a.ny + j];
+                const T b = cy[2 * ny + j], c = cy[3 * ny + j];
                 dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
                 dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
-                if (wr) {
-                    const long s = slab_y(a, pi, j, k);
-                    psig[1][(hw + 0) * a.S[1] + s] = pH[r][1][0];
-                    psig[1][(hw + 1) * a.S[1] + s] = pH[r][1][1];
+                if (wr && kin) {
+                    const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
+                    bst(rpy, oK, hw * SB1 + so, pH[r][1][0]);
+                    bst(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
                 }
             }
             if (ok && zpml) {
                 dEy_z = cpml(dEy_z, bHz, aHz, pH[r][2][0]);
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                 if (wr) {
-                    const long s = slab_z(a, pi, j, k);
-                    psig[2][(hw + 0) * a.S[2] + s] = pH[r][2][0];
-                    psig[2][(hw + 1) * a.S[2] + s] = pH[r][2][1];
+                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                    bst(rpz, oZ, hw * SB2 + so, pH[r][2][0]);
+                    bst(rpz, oZ, (hw + 1) * SB2 + so, pH[r][2][1]);
                 }
             }
             if (ok && xpml) {
                 dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
-                if (wr) {
-                    const long s = slab_x(a, pi, j, k);
-                    psig[0][(hw + 0) * a.S[0] + s] = pH[r][0][0];
-                    psig[0][(hw + 1) * a.S[0] + s] = pH[r][0][1];
+                if (wr && kin) {
+                    bst(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
+                    bst(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
                 }
             }
             // outside a metallic edge H is identically zero (ghost row of the halo)
             h[r][0] = ok ? h0[r][0] - cdt * (dEz_y - dEy_z) : T(0);
             h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
             h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
-            if (wr) {
-                F1[3 * a.N + q] = h[r][0];
-                F1[4 * a.N + q] = h[r][1];
-                F1[5 * a.N + q] = h[r][2];
+            if (wr && kin) {
+                bst(rW, oR[r], 3 * PB, h[r][0]);
+                bst(rW, oR[r], 4 * PB, h[r][1]);
+                bst(rW, oR[r], 5 * PB, h[r][2]);
             }
         }
 
@@ -387,7 +470,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             for (int r = 0; r < TY; ++r) {
                 if (!own_row[r]) continue;
                 const int j = gj[r + 1];
-                const long q = (long)pi * NYZ + (long)j * NZ + k;
+                const uint32_t o = oR[r + 1];
                 const T hxkm = km1 >= 0 ? buf[r * B + km1] : T(0);
                 const T hykm = km1 >= 0 ? buf[(TY + r) * B + km1] : T(0);
                 T dHz_y = h[r + 1][2] - h[r][2];
@@ -398,33 +481,39 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 T dHy_x = h[r + 1][1] - hp[r][0];
                 if (ypml[r + 1]) {
                     GLOBAL const T* cy = coef + a.coff[1];
-                    const T b = cy[j], c = cy[a.ny + j];
+                    const T b = cy[j], c = cy[ny + j];
                     dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
                     dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
-                    const long s = slab_y(a, pi, j, k);
-                    psig[1][0 * a.S[1] + s] = pE[r][1][0];
-                    psig[1][1 * a.S[1] + s] = pE[r][1][1];
+                    if (kin) {
+                        const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
+                        bst(rpy, oK, so, pE[r][1][0]);
+                        bst(rpy, oK, SB1 + so, pE[r][1][1]);
+                    }
                 }
                 if (zpml) {
                     dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
                     dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
-                    const long s = slab_z(a, pi, j, k);
-                    psig[2][0 * a.S[2] + s] = pE[r][2][0];
-                    psig[2][1 * a.S[2] + s] = pE[r][2][1];
+                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                    bst(rpz, oZ, so, pE[r][2][0]);
+                    bst(rpz, oZ, SB2 + so, pE[r][2][1]);
                 }
                 if (xpml) {
                     dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
                     dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
-                    const long s = slab_x(a, pi, j, k);
-                    psig[0][0 * a.S[0] + s] = pE[r][0][0];
-                    psig[0][1 * a.S[0] + s] = pE[r][0][1];
+                    if (kin) {
+                        bst(rpx, o, sxo, pE[r][0][0]);
+                        bst(rpx, o, SB0 + sxo, pE[r][0][1]);
+                    }
                 }
                 const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
                 if (uniform) {
                     const T inv = slut[0].inv_eps;
+                    if (kin) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) F1[c * a.N + q] = ec[r + 1][c] + inv * (cdt * curl[c]);
+                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
+                    }
                 } else {
+                    T enew[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const int m = (mw[r] >> (8 * c)) & 0xff;
@@ -433,25 +522,30 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                         if (m == kDesignMat) {
                             const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) +
                                              (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
-                                            (k - a.dlo[2]);
+                                            (kc - a.dlo[2]);
                             inv = dinv[dq];
                         } else {
                             const MatLut<T>& L = slut[m];
                             inv = L.inv_eps;
+                            const int npl = kin ? L.np : 0;  // lanes past the line store nothing
 #pragma unroll
                             for (int s = 0; s < NP; ++s) {
-                                if (s < L.np) {
+                                if (s < npl) {
                                     T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * e0;
                                     if ((L.lorentz_mask >> s) & 1) {
                                         qn += L.beta[s] * Pv[r][c][s];
-                                        Pg[(s * 3 + c) * a.N + q] = Pv[r][c][s] + qn;
+                                        bst(rP, o, (s * 3 + c) * PB, Pv[r][c][s] + qn);
                                     }
-                                    Qg[(s * 3 + c) * a.N + q] = qn;
+                                    bst(rQ, o, (s * 3 + c) * PB, qn);
                                     sq += qn;
                                 }
                             }
                         }
-                        F1[c * a.N + q] = e0 + inv * (cdt * curl[c] - sq);
+                        enew[c] = e0 + inv * (cdt * curl[c] - sq);
+                    }
+                    if (kin) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, enew[c]);
                     }
                 }
             }
@@ -474,18 +568,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     }
 }
 
-// Loads and stores at a 32-bit byte offset from a wave-uniform base: the base stays in scalar
-// registers and the offset in one vector register (global_load ... v_off, s[base]), so an access
-// costs no 64-bit vector address arithmetic.  The caller guarantees offsets below 4 GiB.
-template <typename T>
-__device__ __forceinline__ T ldb(GLOBAL const T* base, uint32_t off) {
-    return *(GLOBAL const T*)((GLOBAL const char*)base + off);
-}
-template <typename T>
-__device__ __forceinline__ void stb(GLOBAL T* base, uint32_t off, T v) {
-    *(GLOBAL T*)((GLOBAL char*)base + off) = v;
-}
-
 // Fused step, row-of-threads form.  A workgroup holds a.rt rows of nz threads (one cell of a
 // y row per thread); thread row 0 recomputes H on the row below the tile (the y halo) and rows
 // 1..rt-1 own rows j0..j0+rt-2.  The y and z neighbours of E (for H) and of H (for E) are
@@ -493,7 +575,7 @@ __device__ __forceinline__ void stb(GLOBAL T* base, uint32_t off, T v) {
 // the y halo costs one extra row per rt-1 owned rows instead of one per row.  The tile
 // marches along x; E of the next two planes and H of the next plane are prefetched into
 // registers one iteration ahead.  The top thread row loads E of the row above the tile
-// (Ex, Ez) itself.  Requires N * sizeof(T) < 4 GiB (32-bit byte offsets).
+// (Ex, Ez) itself.  Addressing as in step_tile: per-plane descriptors and 32-bit offsets.
 template <typename T, int NP>
 __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -509,17 +591,13 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     }
 
     const long step = *(GLOBAL const long*)a.d_step;
-    GLOBAL const T* F0 = (GLOBAL const T*)a.F[step & 1];
-    GLOBAL T* F1 = (GLOBAL T*)a.F[(step + 1) & 1];
-    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
-    GLOBAL T* Qg = (GLOBAL T*)a.Q;
-    GLOBAL T* Pg = (GLOBAL T*)a.P;
-    GLOBAL T* psig[3] = {(GLOBAL T*)a.psi[0], (GLOBAL T*)a.psi[1], (GLOBAL T*)a.psi[2]};
+    const char* F0 = (const char*)a.F[step & 1];
+    const char* F1 = (const char*)a.F[(step + 1) & 1];
+    const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
-    const long N = a.N;
-    const int hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
-    const int hw = (step & 1) ? 2 : 4;  // and written
+    const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
+    const uint32_t hw = (step & 1) ? 2 : 4;  // and written
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int RT = a.rt, R = RT - 1;
     const int tid = threadIdx.x;
@@ -541,6 +619,13 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     const uint32_t rowc = valid ? (uint32_t)j * nz + k : 0u;
     const uint32_t rowt = jt >= 0 ? (uint32_t)jt * nz + k : 0u;
     constexpr uint32_t W = sizeof(T);
+    const uint32_t PB = plane * W;          // bytes of one component plane
+    const long FPB = 6L * PB;               // bytes of one x plane of a field set
+    const long QPB = (long)a.nqp * PB;      // and of the polarisation
+    const long PW = (long)plane * 4;        // and of the material words
+    const uint32_t oc = rowc * W, ot = rowt * W, ow = rowc * 4u;
+    const uint32_t SB0 = (uint32_t)(a.S[0] * W), SB1 = (uint32_t)(a.S[1] * W), SB2 = (uint32_t)(a.S[2] * W);
+    const Rsrc rpx = rsrc(a.psi[0]), rpy = rsrc(a.psi[1]), rpz = rsrc(a.psi[2]);
 
     const bool ypml = valid && in_pml(a, 1, j, ny);
     const bool zpml = act && in_pml(a, 2, k, nz);
@@ -570,28 +655,34 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     {
         const int p0 = wrap_or_ghost(istart, nx, a.px);
         const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
-        const uint32_t o0 = ((uint32_t)p0 * plane + rowc) * W;
-        const uint32_t o1 = ((uint32_t)(p1 < 0 ? 0 : p1) * plane + rowc) * W;
+        const Rsrc r0 = rsrc(F0 + (long)p0 * FPB);
+        const Rsrc r1 = rsrc(F0 + (long)(p1 < 0 ? 0 : p1) * FPB);
+        const bool v1 = valid && p1 >= 0, t1 = jt >= 0 && p1 >= 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ec[c] = valid ? ldb(F0 + c * N, o0) : T(0);
-            en[c] = (valid && p1 >= 0) ? ldb(F0 + c * N, o1) : T(0);
-            hc[c] = valid ? ldb(F0 + (3 + c) * N, o0) : T(0);
+            const T a0 = bld<T>(r0, oc, c * PB), a1 = bld<T>(r1, oc, c * PB), a2 = bld<T>(r0, oc, (3 + c) * PB);
+            ec[c] = valid ? a0 : T(0);
+            en[c] = v1 ? a1 : T(0);
+            hc[c] = valid ? a2 : T(0);
         }
-        const uint32_t t0 = ((uint32_t)p0 * plane + rowt) * W;
-        const uint32_t t1 = ((uint32_t)(p1 < 0 ? 0 : p1) * plane + rowt) * W;
-        te[0] = jt >= 0 ? ldb(F0, t0) : T(0);
-        te[1] = jt >= 0 ? ldb(F0 + 2 * N, t0) : T(0);
-        ten[0] = (jt >= 0 && p1 >= 0) ? ldb(F0, t1) : T(0);
-        ten[1] = (jt >= 0 && p1 >= 0) ? ldb(F0 + 2 * N, t1) : T(0);
-        mw = (!uniform && owned && istart >= x0) ? ldb(matw, ((uint32_t)p0 * plane + rowc) * 4u) : 0u;
+        const T b0 = bld<T>(r0, ot, 0), b1 = bld<T>(r0, ot, 2 * PB);
+        const T b2 = bld<T>(r1, ot, 0), b3 = bld<T>(r1, ot, 2 * PB);
+        te[0] = jt >= 0 ? b0 : T(0);
+        te[1] = jt >= 0 ? b1 : T(0);
+        ten[0] = t1 ? b2 : T(0);
+        ten[1] = t1 ? b3 : T(0);
+        if (!uniform && istart >= x0) {
+            const uint32_t m0 = bld<uint32_t>(rsrc(matw + (long)p0 * PW), ow, 0);
+            mw = owned ? m0 : 0u;
+        }
     }
 
     for (int i = istart; i < x1; ++i) {
         const bool own_plane = i >= x0;
         const bool has_next = i + 1 < x1;
         const int pi = wrap_or_ghost(i, nx, a.px);
-        const uint32_t qc = (uint32_t)pi * plane + rowc;
+        const Rsrc rF = rsrc(F0 + (long)pi * FPB);
+        const Rsrc rW = rsrc(F1 + (long)pi * FPB);
 
         // ------------------------------------------------------------ loads -----------------
         T enn[3] = {T(0), T(0), T(0)}, tenn[2] = {T(0), T(0)}, hn[3] = {T(0), T(0), T(0)};
@@ -599,22 +690,27 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         if (has_next) {
             const int pn = wrap_or_ghost(i + 1, nx, a.px);
             const int pnn = wrap_or_ghost(i + 2, nx, a.px);
-            const uint32_t on = (uint32_t)pn * plane + rowc;
-            const uint32_t onn = ((uint32_t)(pnn < 0 ? 0 : pnn) * plane + rowc) * W;
-            if (valid && pnn >= 0) {
+            const Rsrc rn = rsrc(F0 + (long)pn * FPB);
+            if (pnn >= 0) {
+                const Rsrc rnn = rsrc(F0 + (long)pnn * FPB);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) enn[c] = ldb(F0 + c * N, onn);
+                for (int c = 0; c < 3; ++c) {
+                    const T v = bld<T>(rnn, oc, c * PB);
+                    enn[c] = valid ? v : T(0);
+                }
+                const T u0 = bld<T>(rnn, ot, 0), u2 = bld<T>(rnn, ot, 2 * PB);
+                tenn[0] = jt >= 0 ? u0 : T(0);
+                tenn[1] = jt >= 0 ? u2 : T(0);
             }
-            if (jt >= 0 && pnn >= 0) {
-                const uint32_t ot = ((uint32_t)pnn * plane + rowt) * W;
-                tenn[0] = ldb(F0, ot);
-                tenn[1] = ldb(F0 + 2 * N, ot);
-            }
-            if (valid) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) hn[c] = ldb(F0 + (3 + c) * N, on * W);
+            for (int c = 0; c < 3; ++c) {
+                const T v = bld<T>(rn, oc, (3 + c) * PB);
+                hn[c] = valid ? v : T(0);
             }
-            if (!uniform && owned) mwn = ldb(matw, on * 4u);
+            if (!uniform) {
+                const uint32_t m = bld<uint32_t>(rsrc(matw + (long)pn * PW), ow, 0);
+                mwn = owned ? m : 0u;
+            }
         }
         const bool xpml = in_pml(a, 0, pi, nx);
         T bEx = T(0), aEx = T(0), bHx = T(0), aHx = T(0);
@@ -633,31 +729,31 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         if (valid && xpml) {
             const int s = pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0];
             sx = (((uint32_t)s * ny + j) * nz + k) * W;
-            pH[0][0] = ldb(psig[0] + hr * a.S[0], sx);
-            pH[0][1] = ldb(psig[0] + (hr + 1) * a.S[0], sx);
+            pH[0][0] = bld<T>(rpx, sx, hr * SB0);
+            pH[0][1] = bld<T>(rpx, sx, (hr + 1) * SB0);
             if (wr) {
-                pE[0][0] = ldb(psig[0], sx);
-                pE[0][1] = ldb(psig[0] + a.S[0], sx);
+                pE[0][0] = bld<T>(rpx, sx, 0);
+                pE[0][1] = bld<T>(rpx, sx, SB0);
             }
         }
         if (ypml) {
             const int s = j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1];
             sy = (((uint32_t)pi * (a.plo[1] + a.phi[1]) + s) * nz + k) * W;
-            pH[1][0] = ldb(psig[1] + hr * a.S[1], sy);
-            pH[1][1] = ldb(psig[1] + (hr + 1) * a.S[1], sy);
+            pH[1][0] = bld<T>(rpy, sy, hr * SB1);
+            pH[1][1] = bld<T>(rpy, sy, (hr + 1) * SB1);
             if (wr) {
-                pE[1][0] = ldb(psig[1], sy);
-                pE[1][1] = ldb(psig[1] + a.S[1], sy);
+                pE[1][0] = bld<T>(rpy, sy, 0);
+                pE[1][1] = bld<T>(rpy, sy, SB1);
             }
         }
         if (valid && zpml) {
             const int s = k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2];
             sz = (((uint32_t)pi * ny + j) * (a.plo[2] + a.phi[2]) + s) * W;
-            pH[2][0] = ldb(psig[2] + hr * a.S[2], sz);
-            pH[2][1] = ldb(psig[2] + (hr + 1) * a.S[2], sz);
+            pH[2][0] = bld<T>(rpz, sz, hr * SB2);
+            pH[2][1] = bld<T>(rpz, sz, (hr + 1) * SB2);
             if (wr) {
-                pE[2][0] = ldb(psig[2], sz);
-                pE[2][1] = ldb(psig[2] + a.S[2], sz);
+                pE[2][0] = bld<T>(rpz, sz, 0);
+                pE[2][1] = bld<T>(rpz, sz, SB2);
             }
         }
         T Qv[3][NP > 0 ? NP : 1], Pv[3][NP > 0 ? NP : 1];
@@ -665,6 +761,8 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[c][s] = Pv[c][s] = T(0);
+        const Rsrc rQ = rsrc((const char*)a.Q + (long)pi * QPB);
+        const Rsrc rP = rsrc((const char*)a.P + (long)pi * QPB);
         if (NP > 0 && !uniform && wr) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -674,8 +772,8 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
 #pragma unroll
                 for (int s = 0; s < NP; ++s) {
                     if (s < np) {
-                        Qv[c][s] = ldb(Qg + (s * 3 + c) * N, qc * W);
-                        if ((lm >> s) & 1) Pv[c][s] = ldb(Pg + (s * 3 + c) * N, qc * W);
+                        Qv[c][s] = bld<T>(rQ, oc, (s * 3 + c) * PB);
+                        if ((lm >> s) & 1) Pv[c][s] = bld<T>(rP, oc, (s * 3 + c) * PB);
                     }
                 }
             }
@@ -714,21 +812,20 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             h[1] = hc[1] - cdt * (dEx_z - dEz_x);
             h[2] = hc[2] - cdt * (dEy_x - dEx_y);
             if (wr) {
-                const uint32_t o = qc * W;
-                stb(F1 + 3 * N, o, h[0]);
-                stb(F1 + 4 * N, o, h[1]);
-                stb(F1 + 5 * N, o, h[2]);
+                bst(rW, oc, 3 * PB, h[0]);
+                bst(rW, oc, 4 * PB, h[1]);
+                bst(rW, oc, 5 * PB, h[2]);
                 if (xpml) {
-                    stb(psig[0] + hw * a.S[0], sx, pH[0][0]);
-                    stb(psig[0] + (hw + 1) * a.S[0], sx, pH[0][1]);
+                    bst(rpx, sx, hw * SB0, pH[0][0]);
+                    bst(rpx, sx, (hw + 1) * SB0, pH[0][1]);
                 }
                 if (ypml) {
-                    stb(psig[1] + hw * a.S[1], sy, pH[1][0]);
-                    stb(psig[1] + (hw + 1) * a.S[1], sy, pH[1][1]);
+                    bst(rpy, sy, hw * SB1, pH[1][0]);
+                    bst(rpy, sy, (hw + 1) * SB1, pH[1][1]);
                 }
                 if (zpml) {
-                    stb(psig[2] + hw * a.S[2], sz, pH[2][0]);
-                    stb(psig[2] + (hw + 1) * a.S[2], sz, pH[2][1]);
+                    bst(rpz, sz, hw * SB2, pH[2][0]);
+                    bst(rpz, sz, (hw + 1) * SB2, pH[2][1]);
                 }
             }
         }
@@ -752,27 +849,26 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             if (ypml) {
                 dHz_y = cpml(dHz_y, bEy, aEy, pE[1][0]);
                 dHx_y = cpml(dHx_y, bEy, aEy, pE[1][1]);
-                stb(psig[1], sy, pE[1][0]);
-                stb(psig[1] + a.S[1], sy, pE[1][1]);
+                bst(rpy, sy, 0, pE[1][0]);
+                bst(rpy, sy, SB1, pE[1][1]);
             }
             if (zpml) {
                 dHy_z = cpml(dHy_z, bEz, aEz, pE[2][0]);
                 dHx_z = cpml(dHx_z, bEz, aEz, pE[2][1]);
-                stb(psig[2], sz, pE[2][0]);
-                stb(psig[2] + a.S[2], sz, pE[2][1]);
+                bst(rpz, sz, 0, pE[2][0]);
+                bst(rpz, sz, SB2, pE[2][1]);
             }
             if (xpml) {
                 dHz_x = cpml(dHz_x, bEx, aEx, pE[0][0]);
                 dHy_x = cpml(dHy_x, bEx, aEx, pE[0][1]);
-                stb(psig[0], sx, pE[0][0]);
-                stb(psig[0] + a.S[0], sx, pE[0][1]);
+                bst(rpx, sx, 0, pE[0][0]);
+                bst(rpx, sx, SB0, pE[0][1]);
             }
             const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
-            const uint32_t o = qc * W;
             if (uniform) {
                 const T inv = slut[0].inv_eps;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) stb(F1 + c * N, o, ec[c] + inv * (cdt * curl[c]));
+                for (int c = 0; c < 3; ++c) bst(rW, oc, c * PB, ec[c] + inv * (cdt * curl[c]));
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -793,14 +889,14 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                                 T qn = L.alpha[s] * Qv[c][s] + L.c3[s] * e0;
                                 if ((L.lorentz_mask >> s) & 1) {
                                     qn += L.beta[s] * Pv[c][s];
-                                    stb(Pg + (s * 3 + c) * N, o, Pv[c][s] + qn);
+                                    bst(rP, oc, (s * 3 + c) * PB, Pv[c][s] + qn);
                                 }
-                                stb(Qg + (s * 3 + c) * N, o, qn);
+                                bst(rQ, oc, (s * 3 + c) * PB, qn);
                                 sq += qn;
                             }
                         }
                     }
-                    stb(F1 + c * N, o, e0 + inv * (cdt * curl[c] - sq));
+                    bst(rW, oc, c * PB, e0 + inv * (cdt * curl[c] - sq));
                 }
             }
         }
@@ -925,7 +1021,7 @@ __device__ inline double source_current(const PulseDev& s, long step) {
 template <typename T>
 struct SrcDev {
     PulseDev p;
-    T* F[2];             // the source component in field set 0 / 1
+    T* F[2];             // field set 0 / 1 (entries hold layout indices of the source component)
     const long* d_step;  // the owning engine's step counter
 };
 
@@ -952,7 +1048,7 @@ __global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ sr
 // ---- point monitors: trilinear gather, FP64 result --------------------------------------
 template <typename T>
 struct MonDev {
-    const T* F[2];       // the sampled component in field set 0 / 1
+    const T* F[2];       // field set 0 / 1 (idx holds layout indices of the sampled component)
     const long* d_step;  // owning engine's counter
     double* out;         // sample buffer of the owning engine: [sample][npt]
     int npt;             // points of the owning engine
@@ -987,7 +1083,7 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
 
 template <typename T>
 struct DftDev {
-    const T* F[2];       // field set 0 / 1 base, [6][N]
+    const T* F[2];       // field set 0 / 1 base, [x][6][y][z]
     const long* d_step;  // owning engine's counter
     long N, nyz, nz;
     int lo[3], ext[3];
@@ -1030,7 +1126,7 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
         sincos(w * tH, &sH, &cH);
         for (int s = 0; s < a.ncomp; ++s) {
             const int c = a.comps[s];
-            const double v = (double)F[(long)c * a.N + q] * a.dt;
+            const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
             const long o = ((long)s * a.nf + f) * a.ncell + cell;
             double2 sm = a.acc[o], cp = a.cmp[o];
             neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
@@ -1411,7 +1507,7 @@ __global__ void __launch_bounds__(MAXT) k_adj_step(const AdjArgs<T> g) {
 
 // record E of the design box and of the objective cell into slot `slot` of a ring buffer
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_record(const T* E, long N, int ny, int nz, int dlo0, int dlo1,
+__global__ void __launch_bounds__(256) k_adj_record(const T* E, long nyz, int ny, int nz, int dlo0, int dlo1,
                                                     int dlo2, int de0, int de1, int de2, long gap,
                                                     T* ring, long slot) {
     const long nd = (long)de0 * de1 * de2;
@@ -1423,20 +1519,20 @@ __global__ void __launch_bounds__(256) k_adj_record(const T* E, long N, int ny, 
         const int jj = (int)(r % de1), ii = (int)(r / de1);
         const long q = ((long)(dlo0 + ii) * ny + (dlo1 + jj)) * nz + (dlo2 + kk);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out[c * nd + v] = E[(long)c * N + q];
+        for (int c = 0; c < 3; ++c) out[c * nd + v] = E[field_index(q, c, nyz)];
     } else if (v < nd + 3) {
         const int c = (int)(v - nd);
-        out[3 * nd + c] = E[(long)c * N + gap];
+        out[3 * nd + c] = E[field_index(gap, c, nyz)];
     }
 }
 
 // objective J += dt * sum_c E_c(gap)^2 for the state after a step (FP64)
 template <typename T>
-__global__ void k_adj_objective(const T* E, long N, long gap, double dt, double* J) {
+__global__ void k_adj_objective(const T* E, long nyz, long gap, double dt, double* J) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s = 0;
         for (int c = 0; c < 3; ++c) {
-            const double v = (double)E[(long)c * N + gap];
+            const double v = (double)E[field_index(gap, c, nyz)];
             s += v * v;
         }
         *J += s * dt;
