@@ -1066,8 +1066,12 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         (L.q && dev_alloc(e, (void**)&Aq, L.q)) || (L.p && dev_alloc(e, (void**)&Ap, L.p)))
         return bail(SJF_E_NOMEM);
     // auxiliary bars: per axis 8 slots (E pair, two H pairs, second E pair)
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d) {
+        // the reverse step addresses the eight slots of a slab within one 2 GiB buffer range
+        if ((double)L.psi[d] / 6 * 8 >= 2147483648.0)
+            return bail(fail(e, SJF_E_UNSUPPORTED, "adjoint PML auxiliaries exceed 2 GiB on one axis"));
         if (L.psi[d] && dev_alloc(e, (void**)&Apsi[d], L.psi[d] / 6 * 8)) return bail(SJF_E_NOMEM);
+    }
 
     const double dt = e->dt;
     // ---- forward pass: checkpoints every K steps, objective after every step
@@ -1123,7 +1127,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
             if ((rc = record(n + 1 - n0))) return bail(rc);
         }
         if (k == nck - 1) {
-            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), e->N, ring, nd,
+            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), (long)e->g.ny * e->g.nz, ring, nd,
                                n1 - n0, gap, dt);
             HIPCHK(e, hipGetLastError());
         }
@@ -1138,7 +1142,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
             if ((rc = launch_adj_step<T>(e, g, npv, block, tiles, lds))) return bail(rc);
             g.rd = 1 - g.rd;
             if (n >= 1) {
-                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), e->N, ring, nd,
+                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), (long)e->g.ny * e->g.nz, ring, nd,
                                    n - n0, gap, dt);
                 HIPCHK(e, hipGetLastError());
             }
@@ -1432,9 +1436,9 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
         }
         if (!e->pml_on[d]) continue;
         e->slab[d] = (long)(lo + hi) * (e->N / n);
-        // the step kernels address the six auxiliary slots of a slab with 32-bit byte offsets
-        if ((double)e->prec * 6 * e->slab[d] >= 4294967296.0)
-            return fail(e, SJF_E_UNSUPPORTED, "PML slab auxiliaries exceed 4 GiB on one axis");
+        // the step kernels address the six auxiliary slots of a slab within one 2 GiB buffer range
+        if ((double)e->prec * 6 * e->slab[d] >= 2147483648.0)
+            return fail(e, SJF_E_UNSUPPORTED, "PML slab auxiliaries exceed 2 GiB on one axis");
         int rc = dev_alloc(e, &e->psibuf[d], (size_t)e->prec * 6 * e->slab[d]);
         if (rc) return rc;
     }

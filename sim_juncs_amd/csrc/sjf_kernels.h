@@ -126,15 +126,18 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // ---- raw buffer access ----------------------------------------------------------------------
-// A raw buffer descriptor over p (no stride, 4 GiB range, the gfx9 dword data format).  A load
-// or store through it names the descriptor (four scalar registers), a 32-bit per-lane offset (one
-// vector register) and a 32-bit wave-uniform offset (one scalar register), so an access needs no
-// vector address arithmetic and a loop-invariant lane offset stays a single register.  Callers
-// keep the sum of both offsets below 4 GiB.
+// A raw buffer descriptor over p (no stride, the gfx9 dword data format) whose range is the first
+// 2 GiB from p.  A load or store through it names the descriptor (four scalar registers), a 32-bit
+// per-lane offset (one vector register) and a 32-bit wave-uniform offset (one scalar register), so
+// an access needs no vector address arithmetic and a loop-invariant lane offset stays a single
+// register.  A lane whose offset is kOOB lies outside the range: its load returns 0 and its store
+// is dropped without a memory access, which predicates single lanes without branching.  Callers
+// keep every in-range access (lane plus uniform offset) below 2 GiB from the descriptor base.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kOOB = 0x80000000u;
 __device__ __forceinline__ Rsrc rsrc(const void* p) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0x80000000u, 0x00020000);
 }
 template <typename T>
 __device__ __forceinline__ T bld(Rsrc r, uint32_t v, uint32_t s);
@@ -156,6 +159,8 @@ __device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, float x) {
 __device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, double x) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, 0);
 }
+// true in every lane of the wavefront when p holds in at least one active lane (wave-uniform)
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
 // Field and polarisation layout: x planes of components, [x][comp][y][z] for a field set (six
 // components) and [x][slot * 3 + comp][y][z] for Q and P.  One x plane of a set is contiguous,
@@ -169,10 +174,11 @@ __host__ __device__ __forceinline__ long field_index(long q, int c, long nyz) { 
 // polarisation state, and E and material words of the planes ahead), then computes H, exchanges
 // the z neighbours of H through LDS, and computes E.
 // Addressing: a thread's byte offset within a component plane depends only on its row and z cell,
-// so it is fixed for the whole march; each access is a per-plane descriptor, that lane offset
-// and a uniform component offset.  Row and plane validity are uniform across the workgroup and
-// ghost values are selected to zero after unconditional loads; lanes past the end of the z line
-// (nz not a multiple of the wavefront) read the last cell of the line and store nothing.
+// so it is fixed for the whole march; each access is a per-plane descriptor, that lane offset and
+// a uniform component offset.  Ghost rows, lanes past the end of the z line and lanes outside a
+// z PML layer, a pole slot or the design region carry the out-of-range offset instead of
+// branching; whole-wavefront conditions (row, plane, x/y PML, any lane in a z layer / using a
+// slot / in the design region) are uniform branches.
 template <typename T, int TY, int NP>
 __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     constexpr uint32_t TS = sizeof(T);
@@ -192,7 +198,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     const char* F1 = (const char*)a.F[(step + 1) & 1];
     const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
-    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -202,10 +207,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     const int x1 = min(x0 + a.xc, nx);
     const int k = threadIdx.x;
     const bool kin = k < nz;
-    const int kc = kin ? k : nz - 1;
-    const int kp1 = wrap_or_ghost(kc + 1, nz, a.pz);
-    const int km1 = wrap_or_ghost(kc - 1, nz, a.pz);
-    const bool kp1ok = kp1 >= 0;
+    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
+    const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
     const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
     const long FPB = 6L * PB;                             // bytes of one x plane of a field set
     const long QPB = (long)a.nqp * PB;                    // and of the polarisation
@@ -218,22 +221,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
 
     int gj[TY + 2];        // global row of local row r-1 (r = 0 is the halo row below the tile)
     bool rv[TY + 2];       // row inside the grid (uniform)
-    uint32_t oR[TY + 2];   // byte offset of (row, kc) within a component plane
+    uint32_t oR[TY + 2];   // byte offset of (row, k) within a component plane, or kOOB
     uint32_t oW[TY + 2];   // the same for the 4-byte material words
 #pragma unroll
     for (int r = 0; r < TY + 2; ++r) {
         gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
         rv[r] = gj[r] >= 0;
-        const int jj = rv[r] ? gj[r] : 0;
-        oR[r] = (uint32_t)(jj * nz + kc) * TS;
-        oW[r] = (uint32_t)(jj * nz + kc) * 4u;
+        const bool ok = rv[r] && kin;
+        oR[r] = ok ? (uint32_t)(gj[r] * nz + k) * TS : kOOB;
+        oW[r] = ok ? (uint32_t)(gj[r] * nz + k) * 4u : kOOB;
     }
-    uint32_t oK1[TY + 1];  // byte offset of (row, kc + 1)
+    uint32_t oK1[TY + 1];  // byte offset of (row, k + 1), or kOOB
 #pragma unroll
-    for (int r = 0; r < TY + 1; ++r) {
-        const int jj = rv[r] ? gj[r] : 0;
-        oK1[r] = (uint32_t)(jj * nz + (kp1ok ? kp1 : kc)) * TS;
-    }
+    for (int r = 0; r < TY + 1; ++r)
+        oK1[r] = (rv[r] && kin && kp1 >= 0) ? (uint32_t)(gj[r] * nz + kp1) * TS : kOOB;
     bool own_row[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) own_row[r] = (j0 + r) < ny;
@@ -246,8 +247,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     }
 
     const bool zpml = kin && in_pml(a, 2, k, nz);
-    const uint32_t oK = (uint32_t)kc * TS;
-    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : 0u;
+    const bool zany = wave_any(zpml);  // some lane of this wavefront lies in a z PML layer
+    const uint32_t oK = kin ? (uint32_t)k * TS : kOOB;
+    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : kOOB;
     GLOBAL const T* cz = coef + a.coff[2];
     const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[nz + k] : T(0);
     const T bHz = zpml ? cz[2 * nz + k] : T(0), aHz = zpml ? cz[3 * nz + k] : T(0);
@@ -267,13 +269,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         const Rsrc r1 = rsrc(F0 + (long)(p1 >= 0 ? p1 : 0) * FPB);
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            const bool ok0 = rv[r] && p0 >= 0, ok1 = rv[r] && p1 >= 0;
+            const uint32_t o0 = p0 >= 0 ? oR[r] : kOOB, o1 = p1 >= 0 ? oR[r] : kOOB;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T v0 = bld<T>(r0, oR[r], c * PB);
-                const T v1 = bld<T>(r1, oR[r], c * PB);
-                ec[r][c] = ok0 ? v0 : T(0);
-                en[r][c] = ok1 ? v1 : T(0);
+                ec[r][c] = bld<T>(r0, o0, c * PB);
+                en[r][c] = bld<T>(r1, o1, c * PB);
             }
         }
         const bool mw0 = !uniform && istart >= x0;
@@ -304,21 +304,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         T enn[TY + 2][3];
         uint32_t mwn[TY];
         {
-            const bool nn_ok = has_next && pnn >= 0;
-            const Rsrc rn = rsrc(F0 + (long)(nn_ok ? pnn : pi) * FPB);
+            // plane i + 2 (a ghost plane reads as zero; after the last plane the values are unused)
+            const Rsrc rn = rsrc(F0 + (long)(pnn >= 0 ? pnn : pi) * FPB);
 #pragma unroll
             for (int r = 0; r < TY + 2; ++r) {
-                const bool ok = nn_ok && rv[r];
-                const T v0 = bld<T>(rn, oR[r], 0);
-                const T v2 = bld<T>(rn, oR[r], 2 * PB);
-                enn[r][0] = ok ? v0 : T(0);
-                enn[r][2] = ok ? v2 : T(0);
-                if (r < TY + 1) {
-                    const T v1 = bld<T>(rn, oR[r], PB);
-                    enn[r][1] = ok ? v1 : T(0);
-                } else {
-                    enn[r][1] = T(0);
-                }
+                const uint32_t o = pnn >= 0 ? oR[r] : kOOB;
+                enn[r][0] = bld<T>(rn, o, 0);
+                enn[r][1] = (r < TY + 1) ? bld<T>(rn, o, PB) : T(0);
+                enn[r][2] = bld<T>(rn, o, 2 * PB);
             }
             const bool mwn_ok = !uniform && has_next && pn >= 0;
             const Rsrc rm = rsrc(matw + (long)(mwn_ok ? pn : pi) * PW);
@@ -331,20 +324,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
 #pragma unroll
         for (int r = 0; r < TY + 1; ++r) {
-            const bool ok = rv[r];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T v = bld<T>(rF, oR[r], (3 + c) * PB);
-                h0[r][c] = ok ? v : T(0);
-            }
+            for (int c = 0; c < 3; ++c) h0[r][c] = bld<T>(rF, oR[r], (3 + c) * PB);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const T v = bld<T>(rF, oK1[r], c * PB);
-                ek1[r][c] = (ok && kp1ok) ? v : T(0);
-            }
+            for (int c = 0; c < 2; ++c) ek1[r][c] = bld<T>(rF, oK1[r], c * PB);
 #pragma unroll
             for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
-            if (ok && xpml) {
+            if (rv[r] && xpml) {
                 pH[r][0][0] = bld<T>(rpx, oR[r], hr * SB0 + sxo);
                 pH[r][0][1] = bld<T>(rpx, oR[r], (hr + 1) * SB0 + sxo);
             }
@@ -353,7 +339,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 pH[r][1][0] = bld<T>(rpy, oK, hr * SB1 + so);
                 pH[r][1][1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
             }
-            if (ok && zpml) {
+            if (rv[r] && zany) {
                 const uint32_t so = (uint32_t)((pi * ny + gj[r]) * Lz) * TS;
                 pH[r][2][0] = bld<T>(rpz, oZ, hr * SB2 + so);
                 pH[r][2][1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
@@ -382,23 +368,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 pE[r][1][0] = bld<T>(rpy, oK, so);
                 pE[r][1][1] = bld<T>(rpy, oK, SB1 + so);
             }
-            if (zpml) {
+            if (zany) {
                 const uint32_t so = (uint32_t)((pi * ny + gj[r + 1]) * Lz) * TS;
                 pE[r][2][0] = bld<T>(rpz, oZ, so);
                 pE[r][2][1] = bld<T>(rpz, oZ, SB2 + so);
             }
             if (NP > 0 && !uniform) {
+                int np[3], lm[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int m = (mw[r] >> (8 * c)) & 0xff;
-                    const int np = (m == kDesignMat || !kin) ? 0 : slut[m].np;
-                    const int lm = m == kDesignMat ? 0 : slut[m].lorentz_mask;
+                    const int mc = m == kDesignMat ? 0 : m;  // LUT index (design voxels have no poles)
+                    np[c] = m == kDesignMat ? 0 : slut[mc].np;
+                    lm[c] = slut[mc].lorentz_mask;
+                }
 #pragma unroll
-                    for (int s = 0; s < NP; ++s) {
-                        if (s < np) {
-                            Qv[r][c][s] = bld<T>(rQ, o, (s * 3 + c) * PB);
-                            if ((lm >> s) & 1) Pv[r][c][s] = bld<T>(rP, o, (s * 3 + c) * PB);
-                        }
+                for (int s = 0; s < NP; ++s) {
+                    if (!wave_any(s < np[0] || s < np[1] || s < np[2])) continue;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        Qv[r][c][s] = bld<T>(rQ, s < np[c] ? o : kOOB, (s * 3 + c) * PB);
+                        Pv[r][c][s] = bld<T>(rP, (s < np[c] && ((lm[c] >> s) & 1)) ? o : kOOB, (s * 3 + c) * PB);
                     }
                 }
             }
@@ -422,13 +412,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 const T b = cy[2 * ny + j], c = cy[3 * ny + j];
                 dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
                 dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
-                if (wr && kin) {
+                if (wr) {
                     const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
                     bst(rpy, oK, hw * SB1 + so, pH[r][1][0]);
                     bst(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
                 }
             }
-            if (ok && zpml) {
+            if (ok && zany) {  // lanes outside the layers: b = a = 0 leaves the term unchanged
                 dEy_z = cpml(dEy_z, bHz, aHz, pH[r][2][0]);
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                 if (wr) {
@@ -440,7 +430,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             if (ok && xpml) {
                 dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
-                if (wr && kin) {
+                if (wr) {
                     bst(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
                     bst(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
                 }
@@ -449,7 +439,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             h[r][0] = ok ? h0[r][0] - cdt * (dEz_y - dEy_z) : T(0);
             h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
             h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
-            if (wr && kin) {
+            if (wr) {
                 bst(rW, oR[r], 3 * PB, h[r][0]);
                 bst(rW, oR[r], 4 * PB, h[r][1]);
                 bst(rW, oR[r], 5 * PB, h[r][2]);
@@ -484,13 +474,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                     const T b = cy[j], c = cy[ny + j];
                     dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
                     dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
-                    if (kin) {
-                        const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
-                        bst(rpy, oK, so, pE[r][1][0]);
-                        bst(rpy, oK, SB1 + so, pE[r][1][1]);
-                    }
+                    const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
+                    bst(rpy, oK, so, pE[r][1][0]);
+                    bst(rpy, oK, SB1 + so, pE[r][1][1]);
                 }
-                if (zpml) {
+                if (zany) {
                     dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
                     dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
@@ -500,53 +488,58 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 if (xpml) {
                     dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
                     dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
-                    if (kin) {
-                        bst(rpx, o, sxo, pE[r][0][0]);
-                        bst(rpx, o, SB0 + sxo, pE[r][0][1]);
-                    }
+                    bst(rpx, o, sxo, pE[r][0][0]);
+                    bst(rpx, o, SB0 + sxo, pE[r][0][1]);
                 }
                 const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
                 if (uniform) {
                     const T inv = slut[0].inv_eps;
-                    if (kin) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
-                    }
+                    for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
                 } else {
-                    T enew[3];
+                    int mm[3];
+                    bool des[3];
+                    T inv[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const int m = (mw[r] >> (8 * c)) & 0xff;
-                        const T e0 = ec[r + 1][c];
-                        T inv, sq = T(0);
-                        if (m == kDesignMat) {
-                            const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) +
-                                             (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
-                                            (kc - a.dlo[2]);
-                            inv = dinv[dq];
-                        } else {
-                            const MatLut<T>& L = slut[m];
-                            inv = L.inv_eps;
-                            const int npl = kin ? L.np : 0;  // lanes past the line store nothing
+                        des[c] = m == kDesignMat;
+                        mm[c] = des[c] ? 0 : m;
+                        inv[c] = slut[mm[c]].inv_eps;
+                    }
+                    if (wave_any(des[0] || des[1] || des[2])) {
+                        // design voxels: 1/eps from the design array [dx][dy][dz]
+                        const Rsrc rd = rsrc(a.dinv);
+                        const uint32_t dq = (uint32_t)(((pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
+                                                           (a.dhi[2] - a.dlo[2]) + (k - a.dlo[2])) * TS;
 #pragma unroll
-                            for (int s = 0; s < NP; ++s) {
-                                if (s < npl) {
-                                    T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * e0;
-                                    if ((L.lorentz_mask >> s) & 1) {
-                                        qn += L.beta[s] * Pv[r][c][s];
-                                        bst(rP, o, (s * 3 + c) * PB, Pv[r][c][s] + qn);
-                                    }
-                                    bst(rQ, o, (s * 3 + c) * PB, qn);
-                                    sq += qn;
-                                }
-                            }
+                        for (int c = 0; c < 3; ++c) {
+                            const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
+                            inv[c] = des[c] ? v : inv[c];
                         }
-                        enew[c] = e0 + inv * (cdt * curl[c] - sq);
                     }
-                    if (kin) {
+                    T sq[3] = {T(0), T(0), T(0)};
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, enew[c]);
+                    for (int s = 0; s < NP; ++s) {
+                        int npl[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) npl[c] = des[c] ? 0 : slut[mm[c]].np;
+                        if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const MatLut<T>& L = slut[mm[c]];
+                            const bool vq = s < npl[c];
+                            const bool vp = vq && ((L.lorentz_mask >> s) & 1);
+                            // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
+                            T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
+                            qn = vq ? qn : T(0);
+                            bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
+                            bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
+                            sq[c] += qn;
+                        }
                     }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv[c] * (cdt * curl[c] - sq[c]));
                 }
             }
         }
@@ -657,7 +650,7 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
         const Rsrc r0 = rsrc(F0 + (long)p0 * FPB);
         const Rsrc r1 = rsrc(F0 + (long)(p1 < 0 ? 0 : p1) * FPB);
-        const bool v1 = valid && p1 >= 0, t1 = jt >= 0 && p1 >= 0;
+        const bool v1 = valid && p1 >= 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T a0 = bld<T>(r0, oc, c * PB), a1 = bld<T>(r1, oc, c * PB), a2 = bld<T>(r0, oc, (3 + c) * PB);
@@ -665,16 +658,18 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             en[c] = v1 ? a1 : T(0);
             hc[c] = valid ? a2 : T(0);
         }
-        const T b0 = bld<T>(r0, ot, 0), b1 = bld<T>(r0, ot, 2 * PB);
-        const T b2 = bld<T>(r1, ot, 0), b3 = bld<T>(r1, ot, 2 * PB);
-        te[0] = jt >= 0 ? b0 : T(0);
-        te[1] = jt >= 0 ? b1 : T(0);
-        ten[0] = t1 ? b2 : T(0);
-        ten[1] = t1 ? b3 : T(0);
-        if (!uniform && istart >= x0) {
-            const uint32_t m0 = bld<uint32_t>(rsrc(matw + (long)p0 * PW), ow, 0);
-            mw = owned ? m0 : 0u;
+        // the row above the tile: only the top thread row (whole wavefronts when nz is a
+        // multiple of 64) loads it
+        te[0] = te[1] = ten[0] = ten[1] = T(0);
+        if (jt >= 0) {
+            te[0] = bld<T>(r0, ot, 0);
+            te[1] = bld<T>(r0, ot, 2 * PB);
+            if (p1 >= 0) {
+                ten[0] = bld<T>(r1, ot, 0);
+                ten[1] = bld<T>(r1, ot, 2 * PB);
+            }
         }
+        if (!uniform && istart >= x0 && owned) mw = bld<uint32_t>(rsrc(matw + (long)p0 * PW), ow, 0);
     }
 
     for (int i = istart; i < x1; ++i) {
@@ -698,19 +693,17 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                     const T v = bld<T>(rnn, oc, c * PB);
                     enn[c] = valid ? v : T(0);
                 }
-                const T u0 = bld<T>(rnn, ot, 0), u2 = bld<T>(rnn, ot, 2 * PB);
-                tenn[0] = jt >= 0 ? u0 : T(0);
-                tenn[1] = jt >= 0 ? u2 : T(0);
+                if (jt >= 0) {
+                    tenn[0] = bld<T>(rnn, ot, 0);
+                    tenn[1] = bld<T>(rnn, ot, 2 * PB);
+                }
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const T v = bld<T>(rn, oc, (3 + c) * PB);
                 hn[c] = valid ? v : T(0);
             }
-            if (!uniform) {
-                const uint32_t m = bld<uint32_t>(rsrc(matw + (long)pn * PW), ow, 0);
-                mwn = owned ? m : 0u;
-            }
+            if (!uniform && owned) mwn = bld<uint32_t>(rsrc(matw + (long)pn * PW), ow, 0);
         }
         const bool xpml = in_pml(a, 0, pi, nx);
         T bEx = T(0), aEx = T(0), bHx = T(0), aHx = T(0);
@@ -934,9 +927,14 @@ __device__ __forceinline__ int tile_instance(const int* __restrict__ tile_first,
 }
 
 // Fused step over all instances of a launch set (rows-per-thread form).
+// Waves per SIMD the register allocator must leave room for: the FP32 one-row tile with up to
+// three pole slots fits 96 registers (5 waves) without spilling; other variants are unconstrained.
+template <typename T, int TY, int NP>
+constexpr int step_waves() { return (sizeof(T) == 4 && TY == 1 && NP <= 3) ? 5 : 1; }
+
 template <typename T, int TY, int NP, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_step(const StepArgs<T>* __restrict__ args,
-                                               const int* __restrict__ tile_first, int ninst) {
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_waves<T, TY, NP>())))
+k_step(const StepArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
     int t;
     const int b = tile_instance(tile_first, ninst, t);
     step_tile<T, TY, NP>(args[b], t);
@@ -1168,10 +1166,10 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
 template <typename T>
 struct AdjArgs {
     StepArgs<T> f;     // forward description (geometry, materials, PML tables, tiles)
-    T* A[2];           // adjoint field sets [Ebar x,y,z, Hbar x,y,z][N]: read A[rd], write A[1 - rd]
+    T* A[2];           // adjoint field sets [x][Ebar x,y,z, Hbar x,y,z][y][z]: read A[rd], write A[1 - rd]
     int rd;
-    T* Aq;             // Qbar [slot][comp][N]
-    T* Ap;             // Pbar [slot][comp][N]
+    T* Aq;             // Qbar [x][slot * 3 + comp][y][z]
+    T* Ap;             // Pbar, the same layout
     T* Apsi[3];        // per axis [8][S]: E aux bar pair (0,1), H aux bar pairs (2,3) and (4,5),
                        // E aux bar pair (6,7)
     int er, ew;        // E aux bar pair read (0 or 6) / written (6 or 0)
@@ -1191,8 +1189,12 @@ __device__ __forceinline__ T inv_eps_cell(const StepArgs<T>& a, const MatLut<T>*
     return slut[m].inv_eps;
 }
 
+// One tile of the fused reverse step: a z line of one y row (j0), marching along x.  Addressing
+// as in step_tile (per-plane descriptors, lane offsets fixed for the march, uniform row validity,
+// lanes past the z line read its last cell and store nothing).
 template <typename T, int NP>
 __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
+    constexpr uint32_t TS = sizeof(T);
     const StepArgs<T>& a = g.f;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int B = blockDim.x;
@@ -1205,33 +1207,42 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         const int words = (int)(sizeof(MatLut<T>) / 4) * a.nmat;
         for (int q = threadIdx.x; q < words; q += B) dst[q] = src[q];
     }
-    GLOBAL const T* AR = (GLOBAL const T*)g.A[g.rd];
-    GLOBAL T* AW = (GLOBAL T*)g.A[1 - g.rd];
-    GLOBAL const uint32_t* matw = (GLOBAL const uint32_t*)a.matw;
-    GLOBAL T* Qb = (GLOBAL T*)g.Aq;
-    GLOBAL T* Pb = (GLOBAL T*)g.Ap;
-    GLOBAL T* psib[3] = {(GLOBAL T*)g.Apsi[0], (GLOBAL T*)g.Apsi[1], (GLOBAL T*)g.Apsi[2]};
+    const char* AR = (const char*)g.A[g.rd];
+    const char* AW = (const char*)g.A[1 - g.rd];
+    const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
-    const long N = a.N, NZ = a.nz, NYZ = (long)a.ny * a.nz;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int j0 = t % a.nty, xci = t / a.nty;
     const int x0 = xci * a.xc, x1 = min(x0 + a.xc, nx);
     const int k = threadIdx.x;
     const bool kin = k < nz;
-    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz), km1 = wrap_or_ghost(k - 1, nz, a.pz);
+    const int kc = kin ? k : nz - 1;
+    const int kp1 = wrap_or_ghost(kc + 1, nz, a.pz), km1 = wrap_or_ghost(kc - 1, nz, a.pz);
     const T cdt = a.cdt;
-    const int er = g.er, ew = g.ew, hr = g.hr, hw = g.hw;
-    // rows: 0 = j0 - 1 (halo), 1 = j0 (owned), 2 = j0 + 1 (its +y neighbour)
-    int gj[3];
-    bool rok[3], ypml[3];
+    const uint32_t er = g.er, ew = g.ew, hr = g.hr, hw = g.hw;
+    const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
+    const long FPB = 6L * PB, QPB = (long)a.nqp * PB, PW = (long)ny * nz * 4;
+    const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
+    const uint32_t SB0 = (uint32_t)(a.S[0] * TS), SB1 = (uint32_t)(a.S[1] * TS), SB2 = (uint32_t)(a.S[2] * TS);
+    const Rsrc rpx = rsrc(g.Apsi[0]), rpy = rsrc(g.Apsi[1]), rpz = rsrc(g.Apsi[2]);
+    // rows: 0 = j0 - 1 (halo), 1 = j0 (owned), 2 = j0 + 1 (its +y neighbour); uniform validity
+    int gj[3], sy[3];
+    bool rv[3], ypml[3];
+    uint32_t oR[3], oW[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
-        rok[r] = kin && gj[r] >= 0;
-        ypml[r] = rok[r] && in_pml(a, 1, gj[r], ny);
+        rv[r] = gj[r] >= 0;
+        ypml[r] = rv[r] && in_pml(a, 1, gj[r], ny);
+        sy[r] = ypml[r] ? (gj[r] < a.plo[1] ? gj[r] : gj[r] - (ny - a.phi[1]) + a.plo[1]) : 0;
+        const int jj = rv[r] ? gj[r] : 0;
+        oR[r] = (uint32_t)(jj * nz + kc) * TS;
+        oW[r] = (uint32_t)(jj * nz + kc) * 4u;
     }
     const bool zpml = kin && in_pml(a, 2, k, nz);
+    const uint32_t oK = (uint32_t)kc * TS;
+    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : 0u;
     GLOBAL const T* cy = coef + a.coff[1];
     GLOBAL const T* cz = coef + a.coff[2];
     GLOBAL const T* cx = coef + a.coff[0];
@@ -1244,13 +1255,25 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     uint32_t wn[3];
     auto load_raw = [&](int p, T (&raw)[3][3], uint32_t (&w)[3]) {
         const int pp = wrap_or_ghost(p, nx, a.px);
+        const long pc = pp >= 0 ? pp : 0;
+        const Rsrc rf = rsrc(AR + pc * FPB);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const bool ok = rok[r] && pp >= 0;
-            const long q = (long)(ok ? pp : 0) * NYZ + (long)(ok ? gj[r] : 0) * NZ + k;
+            const bool ok = rv[r] && pp >= 0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) raw[r][c] = ok ? AR[c * N + q] : T(0);
-            w[r] = (ok && !a.uniform) ? matw[q] : 0u;
+            for (int c = 0; c < 3; ++c) {
+                const T v = bld<T>(rf, oR[r], c * PB);
+                raw[r][c] = ok ? v : T(0);
+            }
+            w[r] = 0u;
+        }
+        if (!a.uniform) {
+            const Rsrc rm = rsrc(matw + pc * PW);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const uint32_t v = bld<uint32_t>(rm, oW[r], 0);
+                w[r] = (rv[r] && pp >= 0) ? v : 0u;
+            }
         }
     };
     auto to_u = [&](int p, const T (&raw)[3][3], const uint32_t (&w)[3], T (&u)[3][3]) {
@@ -1259,7 +1282,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                u[r][c] = (rok[r] && pp >= 0) ? inv_eps_cell(a, slut, dinv, w[r], c, pp, gj[r], k) * raw[r][c] : T(0);
+                u[r][c] = (rv[r] && pp >= 0) ? inv_eps_cell(a, slut, dinv, w[r], c, pp, gj[r], kc) * raw[r][c] : T(0);
     };
     lds_barrier();  // LUT in LDS
     {
@@ -1280,8 +1303,12 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         if (has_next) load_raw(i + 2, rn, wn);
         const bool xpml = in_pml(a, 0, pi, nx);
         const bool xpmln = pn >= 0 && in_pml(a, 0, pn, nx);
+        const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
+        const uint32_t sxn = xpmln ? (uint32_t)(pn < a.plo[0] ? pn : pn - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
         const T bEx = xpml ? cx[pi] : T(1), bHx = xpml ? cx[2 * nx + pi] : T(1);
         const T bExn = xpmln ? cx[pn] : T(1);
+        const Rsrc rA = rsrc(AR + (long)pi * FPB);
+        const Rsrc rB = rsrc(AW + (long)pi * FPB);
 
         // ---- phase 1: E-terms and Hbar_total on rows -1 and 0 of plane i ----------------
         // E-terms on rows r = 0, 1 (labels: y (Ex->Hz), z (Ex->Hy), z (Ey->Hx), x (Ey->Hz),
@@ -1289,28 +1316,26 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         T eb[3][6];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
+            const bool ok = rv[r];
             const int j = gj[r];
-            const bool ok = rok[r];
-            const long q = (long)pi * NYZ + (long)(ok ? j : 0) * NZ + k;
             const T ux = uc[r][0], uy = uc[r][1], uz = uc[r][2];
             T t_ex_y = cdt * ux, t_ex_z = -cdt * ux, t_ey_z = cdt * uy, t_ey_x = -cdt * uy, t_ez_x = cdt * uz,
               t_ez_y = -cdt * uz;
-            if (ok && ypml[r]) {
+            if (ypml[r]) {
                 const T b = cy[j];
-                const long s = slab_y(a, pi, j, k);
-                t_ex_y = b * t_ex_y + (b - T(1)) * psib[1][(er + 0) * a.S[1] + s];
-                t_ez_y = b * t_ez_y + (b - T(1)) * psib[1][(er + 1) * a.S[1] + s];
+                const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
+                t_ex_y = b * t_ex_y + (b - T(1)) * bld<T>(rpy, oK, er * SB1 + so);
+                t_ez_y = b * t_ez_y + (b - T(1)) * bld<T>(rpy, oK, (er + 1) * SB1 + so);
             }
             if (r < 2) {
                 if (ok && zpml) {
-                    const long s = slab_z(a, pi, j, k);
-                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * psib[2][(er + 0) * a.S[2] + s];
-                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * psib[2][(er + 1) * a.S[2] + s];
+                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * bld<T>(rpz, oZ, er * SB2 + so);
+                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * bld<T>(rpz, oZ, (er + 1) * SB2 + so);
                 }
                 if (ok && xpml) {
-                    const long s = slab_x(a, pi, j, k);
-                    t_ey_x = bEx * t_ey_x + (bEx - T(1)) * psib[0][(er + 0) * a.S[0] + s];
-                    t_ez_x = bEx * t_ez_x + (bEx - T(1)) * psib[0][(er + 1) * a.S[0] + s];
+                    t_ey_x = bEx * t_ey_x + (bEx - T(1)) * bld<T>(rpx, oR[r], er * SB0 + sxo);
+                    t_ez_x = bEx * t_ez_x + (bEx - T(1)) * bld<T>(rpx, oR[r], (er + 1) * SB0 + sxo);
                 }
             }
             eb[r][0] = t_ex_y;
@@ -1319,21 +1344,18 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             eb[r][3] = t_ey_x;
             eb[r][4] = t_ez_x;
             eb[r][5] = t_ez_y;
-            (void)q;
         }
         // x-terms of plane i + 1, rows 0 and 1
         T ebn[2][2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            const int j = gj[r];
             T a1 = -cdt * un[r][1], a2 = cdt * un[r][2];  // (Ey -> Hz), (Ez -> Hy)
-            if (rok[r] && xpmln) {
-                const long s = slab_x(a, pn, j, k);
-                a1 = bExn * a1 + (bExn - T(1)) * psib[0][(er + 0) * a.S[0] + s];
-                a2 = bExn * a2 + (bExn - T(1)) * psib[0][(er + 1) * a.S[0] + s];
+            if (rv[r] && xpmln) {
+                a1 = bExn * a1 + (bExn - T(1)) * bld<T>(rpx, oR[r], er * SB0 + sxn);
+                a2 = bExn * a2 + (bExn - T(1)) * bld<T>(rpx, oR[r], (er + 1) * SB0 + sxn);
             }
-            ebn[r][0] = (rok[r] && pn >= 0) ? a1 : T(0);
-            ebn[r][1] = (rok[r] && pn >= 0) ? a2 : T(0);
+            ebn[r][0] = (rv[r] && pn >= 0) ? a1 : T(0);
+            ebn[r][1] = (rv[r] && pn >= 0) ? a2 : T(0);
         }
         // z-terms through LDS
         T* bufe = se + (i & 1) * 4 * B;
@@ -1346,42 +1368,38 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         T tot[2][3];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            const int j = gj[r];
-            const bool ok = rok[r];
-            const long q = (long)pi * NYZ + (long)(ok ? j : 0) * NZ + k;
+            const bool ok = rv[r];
             const T exz_k = kp1 >= 0 ? bufe[(2 * r + 0) * B + kp1] : T(0);
             const T eyz_k = kp1 >= 0 ? bufe[(2 * r + 1) * B + kp1] : T(0);
             // Hx: (Ey, z) and (Ez, y); Hy: (Ex, z) and (Ez, x); Hz: (Ex, y) and (Ey, x)
-            const T hx = ok ? AR[3 * N + q] : T(0);
-            const T hy = ok ? AR[4 * N + q] : T(0);
-            const T hz = ok ? AR[5 * N + q] : T(0);
+            const T hx = bld<T>(rA, oR[r], 3 * PB);
+            const T hy = bld<T>(rA, oR[r], 4 * PB);
+            const T hz = bld<T>(rA, oR[r], 5 * PB);
             tot[r][0] = ok ? hx + (eb[r][2] - eyz_k) + (eb[r][5] - eb[r + 1][5]) : T(0);
             tot[r][1] = ok ? hy + (eb[r][1] - exz_k) + (eb[r][4] - ebn[r][1]) : T(0);
             tot[r][2] = ok ? hz + (eb[r][0] - eb[r + 1][0]) + (eb[r][3] - ebn[r][0]) : T(0);
         }
         // own row: Hbar at the old time and the new H auxiliary bars
-        const bool wr = own_plane && rok[1];
-        if (wr) {
+        const bool wr = own_plane && rv[1];
+        if (wr && kin) {
             const int j = gj[1];
-            const long q = (long)pi * NYZ + (long)j * NZ + k;
-            AW[3 * N + q] = tot[1][0];
-            AW[4 * N + q] = tot[1][1];
-            AW[5 * N + q] = tot[1][2];
+            bst(rB, oR[1], 3 * PB, tot[1][0]);
+            bst(rB, oR[1], 4 * PB, tot[1][1]);
+            bst(rB, oR[1], 5 * PB, tot[1][2]);
             if (ypml[1]) {  // (Hx, y, +, slot 0), (Hz, y, -, slot 1)
                 const T b = cy[2 * ny + j];
-                const long s = slab_y(a, pi, j, k);
-                psib[1][(hw + 0) * a.S[1] + s] = b * (-cdt * tot[1][0] + psib[1][(hr + 0) * a.S[1] + s]);
-                psib[1][(hw + 1) * a.S[1] + s] = b * (cdt * tot[1][2] + psib[1][(hr + 1) * a.S[1] + s]);
+                const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
+                bst(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + bld<T>(rpy, oK, hr * SB1 + so)));
+                bst(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + bld<T>(rpy, oK, (hr + 1) * SB1 + so)));
             }
             if (zpml) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
-                const long s = slab_z(a, pi, j, k);
-                psib[2][(hw + 0) * a.S[2] + s] = bHz * (cdt * tot[1][0] + psib[2][(hr + 0) * a.S[2] + s]);
-                psib[2][(hw + 1) * a.S[2] + s] = bHz * (-cdt * tot[1][1] + psib[2][(hr + 1) * a.S[2] + s]);
+                const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + bld<T>(rpz, oZ, hr * SB2 + so)));
+                bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + bld<T>(rpz, oZ, (hr + 1) * SB2 + so)));
             }
             if (xpml) {  // (Hy, x, -, slot 0), (Hz, x, +, slot 1)
-                const long s = slab_x(a, pi, j, k);
-                psib[0][(hw + 0) * a.S[0] + s] = bHx * (cdt * tot[1][1] + psib[0][(hr + 0) * a.S[0] + s]);
-                psib[0][(hw + 1) * a.S[0] + s] = bHx * (-cdt * tot[1][2] + psib[0][(hr + 1) * a.S[0] + s]);
+                bst(rpx, oR[1], hw * SB0 + sxo, bHx * (cdt * tot[1][1] + bld<T>(rpx, oR[1], hr * SB0 + sxo)));
+                bst(rpx, oR[1], (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[1][2] + bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo)));
             }
         }
 
@@ -1392,21 +1410,20 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             const int j = gj[1];
             T d_hx_y = -cdt * tot[1][0], d_hx_z = cdt * tot[1][0], d_hy_z = -cdt * tot[1][1], d_hy_x = cdt * tot[1][1],
               d_hz_x = -cdt * tot[1][2], d_hz_y = cdt * tot[1][2];
-            if (rok[1] && ypml[1]) {
+            if (ypml[1]) {
                 const T b = cy[2 * ny + j];
-                const long s = slab_y(a, pi, j, k);
-                d_hx_y = b * d_hx_y + (b - T(1)) * psib[1][(hr + 0) * a.S[1] + s];
-                d_hz_y = b * d_hz_y + (b - T(1)) * psib[1][(hr + 1) * a.S[1] + s];
+                const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
+                d_hx_y = b * d_hx_y + (b - T(1)) * bld<T>(rpy, oK, hr * SB1 + so);
+                d_hz_y = b * d_hz_y + (b - T(1)) * bld<T>(rpy, oK, (hr + 1) * SB1 + so);
             }
-            if (rok[1] && zpml) {
-                const long s = slab_z(a, pi, j, k);
-                d_hx_z = bHz * d_hx_z + (bHz - T(1)) * psib[2][(hr + 0) * a.S[2] + s];
-                d_hy_z = bHz * d_hy_z + (bHz - T(1)) * psib[2][(hr + 1) * a.S[2] + s];
+            if (rv[1] && zpml) {
+                const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                d_hx_z = bHz * d_hx_z + (bHz - T(1)) * bld<T>(rpz, oZ, hr * SB2 + so);
+                d_hy_z = bHz * d_hy_z + (bHz - T(1)) * bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
             }
-            if (rok[1] && xpml) {
-                const long s = slab_x(a, pi, j, k);
-                d_hy_x = bHx * d_hy_x + (bHx - T(1)) * psib[0][(hr + 0) * a.S[0] + s];
-                d_hz_x = bHx * d_hz_x + (bHx - T(1)) * psib[0][(hr + 1) * a.S[0] + s];
+            if (rv[1] && xpml) {
+                d_hy_x = bHx * d_hy_x + (bHx - T(1)) * bld<T>(rpx, oR[1], hr * SB0 + sxo);
+                d_hz_x = bHx * d_hz_x + (bHx - T(1)) * bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo);
             }
             db[0] = d_hx_y;
             db[1] = d_hx_z;
@@ -1418,14 +1435,14 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         {
             const int j = gj[0];
             T d_hx_y = -cdt * tot[0][0], d_hz_y = cdt * tot[0][2];
-            if (rok[0] && ypml[0]) {
+            if (ypml[0]) {
                 const T b = cy[2 * ny + j];
-                const long s = slab_y(a, pi, j, k);
-                d_hx_y = b * d_hx_y + (b - T(1)) * psib[1][(hr + 0) * a.S[1] + s];
-                d_hz_y = b * d_hz_y + (b - T(1)) * psib[1][(hr + 1) * a.S[1] + s];
+                const uint32_t so = (uint32_t)((pi * Ly + sy[0]) * nz) * TS;
+                d_hx_y = b * d_hx_y + (b - T(1)) * bld<T>(rpy, oK, hr * SB1 + so);
+                d_hz_y = b * d_hz_y + (b - T(1)) * bld<T>(rpy, oK, (hr + 1) * SB1 + so);
             }
-            dbh[0] = rok[0] ? d_hx_y : T(0);
-            dbh[1] = rok[0] ? d_hz_y : T(0);
+            dbh[0] = rv[0] ? d_hx_y : T(0);
+            dbh[1] = rv[0] ? d_hz_y : T(0);
         }
         if (own_plane) {
             T* bufd = sd + (i & 1) * 2 * B;
@@ -1434,17 +1451,19 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             lds_barrier();
             if (wr) {
                 const int j = gj[1];
-                const long q = (long)pi * NYZ + (long)j * NZ + k;
+                const uint32_t o = oR[1];
                 const T hyz_k = km1 >= 0 ? bufd[0 * B + km1] : T(0);
                 const T hxz_k = km1 >= 0 ? bufd[1 * B + km1] : T(0);
                 // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
                 T out[3];
-                out[0] = AR[q] + (hyz_k - db[2]) + (dbh[1] - db[5]);
-                out[1] = AR[N + q] + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
-                out[2] = AR[2 * N + q] + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
+                out[0] = bld<T>(rA, o, 0) + (hyz_k - db[2]) + (dbh[1] - db[5]);
+                out[1] = bld<T>(rA, o, PB) + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
+                out[2] = bld<T>(rA, o, 2 * PB) + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
                 const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
-                if (NP > 0 && !a.uniform) {
-                    const uint32_t mw = matw[q];
+                if (NP > 0 && !a.uniform && kin) {
+                    const uint32_t mw = bld<uint32_t>(rsrc(matw + (long)pi * PW), oW[1], 0);
+                    const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
+                    const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const int m = (mw >> (8 * c)) & 0xff;
@@ -1453,42 +1472,43 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
 #pragma unroll
                         for (int s = 0; s < NP; ++s) {
                             if (s >= L.np) continue;
-                            const long o = (long)(s * 3 + c) * N + q;
+                            const uint32_t so = (s * 3 + c) * PB;
                             const bool lor = (L.lorentz_mask >> s) & 1;
-                            const T pb = lor ? Pb[o] : T(0);
-                            const T qb = Qb[o] + pb - u0[c];
-                            Qb[o] = L.alpha[s] * qb;
-                            if (lor) Pb[o] = pb + L.beta[s] * qb;
+                            const T pb = lor ? bld<T>(rP, o, so) : T(0);
+                            const T qb = bld<T>(rQ, o, so) + pb - u0[c];
+                            bst(rQ, o, so, L.alpha[s] * qb);
+                            if (lor) bst(rP, o, so, pb + L.beta[s] * qb);
                             out[c] += L.c3[s] * qb;
                         }
                     }
                 }
-                // new E auxiliary bars: E-terms of row 0 with their signs
-                if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
-                    const T b = cy[j];
-                    const long s = slab_y(a, pi, j, k);
-                    psib[1][(ew + 0) * a.S[1] + s] = b * (cdt * u0[0] + psib[1][(er + 0) * a.S[1] + s]);
-                    psib[1][(ew + 1) * a.S[1] + s] = b * (-cdt * u0[2] + psib[1][(er + 1) * a.S[1] + s]);
+                if (kin) {
+                    // new E auxiliary bars: E-terms of row 0 with their signs
+                    if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
+                        const T b = cy[j];
+                        const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
+                        bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + bld<T>(rpy, oK, er * SB1 + so)));
+                        bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + bld<T>(rpy, oK, (er + 1) * SB1 + so)));
+                    }
+                    if (zpml) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
+                        const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                        bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + bld<T>(rpz, oZ, er * SB2 + so)));
+                        bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + bld<T>(rpz, oZ, (er + 1) * SB2 + so)));
+                    }
+                    if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
+                        bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + bld<T>(rpx, o, er * SB0 + sxo)));
+                        bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + bld<T>(rpx, o, (er + 1) * SB0 + sxo)));
+                    }
+                    bst(rB, o, 0, out[0]);
+                    bst(rB, o, PB, out[1]);
+                    bst(rB, o, 2 * PB, out[2]);
                 }
-                if (zpml) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
-                    const long s = slab_z(a, pi, j, k);
-                    psib[2][(ew + 0) * a.S[2] + s] = bEz * (-cdt * u0[0] + psib[2][(er + 0) * a.S[2] + s]);
-                    psib[2][(ew + 1) * a.S[2] + s] = bEz * (cdt * u0[1] + psib[2][(er + 1) * a.S[2] + s]);
-                }
-                if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
-                    const long s = slab_x(a, pi, j, k);
-                    psib[0][(ew + 0) * a.S[0] + s] = bEx * (-cdt * u0[1] + psib[0][(er + 0) * a.S[0] + s]);
-                    psib[0][(ew + 1) * a.S[0] + s] = bEx * (cdt * u0[2] + psib[0][(er + 1) * a.S[0] + s]);
-                }
-                AW[q] = out[0];
-                AW[N + q] = out[1];
-                AW[2 * N + q] = out[2];
             }
         } else {
             lds_barrier();  // keep the barrier count uniform across iterations
         }
-        dbx_prev[0] = rok[1] ? db[4] : T(0);
-        dbx_prev[1] = rok[1] ? db[3] : T(0);
+        dbx_prev[0] = rv[1] ? db[4] : T(0);
+        dbx_prev[1] = rv[1] ? db[3] : T(0);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1541,7 +1561,7 @@ __global__ void k_adj_objective(const T* E, long nyz, long gap, double dt, doubl
 
 // gradient term of step n:
 //   gacc[v] += sum_c Ebar_c^{n+1}(v) (E_c^{n+1}(v) - E_c^n(v))   (before the adjoint step)
-// Ebar: the adjoint field set holding Ebar^{n+1} ([comp][N]).
+// Ebar: the adjoint field set holding Ebar^{n+1} ([x][comp][y][z]).
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_grad(const StepArgs<T> a, const T* Ebar, const T* ring, long s_next,
                                                   long s_cur, double* gacc) {
@@ -1558,17 +1578,18 @@ __global__ void __launch_bounds__(256) k_adj_grad(const StepArgs<T> a, const T* 
     double acc = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        acc += (double)Ebar[(long)c * a.N + q] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
+        acc += (double)Ebar[field_index(q, c, (long)a.ny * a.nz)] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
     gacc[v] += acc;
 }
 
 // Ebar^n(gap) += dJ/dE^n = 2 dt E^n(gap)
 template <typename T>
-__global__ void k_adj_inject(T* Ebar, long N, const T* ring, long nd, long slot, long gap, double dt) {
+__global__ void k_adj_inject(T* Ebar, long nyz, const T* ring, long nd, long slot, long gap, double dt) {
     const int c = threadIdx.x;
     if (blockIdx.x == 0 && c < 3) {
         const double e = (double)ring[slot * (3 * nd + 3) + 3 * nd + c];
-        Ebar[(long)c * N + gap] = (T)((double)Ebar[(long)c * N + gap] + 2.0 * dt * e);
+        const long o = field_index(gap, c, nyz);
+        Ebar[o] = (T)((double)Ebar[o] + 2.0 * dt * e);
     }
 }
 
