@@ -221,6 +221,10 @@ int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int 
  * sjf_read_* calls (which wait for the batch) but cannot be stepped on their own until the
  * batch is destroyed. */
 int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out);
+/* Tile form of the batch's fused launch (SJF_TILE_REGISTER_ROWS with one row per thread, or
+ * SJF_TILE_THREAD_ROWS with each instance's default rows) and the x-chunk of every instance
+ * (0: each engine's own).  New (tuning of batched runs). */
+int sjf_batch_set_tile(sjf_batch* b, int form, int xchunk);
 int sjf_batch_step(sjf_batch* b, long n);
 int sjf_batch_synchronize(sjf_batch* b);
 int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean);

@@ -32,7 +32,7 @@ EXPORTS = [
     "sjf_get_field", "sjf_add_dft", "sjf_read_dft", "sjf_step", "sjf_synchronize",
     "sjf_timed_steps", "sjf_read_field", "sjf_write_field", "sjf_step_bytes",
     "sjf_launch_info", "sjf_set_tile", "sjf_copy_probe", "sjf_set_graph",
-    "sjf_batch_create", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
+    "sjf_batch_create", "sjf_batch_set_tile", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
     "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
 ]
 
@@ -113,6 +113,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
         "sjf_set_graph": (C.c_int, [vp, C.c_int]),
         "sjf_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+        "sjf_batch_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
         "sjf_batch_step": (C.c_int, [vp, C.c_long]),
         "sjf_batch_synchronize": (C.c_int, [vp]),
         "sjf_batch_timed_steps": (C.c_int, [vp, C.c_long, dp, dp]),

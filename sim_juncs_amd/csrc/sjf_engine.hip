@@ -242,9 +242,9 @@ int block_threads_for(const sjf_engine* e, int form, int ty) {
 
 int block_threads(const sjf_engine* e) { return block_threads_for(e, e->form, e->ty); }
 
-int tiles_of(const sjf_engine* e, int ty) {
+int tiles_of(const sjf_engine* e, int ty, int xc) {
     const int nty = (e->g.ny + ty - 1) / ty;
-    const int nxc = (e->g.nx + e->xc - 1) / e->xc;
+    const int nxc = (e->g.nx + xc - 1) / xc;
     return nty * nxc;
 }
 
@@ -258,7 +258,7 @@ int np_variant_of(int maxp, bool uniform) {
 }
 
 template <typename T>
-sjf::StepArgs<T> step_args(const sjf_engine* e, int ty) {
+sjf::StepArgs<T> step_args(const sjf_engine* e, int ty, int xc) {
     sjf::StepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.rt = ty + 1;
@@ -269,7 +269,7 @@ sjf::StepArgs<T> step_args(const sjf_engine* e, int ty) {
     a.py = e->g.periodic[1];
     a.pz = e->g.periodic[2];
     a.nty = (e->g.ny + ty - 1) / ty;
-    a.xc = e->xc;
+    a.xc = xc;
     a.N = e->N;
     a.cdt = (T)(e->dt / e->g.dx);
     a.F[0] = (T*)e->Fbuf[0];
@@ -305,6 +305,7 @@ struct LaunchSet {
     bool own_stream = false;
     int prec = 4, ty = 1, npv = 0, block = 64;
     int form = SJF_TILE_REGISTER_ROWS;
+    int xc = 0;  // x-chunk of every instance of a batch (0: each engine's own)
     size_t lds = 0;
     std::vector<long> versions;
     bool built = false;
@@ -382,9 +383,10 @@ int build_set_t(LaunchSet* s) {
         // the engine's own set runs its tuned rows; a batch picks rows per engine for its form
         const int rows = (s == e->own) ? e->ty
                                        : (s->form == SJF_TILE_THREAD_ROWS ? default_thread_rows(e) : s->ty);
-        args.push_back(step_args<T>(e, rows));
+        const int xc = (s == e->own || s->xc <= 0) ? e->xc : std::min(s->xc, e->g.nx);
+        args.push_back(step_args<T>(e, rows, xc));
         first.push_back(t);
-        t += tiles_of(e, rows);
+        t += tiles_of(e, rows, xc);
         maxp = std::max(maxp, e->maxp);
         all_uniform = all_uniform && e->uniform;
         block = std::max(block, block_threads_for(e, s->form, rows));
@@ -794,14 +796,14 @@ int timed_set(LaunchSet* s, long n, double* ms_total, double* ms_kernel_mean) {
 
 void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
-    // Defaults measured on MI355X: for short z lines (nz <= 128) the row-of-threads form wins
-    // (up to ~1024 threads per workgroup, few halo rows); for longer lines one row per thread
+    // Defaults measured on MI355X: for short z lines (nz <= 96) the row-of-threads form wins
+    // (up to ~1024 threads per workgroup, few halo rows); from 128 cells on one row per thread
     // keeps more waves resident.  The autotuner refines per workload.
     long target_groups = 16384;
     e->form = SJF_TILE_REGISTER_ROWS;
     e->ty = 1;
     const int rows = e->g.nz > 96 ? default_thread_rows(e) : std::min(3, default_thread_rows(e));
-    if (e->g.nz <= 128 && thread_rows_fit(e, rows)) {
+    if (e->g.nz <= 96 && thread_rows_fit(e, rows)) {
         e->form = SJF_TILE_THREAD_ROWS;
         e->ty = rows;
         target_groups = 2048;
@@ -1092,8 +1094,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     // ---- backward pass, segment by segment
     sjf::AdjArgs<T> g;
     std::memset(&g, 0, sizeof(g));
-    g.f = step_args<T>(e, 1);  // one row per thread; x-chunks of adj_xchunk planes
-    g.f.xc = adj_xchunk(e);
+    g.f = step_args<T>(e, 1, adj_xchunk(e));  // one y row per workgroup
     g.f.nty = e->g.ny;
     g.A[0] = A;
     g.A[1] = A + (size_t)6 * e->N;
@@ -1692,7 +1693,7 @@ int sjf_launch_info(const sjf_engine* e, int* form, int* ty, int* xchunk, int* t
     if (ty) *ty = e->ty;
     if (xchunk) *xchunk = e->xc;
     if (threads) *threads = block_threads(e);
-    if (groups) *groups = tiles_of(e, e->ty);
+    if (groups) *groups = tiles_of(e, e->ty, e->xc);
     return SJF_OK;
 }
 
@@ -1765,15 +1766,41 @@ int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out) {
     s->own_stream = true;
     s->prec = e0->prec;
     s->ty = 1;
+    // measured on MI355X for a sweep of 64 instances (64^3 to 192x192x128): the row-of-threads
+    // form with 8-plane x-chunks beats one row per thread and the per-instance chunks
     s->form = SJF_TILE_THREAD_ROWS;
+    s->xc = 8;
     for (int i = 0; i < n; ++i)
-        if (!thread_rows_fit(engines[i], default_thread_rows(engines[i]))) s->form = SJF_TILE_REGISTER_ROWS;
+        if (!thread_rows_fit(engines[i], default_thread_rows(engines[i]))) {
+            s->form = SJF_TILE_REGISTER_ROWS;
+            s->xc = 0;
+        }
     for (int i = 0; i < n; ++i) {
         s->eng.push_back(engines[i]);
         engines[i]->batch = b;
     }
     b->set = s;
     *out = b;
+    return SJF_OK;
+}
+
+int sjf_batch_set_tile(sjf_batch* b, int form, int xchunk) {
+    if (!b || xchunk < 0) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    if (form == SJF_TILE_THREAD_ROWS) {
+        for (sjf_engine* e : s->eng)
+            if (!thread_rows_fit(e, default_thread_rows(e))) {
+                b->err = "an instance does not fit the row-of-threads tile";
+                return SJF_E_ARG;
+            }
+    } else if (form != SJF_TILE_REGISTER_ROWS) {
+        b->err = "unknown tile form";
+        return SJF_E_ARG;
+    }
+    s->form = form;
+    s->ty = 1;
+    s->xc = xchunk;
+    s->built = false;
     return SJF_OK;
 }
 

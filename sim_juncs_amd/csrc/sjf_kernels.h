@@ -304,11 +304,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         T enn[TY + 2][3];
         uint32_t mwn[TY];
         {
-            // plane i + 2 (a ghost plane reads as zero; after the last plane the values are unused)
-            const Rsrc rn = rsrc(F0 + (long)(pnn >= 0 ? pnn : pi) * FPB);
+            // plane i + 2 (a ghost plane reads as zero; after the last plane nothing is read)
+            const bool nn_ok = has_next && pnn >= 0;
+            const Rsrc rn = rsrc(F0 + (long)(nn_ok ? pnn : pi) * FPB);
 #pragma unroll
             for (int r = 0; r < TY + 2; ++r) {
-                const uint32_t o = pnn >= 0 ? oR[r] : kOOB;
+                const uint32_t o = nn_ok ? oR[r] : kOOB;
                 enn[r][0] = bld<T>(rn, o, 0);
                 enn[r][1] = (r < TY + 1) ? bld<T>(rn, o, PB) : T(0);
                 enn[r][2] = bld<T>(rn, o, 2 * PB);
@@ -1190,8 +1191,9 @@ __device__ __forceinline__ T inv_eps_cell(const StepArgs<T>& a, const MatLut<T>*
 }
 
 // One tile of the fused reverse step: a z line of one y row (j0), marching along x.  Addressing
-// as in step_tile (per-plane descriptors, lane offsets fixed for the march, uniform row validity,
-// lanes past the z line read its last cell and store nothing).
+// and predication as in step_tile: per-plane descriptors, lane offsets fixed for the march, the
+// out-of-range offset for ghost rows and for lanes past the z line, outside a z PML layer, a
+// pole slot or the design region; whole-wavefront conditions are uniform branches.
 template <typename T, int NP>
 __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     constexpr uint32_t TS = sizeof(T);
@@ -1211,15 +1213,14 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     const char* AW = (const char*)g.A[1 - g.rd];
     const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
-    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int j0 = t % a.nty, xci = t / a.nty;
     const int x0 = xci * a.xc, x1 = min(x0 + a.xc, nx);
     const int k = threadIdx.x;
     const bool kin = k < nz;
-    const int kc = kin ? k : nz - 1;
-    const int kp1 = wrap_or_ghost(kc + 1, nz, a.pz), km1 = wrap_or_ghost(kc - 1, nz, a.pz);
+    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz), km1 = wrap_or_ghost(k - 1, nz, a.pz);
     const T cdt = a.cdt;
+    const bool uniform = a.uniform != 0;
     const uint32_t er = g.er, ew = g.ew, hr = g.hr, hw = g.hw;
     const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
     const long FPB = 6L * PB, QPB = (long)a.nqp * PB, PW = (long)ny * nz * 4;
@@ -1236,17 +1237,20 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         rv[r] = gj[r] >= 0;
         ypml[r] = rv[r] && in_pml(a, 1, gj[r], ny);
         sy[r] = ypml[r] ? (gj[r] < a.plo[1] ? gj[r] : gj[r] - (ny - a.phi[1]) + a.plo[1]) : 0;
-        const int jj = rv[r] ? gj[r] : 0;
-        oR[r] = (uint32_t)(jj * nz + kc) * TS;
-        oW[r] = (uint32_t)(jj * nz + kc) * 4u;
+        const bool ok = rv[r] && kin;
+        oR[r] = ok ? (uint32_t)(gj[r] * nz + k) * TS : kOOB;
+        oW[r] = ok ? (uint32_t)(gj[r] * nz + k) * 4u : kOOB;
     }
     const bool zpml = kin && in_pml(a, 2, k, nz);
-    const uint32_t oK = (uint32_t)kc * TS;
-    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : 0u;
+    const bool zany = wave_any(zpml);
+    const uint32_t oK = kin ? (uint32_t)k * TS : kOOB;
+    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : kOOB;
     GLOBAL const T* cy = coef + a.coff[1];
     GLOBAL const T* cz = coef + a.coff[2];
     GLOBAL const T* cx = coef + a.coff[0];
     const T bEz = zpml ? cz[k] : T(1), bHz = zpml ? cz[2 * nz + k] : T(1);
+    const Rsrc rd = rsrc(a.dinv);
+    const int dey = a.dhi[1] - a.dlo[1], dez = a.dhi[2] - a.dlo[2];
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
     // u = inv_eps * Ebar' on planes i and i + 1 (rows 0..2), raw Ebar' and material words of
@@ -1257,32 +1261,48 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         const int pp = wrap_or_ghost(p, nx, a.px);
         const long pc = pp >= 0 ? pp : 0;
         const Rsrc rf = rsrc(AR + pc * FPB);
+        const Rsrc rm = rsrc(matw + pc * PW);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const bool ok = rv[r] && pp >= 0;
+            const uint32_t o = pp >= 0 ? oR[r] : kOOB;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T v = bld<T>(rf, oR[r], c * PB);
-                raw[r][c] = ok ? v : T(0);
-            }
+            for (int c = 0; c < 3; ++c) raw[r][c] = bld<T>(rf, o, c * PB);
             w[r] = 0u;
-        }
-        if (!a.uniform) {
-            const Rsrc rm = rsrc(matw + pc * PW);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const uint32_t v = bld<uint32_t>(rm, oW[r], 0);
-                w[r] = (rv[r] && pp >= 0) ? v : 0u;
-            }
+            if (!uniform) w[r] = bld<uint32_t>(rm, pp >= 0 ? oW[r] : kOOB, 0);
         }
     };
     auto to_u = [&](int p, const T (&raw)[3][3], const uint32_t (&w)[3], T (&u)[3][3]) {
         const int pp = wrap_or_ghost(p, nx, a.px);
+        if (uniform) {
+            const T inv = slut[0].inv_eps;
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                u[r][c] = (rv[r] && pp >= 0) ? inv_eps_cell(a, slut, dinv, w[r], c, pp, gj[r], kc) * raw[r][c] : T(0);
+                for (int c = 0; c < 3; ++c) u[r][c] = inv * raw[r][c];
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            bool des[3];
+            T inv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int m = (w[r] >> (8 * c)) & 0xff;
+                des[c] = m == kDesignMat;
+                inv[c] = slut[des[c] ? 0 : m].inv_eps;
+            }
+            if (wave_any(des[0] || des[1] || des[2])) {
+                const uint32_t dq =
+                    (uint32_t)(((pp - a.dlo[0]) * dey + (gj[r] - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
+                    inv[c] = des[c] ? v : inv[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) u[r][c] = inv[c] * raw[r][c];
+        }
     };
     lds_barrier();  // LUT in LDS
     {
@@ -1328,7 +1348,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 t_ez_y = b * t_ez_y + (b - T(1)) * bld<T>(rpy, oK, (er + 1) * SB1 + so);
             }
             if (r < 2) {
-                if (ok && zpml) {
+                if (ok && zany) {  // lanes outside the layers: b = 1 and a zero auxiliary
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
                     t_ex_z = bEz * t_ex_z + (bEz - T(1)) * bld<T>(rpz, oZ, er * SB2 + so);
                     t_ey_z = bEz * t_ey_z + (bEz - T(1)) * bld<T>(rpz, oZ, (er + 1) * SB2 + so);
@@ -1381,7 +1401,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         }
         // own row: Hbar at the old time and the new H auxiliary bars
         const bool wr = own_plane && rv[1];
-        if (wr && kin) {
+        if (wr) {
             const int j = gj[1];
             bst(rB, oR[1], 3 * PB, tot[1][0]);
             bst(rB, oR[1], 4 * PB, tot[1][1]);
@@ -1392,7 +1412,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 bst(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + bld<T>(rpy, oK, hr * SB1 + so)));
                 bst(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + bld<T>(rpy, oK, (hr + 1) * SB1 + so)));
             }
-            if (zpml) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
+            if (zany) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
                 const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
                 bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + bld<T>(rpz, oZ, hr * SB2 + so)));
                 bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + bld<T>(rpz, oZ, (hr + 1) * SB2 + so)));
@@ -1416,7 +1436,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 d_hx_y = b * d_hx_y + (b - T(1)) * bld<T>(rpy, oK, hr * SB1 + so);
                 d_hz_y = b * d_hz_y + (b - T(1)) * bld<T>(rpy, oK, (hr + 1) * SB1 + so);
             }
-            if (rv[1] && zpml) {
+            if (rv[1] && zany) {
                 const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
                 d_hx_z = bHz * d_hx_z + (bHz - T(1)) * bld<T>(rpz, oZ, hr * SB2 + so);
                 d_hy_z = bHz * d_hy_z + (bHz - T(1)) * bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
@@ -1460,49 +1480,54 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 out[1] = bld<T>(rA, o, PB) + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
                 out[2] = bld<T>(rA, o, 2 * PB) + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
                 const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
-                if (NP > 0 && !a.uniform && kin) {
+                if (NP > 0 && !uniform) {
                     const uint32_t mw = bld<uint32_t>(rsrc(matw + (long)pi * PW), oW[1], 0);
                     const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
                     const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
+                    int mm[3], npl[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const int m = (mw >> (8 * c)) & 0xff;
-                        if (m == kDesignMat) continue;
-                        const MatLut<T>& L = slut[m];
+                        mm[c] = m == kDesignMat ? 0 : m;
+                        npl[c] = m == kDesignMat ? 0 : slut[mm[c]].np;
+                    }
 #pragma unroll
-                        for (int s = 0; s < NP; ++s) {
-                            if (s >= L.np) continue;
+                    for (int s = 0; s < NP; ++s) {
+                        if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const MatLut<T>& L = slut[mm[c]];
+                            const bool vq = s < npl[c];
+                            const bool vp = vq && ((L.lorentz_mask >> s) & 1);
                             const uint32_t so = (s * 3 + c) * PB;
-                            const bool lor = (L.lorentz_mask >> s) & 1;
-                            const T pb = lor ? bld<T>(rP, o, so) : T(0);
-                            const T qb = bld<T>(rQ, o, so) + pb - u0[c];
-                            bst(rQ, o, so, L.alpha[s] * qb);
-                            if (lor) bst(rP, o, so, pb + L.beta[s] * qb);
+                            const T pb = bld<T>(rP, vp ? o : kOOB, so);
+                            const T q0 = bld<T>(rQ, vq ? o : kOOB, so);
+                            const T qb = vq ? q0 + pb - u0[c] : T(0);
+                            bst(rQ, vq ? o : kOOB, so, L.alpha[s] * qb);
+                            bst(rP, vp ? o : kOOB, so, pb + L.beta[s] * qb);
                             out[c] += L.c3[s] * qb;
                         }
                     }
                 }
-                if (kin) {
-                    // new E auxiliary bars: E-terms of row 0 with their signs
-                    if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
-                        const T b = cy[j];
-                        const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                        bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + bld<T>(rpy, oK, er * SB1 + so)));
-                        bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + bld<T>(rpy, oK, (er + 1) * SB1 + so)));
-                    }
-                    if (zpml) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
-                        const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                        bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + bld<T>(rpz, oZ, er * SB2 + so)));
-                        bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + bld<T>(rpz, oZ, (er + 1) * SB2 + so)));
-                    }
-                    if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
-                        bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + bld<T>(rpx, o, er * SB0 + sxo)));
-                        bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + bld<T>(rpx, o, (er + 1) * SB0 + sxo)));
-                    }
-                    bst(rB, o, 0, out[0]);
-                    bst(rB, o, PB, out[1]);
-                    bst(rB, o, 2 * PB, out[2]);
+                // new E auxiliary bars: E-terms of row 0 with their signs
+                if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
+                    const T b = cy[j];
+                    const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
+                    bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + bld<T>(rpy, oK, er * SB1 + so)));
+                    bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + bld<T>(rpy, oK, (er + 1) * SB1 + so)));
                 }
+                if (zany) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
+                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + bld<T>(rpz, oZ, er * SB2 + so)));
+                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + bld<T>(rpz, oZ, (er + 1) * SB2 + so)));
+                }
+                if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
+                    bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + bld<T>(rpx, o, er * SB0 + sxo)));
+                    bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + bld<T>(rpx, o, (er + 1) * SB0 + sxo)));
+                }
+                bst(rB, o, 0, out[0]);
+                bst(rB, o, PB, out[1]);
+                bst(rB, o, 2 * PB, out[2]);
             }
         } else {
             lds_barrier();  // keep the barrier count uniform across iterations

@@ -283,6 +283,12 @@ class Batch:
             raise _lib.EngineError(f"{what} failed ({_lib.STATUS.get(rc, rc)}): "
                                    f"{self.lib.sjf_batch_last_error(self.h).decode()}")
 
+    def set_tile(self, form: str, xchunk: int = 0):
+        """Tile form of the fused launch ("register_rows" or "thread_rows") and the x-chunk of
+        every instance (0: each engine's own)."""
+        code = {v: k for k, v in TILE_FORMS.items()}[form]
+        self._chk(self.lib.sjf_batch_set_tile(self.h, code, int(xchunk)), "sjf_batch_set_tile")
+
     def step(self, n: int = 1):
         self._chk(self.lib.sjf_batch_step(self.h, int(n)), "sjf_batch_step")
 

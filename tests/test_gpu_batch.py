@@ -3,7 +3,8 @@
 * Steps replayed from a captured hipGraph give bit-identical fields, monitor samples and DFT
   sums to steps launched one by one, replay after replay with direct steps in between.
 * A batch of engines of uneven shapes, advanced by one fused launch per step, gives
-  bit-identical results to the same engines stepped on their own, and matches the FP64 oracle.
+  bit-identical results to the same engines stepped on their own, and matches the FP64 oracle
+  under every tile form and x-chunk of the fused launch.
 """
 import numpy as np
 import pytest
@@ -72,11 +73,14 @@ def test_batch_matches_individual_engines(prec):
         e.close()
 
 
-def test_batch_matches_oracle_fp64():
+@pytest.mark.parametrize("tile", [None, ("register_rows", 0), ("register_rows", 3), ("thread_rows", 2)])
+def test_batch_matches_oracle_fp64(tile):
     scns = early(scenario.batched_sweep(n=3, seed=11, scale=0.25, precision="fp64"))
-    print("[case] batch vs oracle", [s.shape for s in scns], flush=True)
+    print("[case] batch vs oracle", tile, [s.shape for s in scns], flush=True)
     eng = [s.make(Engine) for s in scns]
     b = Batch(eng)
+    if tile:
+        b.set_tile(*tile)
     b.step(80)
     for s, e in zip(scns, eng):
         o = s.make(Oracle)
