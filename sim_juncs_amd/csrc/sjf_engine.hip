@@ -233,7 +233,8 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
 // Row-of-threads form: (rows + 1) thread rows of nz threads per workgroup, 32-bit offsets.
 bool thread_rows_fit(const sjf_engine* e, int rows) { return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024; }
 
-int default_thread_rows(const sjf_engine* e) { return std::max(1, 1024 / e->g.nz - 1); }
+// FP64 tiles stop at 512 threads: at 1024 the register budget of 128 per thread spills
+int default_thread_rows(const sjf_engine* e) { return std::max(1, (e->prec == 8 ? 512 : 1024) / e->g.nz - 1); }
 
 int block_threads_for(const sjf_engine* e, int form, int ty) {
     const int t = form == SJF_TILE_THREAD_ROWS ? (ty + 1) * e->g.nz : e->g.nz;

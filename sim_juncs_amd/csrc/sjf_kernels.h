@@ -589,7 +589,6 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     const char* F1 = (const char*)a.F[(step + 1) & 1];
     const char* matw = (const char*)a.matw;
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
-    GLOBAL const T* dinv = (GLOBAL const T*)a.dinv;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -602,27 +601,31 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     const int jr = tyi * R + r - 1;  // unwrapped global row of this thread row
     const int j = act ? wrap_or_ghost(jr, ny, a.py) : -1;
     const bool valid = j >= 0;
-    const bool owned = act && r >= 1 && jr < ny;
+    const bool owned = act && r >= 1 && jr < ny;  // implies valid
     const bool top = act && r == RT - 1;
     const int jt = top ? wrap_or_ghost(jr + 1, ny, a.py) : -1;
     const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
     const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
     const T cdt = a.cdt;
     const bool uniform = a.uniform != 0;
-    const uint32_t plane = (uint32_t)ny * (uint32_t)nz;
-    const uint32_t rowc = valid ? (uint32_t)j * nz + k : 0u;
-    const uint32_t rowt = jt >= 0 ? (uint32_t)jt * nz + k : 0u;
     constexpr uint32_t W = sizeof(T);
-    const uint32_t PB = plane * W;          // bytes of one component plane
-    const long FPB = 6L * PB;               // bytes of one x plane of a field set
-    const long QPB = (long)a.nqp * PB;      // and of the polarisation
-    const long PW = (long)plane * 4;        // and of the material words
-    const uint32_t oc = rowc * W, ot = rowt * W, ow = rowc * 4u;
+    const uint32_t PB = (uint32_t)ny * (uint32_t)nz * W;  // bytes of one component plane
+    const long FPB = 6L * PB;                            // bytes of one x plane of a field set
+    const long QPB = (long)a.nqp * PB;                   // and of the polarisation
+    const long PW = (long)ny * nz * 4;                   // and of the material words
+    // lane offsets within a component plane; kOOB for lanes that neither read nor write
+    const uint32_t oc = valid ? ((uint32_t)j * nz + k) * W : kOOB;
+    const uint32_t ot = jt >= 0 ? ((uint32_t)jt * nz + k) * W : kOOB;
+    const uint32_t ow = owned ? ((uint32_t)j * nz + k) * 4u : kOOB;
+    const uint32_t oo = owned ? oc : kOOB;
     const uint32_t SB0 = (uint32_t)(a.S[0] * W), SB1 = (uint32_t)(a.S[1] * W), SB2 = (uint32_t)(a.S[2] * W);
     const Rsrc rpx = rsrc(a.psi[0]), rpy = rsrc(a.psi[1]), rpz = rsrc(a.psi[2]);
+    const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
 
     const bool ypml = valid && in_pml(a, 1, j, ny);
-    const bool zpml = act && in_pml(a, 2, k, nz);
+    const bool zpml = valid && in_pml(a, 2, k, nz);
+    // wavefront-uniform summaries of the per-lane conditions
+    const bool yany = wave_any(ypml), zany = wave_any(zpml), tany = wave_any(jt >= 0), oany = wave_any(owned);
     T bEy = T(0), aEy = T(0), bHy = T(0), aHy = T(0);
     if (ypml) {
         GLOBAL const T* cy = coef + a.coff[1];
@@ -639,46 +642,45 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         bHz = cz[2 * nz + k];
         aHz = cz[3 * nz + k];
     }
+    const int syl = ypml ? (j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1]) : 0;
+    const int szl = zpml ? (k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) : 0;
 
     const int x0 = xci * a.xc;
     const int x1 = min(x0 + a.xc, nx);
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
 
-    T ec[3], en[3], te[2], ten[2], hc[3], hp[2] = {T(0), T(0)};
+    T ec[3], en[3], te[2] = {T(0), T(0)}, ten[2] = {T(0), T(0)}, hc[3], hp[2] = {T(0), T(0)};
     uint32_t mw = 0;
     {
         const int p0 = wrap_or_ghost(istart, nx, a.px);
         const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
         const Rsrc r0 = rsrc(F0 + (long)p0 * FPB);
         const Rsrc r1 = rsrc(F0 + (long)(p1 < 0 ? 0 : p1) * FPB);
-        const bool v1 = valid && p1 >= 0;
+        const uint32_t o1 = p1 >= 0 ? oc : kOOB;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T a0 = bld<T>(r0, oc, c * PB), a1 = bld<T>(r1, oc, c * PB), a2 = bld<T>(r0, oc, (3 + c) * PB);
-            ec[c] = valid ? a0 : T(0);
-            en[c] = v1 ? a1 : T(0);
-            hc[c] = valid ? a2 : T(0);
+            ec[c] = bld<T>(r0, oc, c * PB);
+            en[c] = bld<T>(r1, o1, c * PB);
+            hc[c] = bld<T>(r0, oc, (3 + c) * PB);
         }
-        // the row above the tile: only the top thread row (whole wavefronts when nz is a
-        // multiple of 64) loads it
-        te[0] = te[1] = ten[0] = ten[1] = T(0);
-        if (jt >= 0) {
+        // the row above the tile: only lanes of the top thread row load it
+        if (tany) {
+            const uint32_t t1 = p1 >= 0 ? ot : kOOB;
             te[0] = bld<T>(r0, ot, 0);
             te[1] = bld<T>(r0, ot, 2 * PB);
-            if (p1 >= 0) {
-                ten[0] = bld<T>(r1, ot, 0);
-                ten[1] = bld<T>(r1, ot, 2 * PB);
-            }
+            ten[0] = bld<T>(r1, t1, 0);
+            ten[1] = bld<T>(r1, t1, 2 * PB);
         }
-        if (!uniform && istart >= x0 && owned) mw = bld<uint32_t>(rsrc(matw + (long)p0 * PW), ow, 0);
+        if (!uniform && istart >= x0 && oany) mw = bld<uint32_t>(rsrc(matw + (long)p0 * PW), ow, 0);
     }
 
     for (int i = istart; i < x1; ++i) {
         const bool own_plane = i >= x0;
         const bool has_next = i + 1 < x1;
         const int pi = wrap_or_ghost(i, nx, a.px);
-        const Rsrc rF = rsrc(F0 + (long)pi * FPB);
         const Rsrc rW = rsrc(F1 + (long)pi * FPB);
+        const uint32_t wo = own_plane ? oo : kOOB;  // lanes that write this plane
+        const bool wany = own_plane && oany;
 
         // ------------------------------------------------------------ loads -----------------
         T enn[3] = {T(0), T(0), T(0)}, tenn[2] = {T(0), T(0)}, hn[3] = {T(0), T(0), T(0)};
@@ -687,67 +689,59 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             const int pn = wrap_or_ghost(i + 1, nx, a.px);
             const int pnn = wrap_or_ghost(i + 2, nx, a.px);
             const Rsrc rn = rsrc(F0 + (long)pn * FPB);
-            if (pnn >= 0) {
-                const Rsrc rnn = rsrc(F0 + (long)pnn * FPB);
+            const Rsrc rnn = rsrc(F0 + (long)(pnn < 0 ? 0 : pnn) * FPB);
+            const uint32_t onn = pnn >= 0 ? oc : kOOB;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T v = bld<T>(rnn, oc, c * PB);
-                    enn[c] = valid ? v : T(0);
-                }
-                if (jt >= 0) {
-                    tenn[0] = bld<T>(rnn, ot, 0);
-                    tenn[1] = bld<T>(rnn, ot, 2 * PB);
-                }
+            for (int c = 0; c < 3; ++c) enn[c] = bld<T>(rnn, onn, c * PB);
+            if (tany) {
+                const uint32_t tn = pnn >= 0 ? ot : kOOB;
+                tenn[0] = bld<T>(rnn, tn, 0);
+                tenn[1] = bld<T>(rnn, tn, 2 * PB);
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T v = bld<T>(rn, oc, (3 + c) * PB);
-                hn[c] = valid ? v : T(0);
-            }
-            if (!uniform && owned) mwn = bld<uint32_t>(rsrc(matw + (long)pn * PW), ow, 0);
+            for (int c = 0; c < 3; ++c) hn[c] = bld<T>(rn, oc, (3 + c) * PB);
+            if (!uniform && oany) mwn = bld<uint32_t>(rsrc(matw + (long)pn * PW), ow, 0);
         }
         const bool xpml = in_pml(a, 0, pi, nx);
         T bEx = T(0), aEx = T(0), bHx = T(0), aHx = T(0);
+        uint32_t sxo = 0;
         if (xpml) {
             GLOBAL const T* cx = coef + a.coff[0];
             bEx = cx[pi];
             aEx = cx[nx + pi];
             bHx = cx[2 * nx + pi];
             aHx = cx[3 * nx + pi];
+            sxo = (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB;
         }
-        const bool wr = owned && own_plane;
-        uint32_t sx = 0, sy = 0, sz = 0;
+        // slab lane offsets of this plane (kOOB outside the layer)
+        const uint32_t oy = ypml ? (uint32_t)((pi * Ly + syl) * nz + k) * W : kOOB;
+        const uint32_t oz = zpml ? (uint32_t)((pi * ny + j) * Lz + szl) * W : kOOB;
+        const uint32_t oyw = own_plane && owned ? oy : kOOB, ozw = own_plane && owned ? oz : kOOB;
         T pH[3][2], pE[3][2];
 #pragma unroll
         for (int d = 0; d < 3; ++d) pH[d][0] = pH[d][1] = pE[d][0] = pE[d][1] = T(0);
-        if (valid && xpml) {
-            const int s = pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0];
-            sx = (((uint32_t)s * ny + j) * nz + k) * W;
-            pH[0][0] = bld<T>(rpx, sx, hr * SB0);
-            pH[0][1] = bld<T>(rpx, sx, (hr + 1) * SB0);
-            if (wr) {
-                pE[0][0] = bld<T>(rpx, sx, 0);
-                pE[0][1] = bld<T>(rpx, sx, SB0);
+        if (xpml) {
+            pH[0][0] = bld<T>(rpx, oc, hr * SB0 + sxo);
+            pH[0][1] = bld<T>(rpx, oc, (hr + 1) * SB0 + sxo);
+            if (wany) {
+                pE[0][0] = bld<T>(rpx, wo, sxo);
+                pE[0][1] = bld<T>(rpx, wo, SB0 + sxo);
             }
         }
-        if (ypml) {
-            const int s = j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1];
-            sy = (((uint32_t)pi * (a.plo[1] + a.phi[1]) + s) * nz + k) * W;
-            pH[1][0] = bld<T>(rpy, sy, hr * SB1);
-            pH[1][1] = bld<T>(rpy, sy, (hr + 1) * SB1);
-            if (wr) {
-                pE[1][0] = bld<T>(rpy, sy, 0);
-                pE[1][1] = bld<T>(rpy, sy, SB1);
+        if (yany) {
+            pH[1][0] = bld<T>(rpy, oy, hr * SB1);
+            pH[1][1] = bld<T>(rpy, oy, (hr + 1) * SB1);
+            if (wany) {
+                pE[1][0] = bld<T>(rpy, oyw, 0);
+                pE[1][1] = bld<T>(rpy, oyw, SB1);
             }
         }
-        if (valid && zpml) {
-            const int s = k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2];
-            sz = (((uint32_t)pi * ny + j) * (a.plo[2] + a.phi[2]) + s) * W;
-            pH[2][0] = bld<T>(rpz, sz, hr * SB2);
-            pH[2][1] = bld<T>(rpz, sz, (hr + 1) * SB2);
-            if (wr) {
-                pE[2][0] = bld<T>(rpz, sz, 0);
-                pE[2][1] = bld<T>(rpz, sz, SB2);
+        if (zany) {
+            pH[2][0] = bld<T>(rpz, oz, hr * SB2);
+            pH[2][1] = bld<T>(rpz, oz, (hr + 1) * SB2);
+            if (wany) {
+                pE[2][0] = bld<T>(rpz, ozw, 0);
+                pE[2][1] = bld<T>(rpz, ozw, SB2);
             }
         }
         T Qv[3][NP > 0 ? NP : 1], Pv[3][NP > 0 ? NP : 1];
@@ -757,18 +751,25 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[c][s] = Pv[c][s] = T(0);
         const Rsrc rQ = rsrc((const char*)a.Q + (long)pi * QPB);
         const Rsrc rP = rsrc((const char*)a.P + (long)pi * QPB);
-        if (NP > 0 && !uniform && wr) {
+        int mm[3] = {0, 0, 0}, npl[3] = {0, 0, 0};
+        bool des[3] = {false, false, false};
+        if (NP > 0 && !uniform && wany) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int m = (mw >> (8 * c)) & 0xff;
-                const int np = m == kDesignMat ? 0 : slut[m].np;
-                const int lm = m == kDesignMat ? 0 : slut[m].lorentz_mask;
+                des[c] = m == kDesignMat;
+                mm[c] = des[c] ? 0 : m;
+                npl[c] = des[c] ? 0 : slut[mm[c]].np;
+            }
 #pragma unroll
-                for (int s = 0; s < NP; ++s) {
-                    if (s < np) {
-                        Qv[c][s] = bld<T>(rQ, oc, (s * 3 + c) * PB);
-                        if ((lm >> s) & 1) Pv[c][s] = bld<T>(rP, oc, (s * 3 + c) * PB);
-                    }
+            for (int s = 0; s < NP; ++s) {
+                if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const bool vq = s < npl[c];
+                    const bool vp = vq && ((slut[mm[c]].lorentz_mask >> s) & 1);
+                    Qv[c][s] = bld<T>(rQ, vq ? wo : kOOB, (s * 3 + c) * PB);
+                    Pv[c][s] = bld<T>(rP, vp ? wo : kOOB, (s * 3 + c) * PB);
                 }
             }
         }
@@ -778,10 +779,11 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         sE[B + tid] = ec[1];
         sE[2 * B + tid] = ec[2];
         lds_barrier();
-        T h[3] = {T(0), T(0), T(0)};
-        if (valid) {
-            const T exj = top ? te[0] : sE[tid + nz];
-            const T ezj = top ? te[1] : sE[2 * B + tid + nz];
+        T h[3];
+        {
+            const int up = (top || !act) ? tid : tid + nz;  // the row above within the tile (top row: te)
+            const T exj = top ? te[0] : sE[up];
+            const T ezj = top ? te[1] : sE[2 * B + up];
             const T exk = kp1 >= 0 ? sE[r * nz + kp1] : T(0);
             const T eyk = kp1 >= 0 ? sE[B + r * nz + kp1] : T(0);
             T dEz_y = ezj - ec[2];
@@ -790,11 +792,12 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             T dEx_z = exk - ec[0];
             T dEz_x = en[2] - ec[2];
             T dEy_x = en[1] - ec[1];
-            if (ypml) {
+            // lanes outside a PML layer carry zero coefficients and auxiliaries: no change
+            if (yany) {
                 dEz_y = cpml(dEz_y, bHy, aHy, pH[1][0]);
                 dEx_y = cpml(dEx_y, bHy, aHy, pH[1][1]);
             }
-            if (zpml) {
+            if (zany) {
                 dEy_z = cpml(dEy_z, bHz, aHz, pH[2][0]);
                 dEx_z = cpml(dEx_z, bHz, aHz, pH[2][1]);
             }
@@ -802,24 +805,25 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                 dEz_x = cpml(dEz_x, bHx, aHx, pH[0][0]);
                 dEy_x = cpml(dEy_x, bHx, aHx, pH[0][1]);
             }
-            h[0] = hc[0] - cdt * (dEz_y - dEy_z);
-            h[1] = hc[1] - cdt * (dEx_z - dEz_x);
-            h[2] = hc[2] - cdt * (dEy_x - dEx_y);
-            if (wr) {
-                bst(rW, oc, 3 * PB, h[0]);
-                bst(rW, oc, 4 * PB, h[1]);
-                bst(rW, oc, 5 * PB, h[2]);
+            // outside a metallic edge H is identically zero (ghost rows)
+            h[0] = valid ? hc[0] - cdt * (dEz_y - dEy_z) : T(0);
+            h[1] = valid ? hc[1] - cdt * (dEx_z - dEz_x) : T(0);
+            h[2] = valid ? hc[2] - cdt * (dEy_x - dEx_y) : T(0);
+            if (wany) {
+                bst(rW, wo, 3 * PB, h[0]);
+                bst(rW, wo, 4 * PB, h[1]);
+                bst(rW, wo, 5 * PB, h[2]);
                 if (xpml) {
-                    bst(rpx, sx, hw * SB0, pH[0][0]);
-                    bst(rpx, sx, (hw + 1) * SB0, pH[0][1]);
+                    bst(rpx, wo, hw * SB0 + sxo, pH[0][0]);
+                    bst(rpx, wo, (hw + 1) * SB0 + sxo, pH[0][1]);
                 }
-                if (ypml) {
-                    bst(rpy, sy, hw * SB1, pH[1][0]);
-                    bst(rpy, sy, (hw + 1) * SB1, pH[1][1]);
+                if (yany) {
+                    bst(rpy, oyw, hw * SB1, pH[1][0]);
+                    bst(rpy, oyw, (hw + 1) * SB1, pH[1][1]);
                 }
-                if (zpml) {
-                    bst(rpz, sz, hw * SB2, pH[2][0]);
-                    bst(rpz, sz, (hw + 1) * SB2, pH[2][1]);
+                if (zany) {
+                    bst(rpz, ozw, hw * SB2, pH[2][0]);
+                    bst(rpz, ozw, (hw + 1) * SB2, pH[2][1]);
                 }
             }
         }
@@ -829,9 +833,10 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
         lds_barrier();
 
         // ------------------------------------------------------------ E^{n+1} on plane i ----
-        if (wr) {
-            const T hxj = sH[tid - nz];
-            const T hzj = sH[2 * B + tid - nz];
+        if (wany) {
+            const int dn = owned ? tid - nz : tid;  // the row below within the tile
+            const T hxj = sH[dn];
+            const T hzj = sH[2 * B + dn];
             const T hxk = km1 >= 0 ? sH[r * nz + km1] : T(0);
             const T hyk = km1 >= 0 ? sH[B + r * nz + km1] : T(0);
             T dHz_y = h[2] - hzj;
@@ -840,58 +845,69 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
             T dHx_z = h[0] - hxk;
             T dHz_x = h[2] - hp[1];
             T dHy_x = h[1] - hp[0];
-            if (ypml) {
+            if (yany) {
                 dHz_y = cpml(dHz_y, bEy, aEy, pE[1][0]);
                 dHx_y = cpml(dHx_y, bEy, aEy, pE[1][1]);
-                bst(rpy, sy, 0, pE[1][0]);
-                bst(rpy, sy, SB1, pE[1][1]);
+                bst(rpy, oyw, 0, pE[1][0]);
+                bst(rpy, oyw, SB1, pE[1][1]);
             }
-            if (zpml) {
+            if (zany) {
                 dHy_z = cpml(dHy_z, bEz, aEz, pE[2][0]);
                 dHx_z = cpml(dHx_z, bEz, aEz, pE[2][1]);
-                bst(rpz, sz, 0, pE[2][0]);
-                bst(rpz, sz, SB2, pE[2][1]);
+                bst(rpz, ozw, 0, pE[2][0]);
+                bst(rpz, ozw, SB2, pE[2][1]);
             }
             if (xpml) {
                 dHz_x = cpml(dHz_x, bEx, aEx, pE[0][0]);
                 dHy_x = cpml(dHy_x, bEx, aEx, pE[0][1]);
-                bst(rpx, sx, 0, pE[0][0]);
-                bst(rpx, sx, SB0, pE[0][1]);
+                bst(rpx, wo, sxo, pE[0][0]);
+                bst(rpx, wo, SB0 + sxo, pE[0][1]);
             }
             const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
             if (uniform) {
                 const T inv = slut[0].inv_eps;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) bst(rW, oc, c * PB, ec[c] + inv * (cdt * curl[c]));
+                for (int c = 0; c < 3; ++c) bst(rW, wo, c * PB, ec[c] + inv * (cdt * curl[c]));
             } else {
+                T inv[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const int m = (mw >> (8 * c)) & 0xff;
-                    const T e0 = ec[c];
-                    T inv, sq = T(0);
-                    if (m == kDesignMat) {
-                        const long dq = ((long)(pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
-                                            (a.dhi[2] - a.dlo[2]) +
-                                        (k - a.dlo[2]);
-                        inv = dinv[dq];
-                    } else {
-                        const MatLut<T>& L = slut[m];
-                        inv = L.inv_eps;
-#pragma unroll
-                        for (int s = 0; s < NP; ++s) {
-                            if (s < L.np) {
-                                T qn = L.alpha[s] * Qv[c][s] + L.c3[s] * e0;
-                                if ((L.lorentz_mask >> s) & 1) {
-                                    qn += L.beta[s] * Pv[c][s];
-                                    bst(rP, oc, (s * 3 + c) * PB, Pv[c][s] + qn);
-                                }
-                                bst(rQ, oc, (s * 3 + c) * PB, qn);
-                                sq += qn;
-                            }
-                        }
+                    if (NP == 0) {  // material ids were not decoded for the pole loads
+                        const int m = (mw >> (8 * c)) & 0xff;
+                        des[c] = m == kDesignMat;
+                        mm[c] = des[c] ? 0 : m;
                     }
-                    bst(rW, oc, c * PB, e0 + inv * (cdt * curl[c] - sq));
+                    inv[c] = slut[mm[c]].inv_eps;
                 }
+                if (wave_any(des[0] || des[1] || des[2])) {
+                    const Rsrc rd = rsrc(a.dinv);
+                    const uint32_t dq = (uint32_t)(((pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
+                                                       (a.dhi[2] - a.dlo[2]) + (k - a.dlo[2])) * W;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
+                        inv[c] = des[c] ? v : inv[c];
+                    }
+                }
+                T sq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+                for (int s = 0; s < NP; ++s) {
+                    if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const MatLut<T>& L = slut[mm[c]];
+                        const bool vq = s < npl[c];
+                        const bool vp = vq && ((L.lorentz_mask >> s) & 1);
+                        // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
+                        T qn = L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c] + L.beta[s] * Pv[c][s];
+                        qn = vq ? qn : T(0);
+                        bst(rP, vp ? wo : kOOB, (s * 3 + c) * PB, Pv[c][s] + qn);
+                        bst(rQ, vq ? wo : kOOB, (s * 3 + c) * PB, qn);
+                        sq[c] += qn;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) bst(rW, wo, c * PB, ec[c] + inv[c] * (cdt * curl[c] - sq[c]));
             }
         }
 
