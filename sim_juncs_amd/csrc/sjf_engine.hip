@@ -329,6 +329,7 @@ struct LaunchSet {
     void* d_dft = nullptr;
     int ndft = 0;
     long dft_total = 0;
+    int dft_maxnf = 0;  // most frequencies of any region (the launch's second dimension)
     // step counters
     long** d_counters = nullptr;
     // graph
@@ -505,6 +506,8 @@ int build_set_t(LaunchSet* s) {
     }
     s->ndft = (int)regs.size();
     s->dft_total = tot;
+    s->dft_maxnf = 0;
+    for (const auto& r : regs) s->dft_maxnf = std::max(s->dft_maxnf, r.nf);
     if (s->ndft) {
         rc = upload_vec(e0, s->stream, &s->d_dft, regs);
         if (rc) return rc;
@@ -595,7 +598,8 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
         SETCHK(s, hipGetLastError());
     }
     if (s->ndft && !bare) {
-        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256), (unsigned)s->dft_maxnf),
+                           dim3(256), 0,
                            s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total);
         SETCHK(s, hipGetLastError());
     }

@@ -1113,6 +1113,7 @@ struct DftDev {
     double2* cmp;        // compensation terms, same layout
 };
 
+// One thread per (cell, frequency): blockIdx.y is the frequency index within the region.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1124,6 +1125,8 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
         else hi = mid - 1;
     }
     const DftDev<T>& a = regs[lo];
+    const int f = blockIdx.y;
+    if (f >= a.nf) return;
     const long cell = g - a.first;
     const long step = *a.d_step;
     const T* F = a.F[(step + 1) & 1];
@@ -1134,21 +1137,19 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
     const long q = (long)(a.lo[0] + ii) * a.nyz + (long)(a.lo[1] + jj) * a.nz + (a.lo[2] + kk);
     // E after the step is at t = (n+1) dt, H at (n + 1/2) dt
     const double tE = (step + 1) * a.dt, tH = (step + 0.5) * a.dt;
-    for (int f = 0; f < a.nf; ++f) {
-        const double w = 2 * M_PI * a.freq[f];
-        double sE, cE, sH, cH;
-        sincos(w * tE, &sE, &cE);
-        sincos(w * tH, &sH, &cH);
-        for (int s = 0; s < a.ncomp; ++s) {
-            const int c = a.comps[s];
-            const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
-            const long o = ((long)s * a.nf + f) * a.ncell + cell;
-            double2 sm = a.acc[o], cp = a.cmp[o];
-            neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
-            neumaier(sm.y, cp.y, v * (c < 3 ? sE : sH));
-            a.acc[o] = sm;
-            a.cmp[o] = cp;
-        }
+    const double w = 2 * M_PI * a.freq[f];
+    double sE, cE, sH, cH;
+    sincos(w * tE, &sE, &cE);
+    sincos(w * tH, &sH, &cH);
+    for (int s = 0; s < a.ncomp; ++s) {
+        const int c = a.comps[s];
+        const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
+        const long o = ((long)s * a.nf + f) * a.ncell + cell;
+        double2 sm = a.acc[o], cp = a.cmp[o];
+        neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
+        neumaier(sm.y, cp.y, v * (c < 3 ? sE : sH));
+        a.acc[o] = sm;
+        a.cmp[o] = cp;
     }
 }
 
