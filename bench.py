@@ -68,11 +68,18 @@ def cpu_baseline(scns, steps):
                       f"same workload, {steps} steps each after 1 warm-up step, {secs:.1f} s"}
 
 
-def roofline(bytes_step, ms_kernel):
+def roofline(bytes_step, ms_kernel, workload=None):
+    """Roofline of the fused step kernel: algorithmic bytes per launch over its mean duration.
+    `traffic` is the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (stored by `python -m sim_juncs_amd.profsum traffic ... --store`), reported only when it was
+    measured on the kernel sources being run."""
+    from sim_juncs_amd import profsum
+
     achieved = bytes_step / (ms_kernel * 1e-3) / 1e9
+    rec = profsum.stored_traffic(workload) if workload else None
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step",
-            "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": rec["traffic_bytes"] if rec else None,
+            "kernel": "k_step", "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step}
 
 
 def single(a, scn, tag):
@@ -95,7 +102,8 @@ def single(a, scn, tag):
     wall = time.perf_counter() - t
     log(f"timed {steps} steps in {wall:.3f}s")
     _, ms_kernel = eng.timed_steps(a.kernel_steps)
-    rl = roofline(eng.step_bytes(), ms_kernel)
+    # stored traffic was measured with the tuned tile: not reported for a --tile override
+    rl = roofline(eng.step_bytes(), ms_kernel, None if a.tile else scn.name)
     info = eng.launch_info()
     eng.close()
     cpu = None
@@ -132,7 +140,7 @@ def sweep(a):
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
                         "launch": "one fused launch per step over all instances, hipGraph replay"},
-                roofline=roofline(bytes_step, ms_kernel), cpu=cpu, dtype="f32")
+                roofline=roofline(bytes_step, ms_kernel, f"sweep_{len(scns)}_instances"), cpu=cpu, dtype="f32")
 
 
 def adjoint(a, variants=False):
@@ -156,7 +164,7 @@ def adjoint(a, variants=False):
     eng = s0.make(Engine)
     eng.step(a.warmup)
     _, ms_kernel = eng.timed_steps(a.kernel_steps)
-    rl = roofline(eng.step_bytes(), ms_kernel)
+    rl = roofline(eng.step_bytes(), ms_kernel, None if a.tile else s0.name)
     eng.close()
     cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or 60)
     if cpu:

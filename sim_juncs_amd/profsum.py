@@ -2,15 +2,76 @@
 
     python -m sim_juncs_amd.profsum trace  <results.db>            # per-kernel duration stats
     python -m sim_juncs_amd.profsum pmc    <results.db> [kernel]   # per-kernel counter means
+    python -m sim_juncs_amd.profsum traffic <workload> <kernel> <fetch.db> <write.db> \
+        <copy_fetch.db> <copy_write.db> [--store]                  # HBM bytes per launch
 
 Counter values such as FETCH_SIZE / WRITE_SIZE are reported in KiB per dispatch, as
 rocprofv3 records them.
 """
 from __future__ import annotations
 
+import hashlib
+import json
+import os
 import sqlite3
 import sys
 from collections import defaultdict
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC = os.path.join(HERE, "measured_traffic.json")
+COPY_BYTES = 1 << 30  # the calibration copy kernel reads and writes 1 GiB per launch
+
+
+def kernel_source_hash() -> str:
+    """Hash of the engine sources: a stored traffic figure is valid only for the kernels it was
+    measured on."""
+    h = hashlib.sha256()
+    for f in ("sjf_kernels.h", "sjf_engine.hip"):
+        with open(os.path.join(HERE, "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def counter_mean(db: str, kernel: str, counter: str) -> float:
+    con = sqlite3.connect(db)
+    v = [cv for name, cn, cv in con.execute("select name, counter_name, counter_value from pmc_events")
+         if kernel in name and cn == counter]
+    if not v:
+        raise SystemExit(f"no {counter} samples of {kernel} in {db}")
+    return sum(v) / len(v)
+
+
+def traffic(workload, kernel, fetch_db, write_db, copy_fetch_db, copy_write_db, store=False):
+    """HBM bytes per launch of `kernel`: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by the ratio of known bytes to the same counter on the calibration copy kernel."""
+    ff = COPY_BYTES / (1024.0 * counter_mean(copy_fetch_db, "k_copy", "FETCH_SIZE"))
+    fw = COPY_BYTES / (1024.0 * counter_mean(copy_write_db, "k_copy", "WRITE_SIZE"))
+    fk = counter_mean(fetch_db, kernel, "FETCH_SIZE")
+    wk = counter_mean(write_db, kernel, "WRITE_SIZE")
+    rec = dict(kernel=kernel, fetch_kib=fk, write_kib=wk, fetch_factor=ff, write_factor=fw,
+               fetched_bytes=fk * 1024.0 * ff, written_bytes=wk * 1024.0 * fw,
+               traffic_bytes=(fk * ff + wk * fw) * 1024.0, kernel_source=kernel_source_hash())
+    print(json.dumps(dict(workload=workload, **rec), indent=1))
+    if store:
+        c = {}
+        if os.path.exists(TRAFFIC):
+            with open(TRAFFIC) as f:
+                c = json.load(f)
+        c[workload] = rec
+        with open(TRAFFIC, "w") as f:
+            json.dump(c, f, indent=1, sort_keys=True)
+
+
+def stored_traffic(workload: str):
+    """Measured HBM bytes per launch for `workload`, or None when absent or measured on other
+    kernel sources."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        rec = json.load(f).get(workload)
+    if not rec or rec.get("kernel_source") != kernel_source_hash():
+        return None
+    return rec
 
 
 def short(name: str) -> str:
@@ -47,8 +108,11 @@ def pmc(db: str, kernel: str | None = None):
 
 
 if __name__ == "__main__":
-    mode, db = sys.argv[1], sys.argv[2]
+    mode = sys.argv[1]
     if mode == "trace":
-        trace(db)
+        trace(sys.argv[2])
+    elif mode == "traffic":
+        args = [a for a in sys.argv[2:] if a != "--store"]
+        traffic(*args, store="--store" in sys.argv)
     else:
-        pmc(db, sys.argv[3] if len(sys.argv) > 3 else None)
+        pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
