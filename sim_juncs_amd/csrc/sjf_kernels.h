@@ -1271,9 +1271,12 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
     // u = inv_eps * Ebar' on planes i and i + 1 (rows 0..2), raw Ebar' and material words of
-    // plane i + 2 in flight
+    // plane i + 2 in flight; the owned row's raw Ebar' and material word of planes i and i + 1
+    // are kept for phase 2
     T uc[3][3], un[3][3], rn[3][3];
     uint32_t wn[3];
+    T ec1[3], en1[3];
+    uint32_t wc1 = 0u, wn1 = 0u;
     auto load_raw = [&](int p, T (&raw)[3][3], uint32_t (&w)[3]) {
         const int pp = wrap_or_ghost(p, nx, a.px);
         const long pc = pp >= 0 ? pp : 0;
@@ -1327,8 +1330,14 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         uint32_t w[3];
         load_raw(istart, raw, w);
         to_u(istart, raw, w, uc);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ec1[c] = raw[1][c];
+        wc1 = w[1];
         load_raw(istart + 1, raw, w);
         to_u(istart + 1, raw, w, un);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) en1[c] = raw[1][c];
+        wn1 = w[1];
     }
     T dbx_prev[2] = {T(0), T(0)};  // x H-terms of row 0 on plane i - 1: (Hz -> Ey), (Hy -> Ez)
 
@@ -1351,6 +1360,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         // E-terms on rows r = 0, 1 (labels: y (Ex->Hz), z (Ex->Hy), z (Ey->Hx), x (Ey->Hz),
         // x (Ez->Hy), y (Ez->Hx)) and the y-terms of row 2
         T eb[3][6];
+        T pey[2] = {T(0), T(0)}, pez[2] = {T(0), T(0)}, pex[2] = {T(0), T(0)};  // owned row's E aux bars
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const bool ok = rv[r];
@@ -1361,18 +1371,33 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             if (ypml[r]) {
                 const T b = cy[j];
                 const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
-                t_ex_y = b * t_ex_y + (b - T(1)) * bld<T>(rpy, oK, er * SB1 + so);
-                t_ez_y = b * t_ez_y + (b - T(1)) * bld<T>(rpy, oK, (er + 1) * SB1 + so);
+                const T p0 = bld<T>(rpy, oK, er * SB1 + so), p1 = bld<T>(rpy, oK, (er + 1) * SB1 + so);
+                t_ex_y = b * t_ex_y + (b - T(1)) * p0;
+                t_ez_y = b * t_ez_y + (b - T(1)) * p1;
+                if (r == 1) {
+                    pey[0] = p0;
+                    pey[1] = p1;
+                }
             }
             if (r < 2) {
                 if (ok && zany) {  // lanes outside the layers: b = 1 and a zero auxiliary
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * bld<T>(rpz, oZ, er * SB2 + so);
-                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * bld<T>(rpz, oZ, (er + 1) * SB2 + so);
+                    const T p0 = bld<T>(rpz, oZ, er * SB2 + so), p1 = bld<T>(rpz, oZ, (er + 1) * SB2 + so);
+                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * p0;
+                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * p1;
+                    if (r == 1) {
+                        pez[0] = p0;
+                        pez[1] = p1;
+                    }
                 }
                 if (ok && xpml) {
-                    t_ey_x = bEx * t_ey_x + (bEx - T(1)) * bld<T>(rpx, oR[r], er * SB0 + sxo);
-                    t_ez_x = bEx * t_ez_x + (bEx - T(1)) * bld<T>(rpx, oR[r], (er + 1) * SB0 + sxo);
+                    const T p0 = bld<T>(rpx, oR[r], er * SB0 + sxo), p1 = bld<T>(rpx, oR[r], (er + 1) * SB0 + sxo);
+                    t_ey_x = bEx * t_ey_x + (bEx - T(1)) * p0;
+                    t_ez_x = bEx * t_ez_x + (bEx - T(1)) * p1;
+                    if (r == 1) {
+                        pex[0] = p0;
+                        pex[1] = p1;
+                    }
                 }
             }
             eb[r][0] = t_ex_y;
@@ -1416,6 +1441,22 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             tot[r][1] = ok ? hy + (eb[r][1] - exz_k) + (eb[r][4] - ebn[r][1]) : T(0);
             tot[r][2] = ok ? hz + (eb[r][0] - eb[r + 1][0]) + (eb[r][3] - ebn[r][0]) : T(0);
         }
+        // H auxiliary bars of the owned row (read by the update below and by phase 2)
+        T phy[2] = {T(0), T(0)}, phz[2] = {T(0), T(0)}, phx[2] = {T(0), T(0)};
+        if (ypml[1]) {
+            const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
+            phy[0] = bld<T>(rpy, oK, hr * SB1 + so);
+            phy[1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
+        }
+        if (rv[1] && zany) {
+            const uint32_t so = (uint32_t)((pi * ny + gj[1]) * Lz) * TS;
+            phz[0] = bld<T>(rpz, oZ, hr * SB2 + so);
+            phz[1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
+        }
+        if (rv[1] && xpml) {
+            phx[0] = bld<T>(rpx, oR[1], hr * SB0 + sxo);
+            phx[1] = bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo);
+        }
         // own row: Hbar at the old time and the new H auxiliary bars
         const bool wr = own_plane && rv[1];
         if (wr) {
@@ -1426,17 +1467,17 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             if (ypml[1]) {  // (Hx, y, +, slot 0), (Hz, y, -, slot 1)
                 const T b = cy[2 * ny + j];
                 const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                bst(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + bld<T>(rpy, oK, hr * SB1 + so)));
-                bst(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + bld<T>(rpy, oK, (hr + 1) * SB1 + so)));
+                bst(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + phy[0]));
+                bst(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + phy[1]));
             }
             if (zany) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
                 const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + bld<T>(rpz, oZ, hr * SB2 + so)));
-                bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + bld<T>(rpz, oZ, (hr + 1) * SB2 + so)));
+                bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + phz[0]));
+                bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + phz[1]));
             }
             if (xpml) {  // (Hy, x, -, slot 0), (Hz, x, +, slot 1)
-                bst(rpx, oR[1], hw * SB0 + sxo, bHx * (cdt * tot[1][1] + bld<T>(rpx, oR[1], hr * SB0 + sxo)));
-                bst(rpx, oR[1], (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[1][2] + bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo)));
+                bst(rpx, oR[1], hw * SB0 + sxo, bHx * (cdt * tot[1][1] + phx[0]));
+                bst(rpx, oR[1], (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[1][2] + phx[1]));
             }
         }
 
@@ -1449,18 +1490,16 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
               d_hz_x = -cdt * tot[1][2], d_hz_y = cdt * tot[1][2];
             if (ypml[1]) {
                 const T b = cy[2 * ny + j];
-                const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                d_hx_y = b * d_hx_y + (b - T(1)) * bld<T>(rpy, oK, hr * SB1 + so);
-                d_hz_y = b * d_hz_y + (b - T(1)) * bld<T>(rpy, oK, (hr + 1) * SB1 + so);
+                d_hx_y = b * d_hx_y + (b - T(1)) * phy[0];
+                d_hz_y = b * d_hz_y + (b - T(1)) * phy[1];
             }
             if (rv[1] && zany) {
-                const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                d_hx_z = bHz * d_hx_z + (bHz - T(1)) * bld<T>(rpz, oZ, hr * SB2 + so);
-                d_hy_z = bHz * d_hy_z + (bHz - T(1)) * bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
+                d_hx_z = bHz * d_hx_z + (bHz - T(1)) * phz[0];
+                d_hy_z = bHz * d_hy_z + (bHz - T(1)) * phz[1];
             }
             if (rv[1] && xpml) {
-                d_hy_x = bHx * d_hy_x + (bHx - T(1)) * bld<T>(rpx, oR[1], hr * SB0 + sxo);
-                d_hz_x = bHx * d_hz_x + (bHx - T(1)) * bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo);
+                d_hy_x = bHx * d_hy_x + (bHx - T(1)) * phx[0];
+                d_hz_x = bHx * d_hz_x + (bHx - T(1)) * phx[1];
             }
             db[0] = d_hx_y;
             db[1] = d_hx_z;
@@ -1493,12 +1532,12 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 const T hxz_k = km1 >= 0 ? bufd[1 * B + km1] : T(0);
                 // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
                 T out[3];
-                out[0] = bld<T>(rA, o, 0) + (hyz_k - db[2]) + (dbh[1] - db[5]);
-                out[1] = bld<T>(rA, o, PB) + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
-                out[2] = bld<T>(rA, o, 2 * PB) + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
+                out[0] = ec1[0] + (hyz_k - db[2]) + (dbh[1] - db[5]);
+                out[1] = ec1[1] + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
+                out[2] = ec1[2] + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
                 const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
                 if (NP > 0 && !uniform) {
-                    const uint32_t mw = bld<uint32_t>(rsrc(matw + (long)pi * PW), oW[1], 0);
+                    const uint32_t mw = wc1;
                     const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
                     const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
                     int mm[3], npl[3];
@@ -1530,17 +1569,17 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
                     const T b = cy[j];
                     const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                    bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + bld<T>(rpy, oK, er * SB1 + so)));
-                    bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + bld<T>(rpy, oK, (er + 1) * SB1 + so)));
+                    bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + pey[0]));
+                    bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + pey[1]));
                 }
                 if (zany) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + bld<T>(rpz, oZ, er * SB2 + so)));
-                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + bld<T>(rpz, oZ, (er + 1) * SB2 + so)));
+                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + pez[0]));
+                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + pez[1]));
                 }
                 if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
-                    bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + bld<T>(rpx, o, er * SB0 + sxo)));
-                    bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + bld<T>(rpx, o, (er + 1) * SB0 + sxo)));
+                    bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + pex[0]));
+                    bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + pex[1]));
                 }
                 bst(rB, o, 0, out[0]);
                 bst(rB, o, PB, out[1]);
@@ -1555,13 +1594,26 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) uc[r][c] = un[r][c];
-        if (has_next) to_u(i + 2, rn, wn, un);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ec1[c] = en1[c];
+        wc1 = wn1;
+        if (has_next) {
+            to_u(i + 2, rn, wn, un);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) en1[c] = rn[1][c];
+            wn1 = wn[1];
+        }
     }
 }
 
 // Fused reverse step over one engine: workgroup w handles tile w after the XCD-aware remap.
+// The FP32 reverse step fits 80 registers (6 wavefronts per SIMD) without spilling.
+template <typename T>
+constexpr int adj_waves() { return sizeof(T) == 4 ? 6 : 1; }
+
 template <typename T, int NP, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_adj_step(const AdjArgs<T> g) {
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(adj_waves<T>())))
+k_adj_step(const AdjArgs<T> g) {
     const int w = blockIdx.x, nw = gridDim.x;
     const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
     adj_tile<T, NP>(g, t);
