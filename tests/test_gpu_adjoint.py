@@ -136,3 +136,34 @@ def test_torch_autograd_binding():
     J_ref, g_ref = run_adjoint(scn, eps, scn.steps, 20, gap)
     assert abs(J.item() - J_ref) <= 1e-12 * abs(J_ref)
     assert np.allclose(t.grad.numpy(), g_ref, rtol=1e-12, atol=0)
+
+
+def test_adjoint_beyond_4gib_matches_small_grid():
+    """The reverse step's per-plane descriptors past 4 GiB: an adjoint run on a 720x512x512 FP32
+    grid (adjoint field sets 4.5 GB each, pole bars 9 planes per x plane) with the design box,
+    Au box, source and objective cell near the far corner gives the same J and gradient, bit for
+    bit, as the same arrangement at the centre of a 60^3 grid (20 steps: nothing reaches a
+    boundary).  The cell size is a power of two so that positions convert exactly."""
+    dx, steps = 1.0 / 64, 20
+    lut = [M.VACUUM, M.AU.scaled(1.0)]
+    pulse = scenario.gaussian_pulse(0.8, 0.2, 0.3, start_fs=0.0)
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(1, 4, size=(4, 4, 3))
+    res = []
+    for shape, c in (((60, 60, 60), (30, 30, 30)), ((720, 512, 512), (690, 470, 470))):
+        print(f"[case] adjoint translation {shape}", flush=True)
+        eng = Engine(shape, dx, precision="fp32")
+        words = np.zeros(shape, np.uint32)
+        words[c[0] - 5:c[0] + 6, c[1] - 5:c[1] - 1, c[2] - 5:c[2] + 6] = 0x00010101
+        eng.set_materials(words, lut)
+        del words
+        lo = (c[0] - 2, c[1] + 1, c[2] - 1)
+        eng.set_design_eps(lo, tuple(l + n for l, n in zip(lo, eps.shape)), eps)
+        at = [c[0] * dx, c[1] * dx, (c[2] + 0.5) * dx]
+        eng.add_source("Ez", at + at, **pulse)
+        J, g = eng.adjoint_run(steps, 7, (c[0], c[1] + 2, c[2]), eps.size)
+        eng.close()
+        res.append((J, g))
+    assert res[0][0] > 0 and np.abs(res[0][1]).max() > 0
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])

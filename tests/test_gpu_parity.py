@@ -187,3 +187,45 @@ def test_step_one_at_a_time_equals_batch():
     assert np.array_equal(ma, mb)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("tile", [None, ("thread_rows", 1, 16)])
+def test_field_set_beyond_4gib_matches_small_grid(tile):
+    """Per-plane descriptors past 4 GiB: a 720x512x512 FP32 grid (4.5 GB per field set; the Au
+    box's polarisation takes 9 planes per x plane, 6.8 GB) runs the same per-cell arithmetic as a
+    60^3 grid.  The same point source inside the same Au box, near the far corner of the large
+    grid (x plane 690: field and polarisation offsets above 4 GiB) and at the centre of the small
+    one, gives bit-identical monitor series and fields around the source for 20 steps.  The
+    disturbance travels at most one cell per step, so neither grid's boundary is reached.  The
+    cell size is a power of two so that source and monitor positions convert to exactly the same
+    fractional cell coordinates in both grids."""
+    dx, steps = 1.0 / 64, 20
+    lut = [M.VACUUM, M.AU.scaled(1.0)]
+    pulse = scenario.gaussian_pulse(0.8, 0.2, 0.3, start_fs=0.0)  # peaks near step 15
+    off = np.array([[a, b, c] for a in (-7, 0, 5) for b in (-6, 2) for c in (-4, 0, 3)], dtype=float)
+    out = []
+    for shape, c in (((60, 60, 60), (30, 30, 30)), ((720, 512, 512), (690, 470, 470))):
+        print(f"[case] translation {shape} source {c} tile {tile}", flush=True)
+        eng = Engine(shape, dx, precision="fp32")
+        if tile:
+            eng.set_tile(*tile)
+        words = np.zeros(shape, np.uint32)
+        words[c[0] - 5:c[0] + 6, c[1] - 5:c[1] + 6, c[2] - 5:c[2] + 6] = 0x00010101  # Au on all three E points
+        eng.set_materials(words, lut)
+        del words
+        at = [c[0] * dx, c[1] * dx, (c[2] + 0.5) * dx]  # an Ez lattice point
+        eng.add_source("Ez", at + at, **pulse)
+        for comp, o in (("Ex", (0.5, 0, 0)), ("Ez", (0, 0, 0.5)), ("Hy", (0.5, 0, 0.5))):
+            eng.add_monitors(comp, (np.array(c) + off + np.array(o)) * dx)
+        eng.set_save_span(1)
+        eng.step(steps)
+        mon = eng.read_monitors()
+        box = tuple(slice(v - 22, v + 23) for v in c)
+        fields = [eng.read_field(comp)[box] for comp in ("Ex", "Ez", "Hy")]
+        eng.close()
+        out.append((mon, fields))
+    (m0, f0), (m1, f1) = out
+    assert np.abs(f0[1]).max() > 0 and np.abs(m0).max() > 0
+    assert np.array_equal(m0, m1)
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
