@@ -945,9 +945,11 @@ __device__ __forceinline__ int tile_instance(const int* __restrict__ tile_first,
 
 // Fused step over all instances of a launch set (rows-per-thread form).
 // Waves per SIMD the register allocator must leave room for: the FP32 one-row tile with up to
-// three pole slots fits 96 registers (5 waves) without spilling; other variants are unconstrained.
+// three pole slots fits 96 registers (5 waves) without spilling; the FP64 one-row tile is held to
+// 168 registers (3 waves; with three pole slots it spills 28 bytes per lane and still runs 3%
+// faster than at 2 waves on the 96^3 box); other variants are unconstrained.
 template <typename T, int TY, int NP>
-constexpr int step_waves() { return (sizeof(T) == 4 && TY == 1 && NP <= 3) ? 5 : 1; }
+constexpr int step_waves() { return (sizeof(T) == 4 && TY == 1 && NP <= 3) ? 5 : (sizeof(T) == 8 && TY == 1) ? 3 : 1; }
 
 template <typename T, int TY, int NP, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_waves<T, TY, NP>())))
