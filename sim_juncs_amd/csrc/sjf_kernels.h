@@ -1115,19 +1115,41 @@ struct DftDev {
     double2* cmp;        // compensation terms, same layout
 };
 
-// One thread per (cell, frequency): blockIdx.y is the frequency index within the region.
+// One thread per (cell, frequency): blockIdx.y is the frequency index.  The phase factors depend
+// only on the step and the frequency, so one thread per block evaluates them (FP64 sincos) and
+// shares them through LDS; a block's cells all lie in one region when regions start on block
+// boundaries, otherwise each thread evaluates its own.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total) {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    int lo = 0, hi = nreg - 1;
-    while (lo < hi) {  // region with first <= g
-        const int mid = (lo + hi + 1) / 2;
-        if (regs[mid].first <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    const DftDev<T>& a = regs[lo];
+    __shared__ double ph[4];  // cos, sin at the E and H time levels for this block's region
+    __shared__ int blk_reg;
+    const long g0 = (long)blockIdx.x * blockDim.x;
+    const long g = g0 + threadIdx.x;
+    auto region_of = [&](long x) {
+        int lo = 0, hi = nreg - 1;
+        while (lo < hi) {  // region with first <= x
+            const int mid = (lo + hi + 1) / 2;
+            if (regs[mid].first <= x) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
     const int f = blockIdx.y;
+    if (threadIdx.x == 0) {
+        const int r0 = region_of(g0);
+        blk_reg = r0;
+        const DftDev<T>& a = regs[r0];
+        if (f < a.nf) {
+            const long step = *a.d_step;
+            const double w = 2 * M_PI * a.freq[f];
+            sincos(w * ((step + 1) * a.dt), &ph[1], &ph[0]);  // E after the step: t = (n+1) dt
+            sincos(w * ((step + 0.5) * a.dt), &ph[3], &ph[2]);  // H: t = (n + 1/2) dt
+        }
+    }
+    __syncthreads();
+    if (g >= total) return;
+    const int ri = region_of(g);
+    const DftDev<T>& a = regs[ri];
     if (f >= a.nf) return;
     const long cell = g - a.first;
     const long step = *a.d_step;
@@ -1137,12 +1159,17 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
     const int jj = (int)(rest % a.ext[1]);
     const int ii = (int)(rest / a.ext[1]);
     const long q = (long)(a.lo[0] + ii) * a.nyz + (long)(a.lo[1] + jj) * a.nz + (a.lo[2] + kk);
-    // E after the step is at t = (n+1) dt, H at (n + 1/2) dt
-    const double tE = (step + 1) * a.dt, tH = (step + 0.5) * a.dt;
-    const double w = 2 * M_PI * a.freq[f];
-    double sE, cE, sH, cH;
-    sincos(w * tE, &sE, &cE);
-    sincos(w * tH, &sH, &cH);
+    double cE, sE, cH, sH;
+    if (ri == blk_reg) {
+        cE = ph[0];
+        sE = ph[1];
+        cH = ph[2];
+        sH = ph[3];
+    } else {
+        const double w = 2 * M_PI * a.freq[f];
+        sincos(w * ((step + 1) * a.dt), &sE, &cE);
+        sincos(w * ((step + 0.5) * a.dt), &sH, &cH);
+    }
     for (int s = 0; s < a.ncomp; ++s) {
         const int c = a.comps[s];
         const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
