@@ -78,11 +78,6 @@ struct StepArgs {
 
 #define GLOBAL __attribute__((address_space(1)))
 
-template <typename T>
-__device__ __forceinline__ T ldg(GLOBAL const T* p, long q, bool ok) {
-    return ok ? p[q] : T(0);
-}
-
 __device__ __forceinline__ int wrap_or_ghost(int c, int n, int periodic) {
     if (c >= 0 && c < n) return c;
     if (!periodic) return -1;
@@ -92,22 +87,6 @@ __device__ __forceinline__ int wrap_or_ghost(int c, int n, int periodic) {
 template <typename T>
 __device__ __forceinline__ bool in_pml(const StepArgs<T>& a, int d, int c, int n) {
     return a.pml_on[d] && (c < a.plo[d] || c >= n - a.phi[d]);
-}
-
-template <typename T>
-__device__ __forceinline__ long slab_x(const StepArgs<T>& a, int i, int j, int k) {
-    int s = i < a.plo[0] ? i : i - (a.nx - a.phi[0]) + a.plo[0];
-    return ((long)s * a.ny + j) * a.nz + k;
-}
-template <typename T>
-__device__ __forceinline__ long slab_y(const StepArgs<T>& a, int i, int j, int k) {
-    int s = j < a.plo[1] ? j : j - (a.ny - a.phi[1]) + a.plo[1];
-    return ((long)i * (a.plo[1] + a.phi[1]) + s) * a.nz + k;
-}
-template <typename T>
-__device__ __forceinline__ long slab_z(const StepArgs<T>& a, int i, int j, int k) {
-    int s = k < a.plo[2] ? k : k - (a.nz - a.phi[2]) + a.plo[2];
-    return ((long)i * a.ny + j) * (a.plo[2] + a.phi[2]) + s;
 }
 
 // CPML update of one auxiliary: psi <- b psi + a d; returns d + psi.
@@ -1222,19 +1201,6 @@ struct AdjArgs {
     int er, ew;        // E aux bar pair read (0 or 6) / written (6 or 0)
     int hr, hw;        // H aux bar pair read (2 or 4) / written (4 or 2)
 };
-
-template <typename T>
-__device__ __forceinline__ T inv_eps_cell(const StepArgs<T>& a, const MatLut<T>* slut, GLOBAL const T* dinv,
-                                          uint32_t mw, int c, int i, int j, int k) {
-    if (a.uniform) return slut[0].inv_eps;
-    const int m = (mw >> (8 * c)) & 0xff;
-    if (m == kDesignMat) {
-        const long dq = ((long)(i - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) * (a.dhi[2] - a.dlo[2]) +
-                        (k - a.dlo[2]);
-        return dinv[dq];
-    }
-    return slut[m].inv_eps;
-}
 
 // One tile of the fused reverse step: a z line of one y row (j0), marching along x.  Addressing
 // and predication as in step_tile: per-plane descriptors, lane offsets fixed for the march, the
