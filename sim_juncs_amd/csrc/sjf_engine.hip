@@ -1772,9 +1772,10 @@ int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out) {
     s->prec = e0->prec;
     s->ty = 1;
     // measured on MI355X for a sweep of 64 instances (64^3 to 192x192x128): the row-of-threads
-    // form with 8-plane x-chunks beats one row per thread and the per-instance chunks
+    // form with 6-plane x-chunks (3.13 ms per step) beats 3 to 12 planes (3.15-3.27 ms), one row
+    // per thread (3.46 ms) and the per-instance chunks (3.37 ms)
     s->form = SJF_TILE_THREAD_ROWS;
-    s->xc = 8;
+    s->xc = 6;
     for (int i = 0; i < n; ++i)
         if (!thread_rows_fit(engines[i], default_thread_rows(engines[i]))) {
             s->form = SJF_TILE_REGISTER_ROWS;
