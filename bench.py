@@ -82,17 +82,23 @@ def roofline(bytes_step, ms_kernel, workload=None):
             "kernel": "k_step", "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step}
 
 
-def single(a, scn, tag):
+def set_tile(a, eng):
+    """--tile form,rows,xchunk, else the tuned tile of the shape when cached."""
     from sim_juncs_amd import autotune
-    from sim_juncs_amd.engine import Engine
 
-    steps = a.steps if a.steps is not None else scn.steps
-    eng = scn.make(Engine)
     if a.tile:
         form, rows, xc = a.tile.split(",")
         eng.set_tile(form, int(rows), int(xc))
     else:
         autotune.apply_cached(eng)
+
+
+def single(a, scn, tag):
+    from sim_juncs_amd.engine import Engine
+
+    steps = a.steps if a.steps is not None else scn.steps
+    eng = scn.make(Engine)
+    set_tile(a, eng)
     log(f"engine ready {eng.launch_info()}")
     eng.step(a.warmup)
     eng.synchronize()
@@ -153,6 +159,7 @@ def adjoint(a, variants=False):
     wall = 0.0
     for s in scns:
         eng = s.make(Engine)
+        set_tile(a, eng)  # the forward and recomputed steps run the tuned tile
         nd = s.design[2].size
         t = time.perf_counter()
         J, g = eng.adjoint_run(steps, 100, s.meta["gap"], nd)
@@ -162,6 +169,7 @@ def adjoint(a, variants=False):
         eng.close()
     s0 = scns[0]
     eng = s0.make(Engine)
+    set_tile(a, eng)
     eng.step(a.warmup)
     _, ms_kernel = eng.timed_steps(a.kernel_steps)
     rl = roofline(eng.step_bytes(), ms_kernel, None if a.tile else s0.name)

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned_tiles.json")
 
 REGISTER_ROWS = (1, 2)
-XCHUNKS = (4, 8, 16, 32, 64)
+XCHUNKS = (4, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64)
 
 
 def candidates(shape, xchunks=XCHUNKS):
@@ -96,12 +96,13 @@ def main():
     from sim_juncs_amd.engine import Engine
 
     p = argparse.ArgumentParser()
-    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain", "adjoint", "large"])
+    p.add_argument("--workload", default="bowtie", choices=["bowtie", "plain", "adjoint", "large", "variant"])
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--store", action="store_true")
     a = p.parse_args()
     scn = {"bowtie": scenario.bowtie_junction, "plain": scenario.plain_box,
-           "adjoint": scenario.adjoint_junction, "large": scenario.large_junction}[a.workload]()
+           "adjoint": scenario.adjoint_junction, "large": scenario.large_junction,
+           "variant": lambda: scenario.adjoint_variants()[0]}[a.workload]()
     eng = scn.make(Engine)
     best, res = tune(eng, steps=a.steps)
     bytes_step = eng.step_bytes()
