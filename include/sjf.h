@@ -205,10 +205,15 @@ int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
  * objective J = dt * sum_{n=1..nsteps} |E^n(gap)|^2 (E of the cell `gap`, all three components)
  * and its exact gradient with respect to every design-region permittivity (sjf_set_design_eps;
  * grad in the same z-fastest order, cap >= number of design voxels).  The gradient is the
- * discrete adjoint of the update the engine runs: the forward pass stores the full state every
- * `checkpoint_every` steps, the backward pass recomputes each segment from its checkpoint and
- * applies the transposed step (Yee curls, CPML and pole recursions) in reverse.  Monitors and
- * DFT regions see the forward pass once; the engine ends in its final forward state. */
+ * discrete adjoint of the update the engine runs, applied in reverse (Yee curls, CPML and pole
+ * recursions).  The reverse pass needs E of the design region and of `gap` at every step:
+ *   checkpoint_every = K > 0: the forward pass stores the full state every K steps and the
+ *     reverse pass recomputes each segment from its checkpoint;
+ *   checkpoint_every = 0: the forward pass records those values after every step when the record
+ *     fits a quarter of the free device memory (no recomputation), else it checkpoints every
+ *     ~sqrt(nsteps) steps.
+ * Both give identical results.  Monitors and DFT regions see the forward pass once; the engine
+ * ends in its final forward state. */
 int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                     double* grad, size_t cap);
 

@@ -1041,9 +1041,22 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     int rc = prepare_set(s, nsteps);
     if (rc) return rc;
     const StateLayout L = state_layout(e);
-    const long nck = (nsteps + K - 1) / K;
+    const long slot = 3 * nd + 3;  // E of the design region and of the objective cell
+    // Record mode (K == 0 and the record fits a quarter of the free device memory): the forward
+    // pass records E of the design region and the objective cell after every step, so the
+    // reverse pass needs no recomputation.  Otherwise the forward pass checkpoints the full state
+    // every K steps (K == 0: about sqrt(nsteps)) and the reverse pass recomputes each segment.
+    bool rec_all = false;
+    if (K == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(e, hipMemGetInfo(&fr, &tot));
+        rec_all = (double)sizeof(T) * slot * (nsteps + 1) <= 0.25 * (double)fr;
+        if (!rec_all) K = std::max(1L, std::lround(std::sqrt((double)nsteps)));
+    }
+    const long nck = rec_all ? 0 : (nsteps + K - 1) / K;
+    const long nslots = rec_all ? nsteps + 1 : K + 1;
     char* cks = nullptr;
-    if (hipMalloc((void**)&cks, L.total * (nck + 1)) != hipSuccess)
+    if (!rec_all && hipMalloc((void**)&cks, L.total * (nck + 1)) != hipSuccess)
         return fail(e, SJF_E_NOMEM, "checkpoint storage allocation failed (" +
                                         std::to_string((double)L.total * (nck + 1) / 1e9) + " GB)");
     double* dJ = nullptr;
@@ -1067,10 +1080,9 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         cleanup();
         return r;
     };
-    const long slot = 3 * nd + 3;
     // adjoint fields: two sets of six components (read / written per reverse step)
     if (dev_alloc(e, (void**)&dJ, sizeof(double)) || dev_alloc(e, (void**)&gacc, sizeof(double) * nd) ||
-        dev_alloc(e, (void**)&ring, sizeof(T) * slot * (K + 1)) || dev_alloc(e, (void**)&A, sizeof(T) * 12 * e->N) ||
+        dev_alloc(e, (void**)&ring, sizeof(T) * slot * nslots) || dev_alloc(e, (void**)&A, sizeof(T) * 12 * e->N) ||
         (L.q && dev_alloc(e, (void**)&Aq, L.q)) || (L.p && dev_alloc(e, (void**)&Ap, L.p)))
         return bail(SJF_E_NOMEM);
     // auxiliary bars: per axis 8 slots (E pair, two H pairs, second E pair)
@@ -1082,22 +1094,34 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     }
 
     const double dt = e->dt;
-    // ---- forward pass: checkpoints every K steps, objective after every step
+    const unsigned gdes = (unsigned)((nd + 3 + 255) / 256);
+    const int de0 = e->dhi[0] - e->dlo[0], de1 = e->dhi[1] - e->dlo[1], de2 = e->dhi[2] - e->dlo[2];
+    const long nyz = (long)e->g.ny * e->g.nz;
+    auto record = [&](long sl) -> int {
+        const T* E = (const T*)e->Fbuf[e->step & 1];
+        hipLaunchKernelGGL((sjf::k_adj_record<T>), dim3(gdes), dim3(256), 0, e->stream, E, nyz, e->g.ny, e->g.nz,
+                           e->dlo[0], e->dlo[1], e->dlo[2], de0, de1, de2, gap, ring, sl);
+        HIPCHK(e, hipGetLastError());
+        return SJF_OK;
+    };
+    // ---- forward pass: the objective after every step; checkpoints every K steps or the record
+    if (rec_all && (rc = record(0))) return bail(rc);
     for (long n = 0; n < nsteps; ++n) {
-        if (n % K == 0) {
+        if (!rec_all && n % K == 0) {
             if ((rc = copy_state(e, cks + L.total * (n / K), L, true))) return bail(rc);
         }
         if ((rc = launch_one_step(s, samples_at(s, e->step)))) return bail(rc);
         advance_hosts(s, 1);
         const T* E = (const T*)e->Fbuf[e->step & 1];
-        hipLaunchKernelGGL((sjf::k_adj_objective<T>), dim3(1), dim3(64), 0, e->stream, E, (long)e->g.ny * e->g.nz, gap, dt, dJ);
+        hipLaunchKernelGGL((sjf::k_adj_objective<T>), dim3(1), dim3(64), 0, e->stream, E, nyz, gap, dt, dJ);
         HIPCHK(e, hipGetLastError());
+        if (rec_all && (rc = record(n + 1))) return bail(rc);
     }
-    if ((rc = copy_state(e, cks + L.total * nck, L, true))) return bail(rc);
+    if (!rec_all && (rc = copy_state(e, cks + L.total * nck, L, true))) return bail(rc);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     if ((rc = d2h(e, J_out, dJ, sizeof(double)))) return bail(rc);
 
-    // ---- backward pass, segment by segment
+    // ---- backward pass
     sjf::AdjArgs<T> g;
     std::memset(&g, 0, sizeof(g));
     g.f = step_args<T>(e, 1, adj_xchunk(e));  // one y row per workgroup
@@ -1112,57 +1136,58 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     const unsigned tiles = (unsigned)((long)e->g.ny * ((e->g.nx + g.f.xc - 1) / g.f.xc));
     const size_t lds = sizeof(T) * 12 * block + sizeof(sjf::MatLut<T>) * e->nmat;
     if (lds > 65536) return bail(fail(e, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
-    const unsigned gdes = (unsigned)((nd + 3 + 255) / 256);
-    const int de0 = e->dhi[0] - e->dlo[0], de1 = e->dhi[1] - e->dlo[1], de2 = e->dhi[2] - e->dlo[2];
-    auto record = [&](long sl) -> int {
-        const T* E = (const T*)e->Fbuf[e->step & 1];
-        hipLaunchKernelGGL((sjf::k_adj_record<T>), dim3(gdes), dim3(256), 0, e->stream, E, (long)e->g.ny * e->g.nz, e->g.ny, e->g.nz,
-                           e->dlo[0], e->dlo[1], e->dlo[2], de0, de1, de2, gap, ring, sl);
+    const int npv = np_variant_of(e->maxp, e->uniform);
+    auto ebar = [&]() { return g.A[g.rd]; };
+    auto inject = [&](long sl) -> int {
+        hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), nyz, ring, nd, sl, gap, dt);
         HIPCHK(e, hipGetLastError());
         return SJF_OK;
     };
-    const int npv = np_variant_of(e->maxp, e->uniform);
-    auto ebar = [&]() { return g.A[g.rd]; };
-    for (long k = nck - 1; k >= 0; --k) {
-        const long n0 = k * K, n1 = std::min(n0 + K, nsteps);
-        if ((rc = set_step(e, n0))) return bail(rc);
-        if ((rc = copy_state(e, cks + L.total * k, L, false))) return bail(rc);
-        if ((rc = record(0))) return bail(rc);
-        for (long n = n0; n < n1; ++n) {
-            if ((rc = launch_bare_step(s))) return bail(rc);
-            e->step++;
-            if ((rc = record(n + 1 - n0))) return bail(rc);
-        }
-        if (k == nck - 1) {
-            hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), (long)e->g.ny * e->g.nz, ring, nd,
-                               n1 - n0, gap, dt);
-            HIPCHK(e, hipGetLastError());
-        }
+    // reverse steps n1 - 1 .. n0 with the record slots of steps n0 .. n1 at slot (n - base)
+    auto reverse = [&](long n0, long n1, long base) -> int {
         for (long n = n1 - 1; n >= n0; --n) {
             hipLaunchKernelGGL((sjf::k_adj_grad<T>), dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, e->stream,
-                               g.f, ebar(), ring, n + 1 - n0, n - n0, gacc);
+                               g.f, ebar(), ring, n + 1 - base, n - base, gacc);
             HIPCHK(e, hipGetLastError());
             g.hr = (n & 1) ? 2 : 4;
             g.hw = (n & 1) ? 4 : 2;
             g.er = (n & 1) ? 0 : 6;
             g.ew = (n & 1) ? 6 : 0;
-            if ((rc = launch_adj_step<T>(e, g, npv, block, tiles, lds))) return bail(rc);
+            int r = launch_adj_step<T>(e, g, npv, block, tiles, lds);
+            if (r) return r;
             g.rd = 1 - g.rd;
-            if (n >= 1) {
-                hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), (long)e->g.ny * e->g.nz, ring, nd,
-                                   n - n0, gap, dt);
-                HIPCHK(e, hipGetLastError());
+            if (n >= 1 && (r = inject(n - base))) return r;
+        }
+        return SJF_OK;
+    };
+    if (rec_all) {
+        if ((rc = inject(nsteps))) return bail(rc);
+        if ((rc = reverse(0, nsteps, 0))) return bail(rc);
+    } else {
+        for (long k = nck - 1; k >= 0; --k) {
+            const long n0 = k * K, n1 = std::min(n0 + K, nsteps);
+            if ((rc = set_step(e, n0))) return bail(rc);
+            if ((rc = copy_state(e, cks + L.total * k, L, false))) return bail(rc);
+            if ((rc = record(0))) return bail(rc);
+            for (long n = n0; n < n1; ++n) {
+                if ((rc = launch_bare_step(s))) return bail(rc);
+                e->step++;
+                if ((rc = record(n + 1 - n0))) return bail(rc);
             }
+            if (k == nck - 1 && (rc = inject(n1 - n0))) return bail(rc);
+            if ((rc = reverse(n0, n1, n0))) return bail(rc);
         }
     }
     HIPCHK(e, hipStreamSynchronize(e->stream));
     std::vector<double> gh(nd);
     if ((rc = d2h(e, gh.data(), gacc, sizeof(double) * nd))) return bail(rc);
     for (long v = 0; v < nd; ++v) grad[v] = -gh[v] / e->deps_h[v];
-    // leave the engine in its final forward state
-    if ((rc = set_step(e, nsteps))) return bail(rc);
-    if ((rc = copy_state(e, cks + L.total * nck, L, false))) return bail(rc);
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    // leave the engine in its final forward state (record mode never left it)
+    if (!rec_all) {
+        if ((rc = set_step(e, nsteps))) return bail(rc);
+        if ((rc = copy_state(e, cks + L.total * nck, L, false))) return bail(rc);
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
     cleanup();
     return SJF_OK;
 }
@@ -1171,7 +1196,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
 
 extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                                double* grad, size_t cap) {
-    if (!e || nsteps < 1 || checkpoint_every < 1 || !gap || !J || !grad) return SJF_E_ARG;
+    if (!e || nsteps < 1 || checkpoint_every < 0 || !gap || !J || !grad) return SJF_E_ARG;
     if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
     if (e->step != 0) return fail(e, SJF_E_STATE, "the adjoint run starts from step 0");
     if (e->deps_h.empty()) return fail(e, SJF_E_STATE, "no design region (sjf_set_design_eps)");

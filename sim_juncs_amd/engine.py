@@ -251,7 +251,9 @@ class Engine:
 
     def adjoint_run(self, nsteps: int, checkpoint_every: int, gap, n_design: int):
         """Forward run of `nsteps` steps plus the discrete adjoint: returns (J, dJ/d eps) with
-        J = dt * sum_n |E^n(gap)|^2 and the gradient over the design voxels (z fastest)."""
+        J = dt * sum_n |E^n(gap)|^2 and the gradient over the design voxels (z fastest).
+        checkpoint_every = K > 0 checkpoints every K steps and recomputes; 0 records the design
+        fields of every step when they fit (else checkpoints every ~sqrt(nsteps))."""
         J = C.c_double()
         grad = np.zeros(int(n_design))
         self._chk(self.lib.sjf_adjoint_run(self.h, int(nsteps), int(checkpoint_every), _ip(gap),

@@ -95,9 +95,10 @@ def test_adjoint_gradient_matches_oracle_finite_differences(kind):
 
 
 def test_adjoint_independent_of_checkpoint_interval_and_leaves_final_state():
+    """Checkpoint intervals 1, 13, 90 and the record mode (0) give the same bits."""
     scn, gap, eps = design_case("poles", steps=90)
     res = []
-    for K in (1, 13, 90):
+    for K in (1, 13, 90, 0):
         eng = scn.make(Engine)
         J, g = eng.adjoint_run(scn.steps, K, gap, eps.size)
         assert eng.step_count() == scn.steps
