@@ -882,6 +882,7 @@ int upload_lut_t(sjf_engine* e) {
         sjf::MatLut<T>& L = l[m];
         std::memset(&L, 0, sizeof(L));
         L.inv_eps = (T)(1.0 / M.eps_inf);
+        L.eps_inf = (T)M.eps_inf;
         L.np = M.npoles;
         for (int s = 0; s < M.npoles; ++s) {
             // coefficients formed in FP64 (sigma * w0^2 spans ~1e21 * 1e-22 for Au Drude)
@@ -1138,8 +1139,10 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     if (lds > 65536) return bail(fail(e, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
     const int npv = np_variant_of(e->maxp, e->uniform);
     auto ebar = [&]() { return g.A[g.rd]; };
+    const double ie[3] = {inv_eps_at(e, gap, 0), inv_eps_at(e, gap, 1), inv_eps_at(e, gap, 2)};
     auto inject = [&](long sl) -> int {
-        hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), nyz, ring, nd, sl, gap, dt);
+        hipLaunchKernelGGL((sjf::k_adj_inject<T>), dim3(1), dim3(64), 0, e->stream, ebar(), nyz, ring, nd, sl, gap, dt,
+                           ie[0], ie[1], ie[2]);
         HIPCHK(e, hipGetLastError());
         return SJF_OK;
     };
@@ -1181,7 +1184,8 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     HIPCHK(e, hipStreamSynchronize(e->stream));
     std::vector<double> gh(nd);
     if ((rc = d2h(e, gh.data(), gacc, sizeof(double) * nd))) return bail(rc);
-    for (long v = 0; v < nd; ++v) grad[v] = -gh[v] / e->deps_h[v];
+    // the state is Ubar = Ebar / eps, so the accumulated sum already carries the 1/eps_v factor
+    for (long v = 0; v < nd; ++v) grad[v] = -gh[v];
     // leave the engine in its final forward state (record mode never left it)
     if (!rec_all) {
         if ((rc = set_step(e, nsteps))) return bail(rc);

@@ -37,9 +37,9 @@ constexpr int kMaxDftFreqs = 64;
 template <typename T>
 struct MatLut {
     T inv_eps;
+    T eps_inf;
     int np;
     int lorentz_mask;
-    int pad;
     T alpha[kMaxPoles];
     T beta[kMaxPoles];
     T c3[kMaxPoles];
@@ -1192,7 +1192,8 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
 template <typename T>
 struct AdjArgs {
     StepArgs<T> f;     // forward description (geometry, materials, PML tables, tiles)
-    T* A[2];           // adjoint field sets [x][Ebar x,y,z, Hbar x,y,z][y][z]: read A[rd], write A[1 - rd]
+    T* A[2];           // adjoint field sets [x][Ubar x,y,z, Hbar x,y,z][y][z] with Ubar = Ebar / eps:
+                       // read A[rd], write A[1 - rd]
     int rd;
     T* Aq;             // Qbar [x][slot * 3 + comp][y][z]
     T* Ap;             // Pbar, the same layout
@@ -1265,75 +1266,25 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     const int dey = a.dhi[1] - a.dlo[1], dez = a.dhi[2] - a.dlo[2];
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
-    // u = inv_eps * Ebar' on planes i and i + 1 (rows 0..2), raw Ebar' and material words of
-    // plane i + 2 in flight; the owned row's raw Ebar' and material word of planes i and i + 1
-    // are kept for phase 2
+    // u = Ubar' (the state itself) on planes i and i + 1 (rows 0..2) and plane i + 2 in flight;
+    // the owned row's material word of planes i, i + 1 and i + 2 for phase 2
     T uc[3][3], un[3][3], rn[3][3];
-    uint32_t wn[3];
-    T ec1[3], en1[3];
-    uint32_t wc1 = 0u, wn1 = 0u;
-    auto load_raw = [&](int p, T (&raw)[3][3], uint32_t (&w)[3]) {
+    uint32_t wc1 = 0u, wn1 = 0u, wnn = 0u;
+    auto load_u = [&](int p, T (&u)[3][3], uint32_t& w1) {
         const int pp = wrap_or_ghost(p, nx, a.px);
         const long pc = pp >= 0 ? pp : 0;
         const Rsrc rf = rsrc(AR + pc * FPB);
-        const Rsrc rm = rsrc(matw + pc * PW);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const uint32_t o = pp >= 0 ? oR[r] : kOOB;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) raw[r][c] = bld<T>(rf, o, c * PB);
-            w[r] = 0u;
-            if (!uniform) w[r] = bld<uint32_t>(rm, pp >= 0 ? oW[r] : kOOB, 0);
+            for (int c = 0; c < 3; ++c) u[r][c] = bld<T>(rf, o, c * PB);
         }
-    };
-    auto to_u = [&](int p, const T (&raw)[3][3], const uint32_t (&w)[3], T (&u)[3][3]) {
-        const int pp = wrap_or_ghost(p, nx, a.px);
-        if (uniform) {
-            const T inv = slut[0].inv_eps;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) u[r][c] = inv * raw[r][c];
-            return;
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            bool des[3];
-            T inv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int m = (w[r] >> (8 * c)) & 0xff;
-                des[c] = m == kDesignMat;
-                inv[c] = slut[des[c] ? 0 : m].inv_eps;
-            }
-            if (wave_any(des[0] || des[1] || des[2])) {
-                const uint32_t dq =
-                    (uint32_t)(((pp - a.dlo[0]) * dey + (gj[r] - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
-                    inv[c] = des[c] ? v : inv[c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) u[r][c] = inv[c] * raw[r][c];
-        }
+        w1 = uniform ? 0u : bld<uint32_t>(rsrc(matw + pc * PW), pp >= 0 ? oW[1] : kOOB, 0);
     };
     lds_barrier();  // LUT in LDS
-    {
-        T raw[3][3];
-        uint32_t w[3];
-        load_raw(istart, raw, w);
-        to_u(istart, raw, w, uc);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) ec1[c] = raw[1][c];
-        wc1 = w[1];
-        load_raw(istart + 1, raw, w);
-        to_u(istart + 1, raw, w, un);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) en1[c] = raw[1][c];
-        wn1 = w[1];
-    }
+    load_u(istart, uc, wc1);
+    load_u(istart + 1, un, wn1);
     T dbx_prev[2] = {T(0), T(0)};  // x H-terms of row 0 on plane i - 1: (Hz -> Ey), (Hy -> Ez)
 
     for (int i = istart; i < x1; ++i) {
@@ -1341,7 +1292,7 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         const bool has_next = i + 1 < x1;
         const int pi = wrap_or_ghost(i, nx, a.px);
         const int pn = wrap_or_ghost(i + 1, nx, a.px);
-        if (has_next) load_raw(i + 2, rn, wn);
+        if (has_next) load_u(i + 2, rn, wnn);
         const bool xpml = in_pml(a, 0, pi, nx);
         const bool xpmln = pn >= 0 && in_pml(a, 0, pn, nx);
         const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
@@ -1526,22 +1477,47 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 const T hyz_k = km1 >= 0 ? bufd[0 * B + km1] : T(0);
                 const T hxz_k = km1 >= 0 ? bufd[1 * B + km1] : T(0);
                 // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
-                T out[3];
-                out[0] = ec1[0] + (hyz_k - db[2]) + (dbh[1] - db[5]);
-                out[1] = ec1[1] + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
-                out[2] = ec1[2] + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
                 const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
-                if (NP > 0 && !uniform) {
-                    const uint32_t mw = wc1;
-                    const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
-                    const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
-                    int mm[3], npl[3];
+                // material of the owned cell: 1/eps and eps per component (design voxels: 1/eps
+                // from the design array)
+                int mm[3] = {0, 0, 0}, npl[3] = {0, 0, 0};
+                T inv[3], epsc[3];
+                if (uniform) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const int m = (mw >> (8 * c)) & 0xff;
-                        mm[c] = m == kDesignMat ? 0 : m;
-                        npl[c] = m == kDesignMat ? 0 : slut[mm[c]].np;
+                        inv[c] = slut[0].inv_eps;
+                        epsc[c] = slut[0].eps_inf;
                     }
+                } else {
+                    bool des[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const int m = (wc1 >> (8 * c)) & 0xff;
+                        des[c] = m == kDesignMat;
+                        mm[c] = des[c] ? 0 : m;
+                        npl[c] = des[c] ? 0 : slut[mm[c]].np;
+                        inv[c] = slut[mm[c]].inv_eps;
+                        epsc[c] = slut[mm[c]].eps_inf;
+                    }
+                    if (wave_any(des[0] || des[1] || des[2])) {
+                        const uint32_t dq =
+                            (uint32_t)(((pi - a.dlo[0]) * dey + (j - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
+                            inv[c] = des[c] ? v : inv[c];
+                            epsc[c] = des[c] ? T(1) / v : epsc[c];
+                        }
+                    }
+                }
+                // Ebar' = eps Ubar' plus the differences
+                T out[3];
+                out[0] = epsc[0] * u0[0] + (hyz_k - db[2]) + (dbh[1] - db[5]);
+                out[1] = epsc[1] * u0[1] + (hxz_k - db[1]) + (dbx_prev[0] - db[4]);
+                out[2] = epsc[2] * u0[2] + (dbh[0] - db[0]) + (dbx_prev[1] - db[3]);
+                if (NP > 0 && !uniform) {
+                    const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
+                    const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
                         if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
@@ -1576,9 +1552,10 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                     bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + pex[0]));
                     bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + pex[1]));
                 }
-                bst(rB, o, 0, out[0]);
-                bst(rB, o, PB, out[1]);
-                bst(rB, o, 2 * PB, out[2]);
+                // the state is Ubar = Ebar / eps
+                bst(rB, o, 0, inv[0] * out[0]);
+                bst(rB, o, PB, inv[1] * out[1]);
+                bst(rB, o, 2 * PB, inv[2] * out[2]);
             }
         } else {
             lds_barrier();  // keep the barrier count uniform across iterations
@@ -1589,14 +1566,13 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) uc[r][c] = un[r][c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) ec1[c] = en1[c];
         wc1 = wn1;
         if (has_next) {
-            to_u(i + 2, rn, wn, un);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) en1[c] = rn[1][c];
-            wn1 = wn[1];
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) un[r][c] = rn[r][c];
+            wn1 = wnn;
         }
     }
 }
@@ -1649,8 +1625,10 @@ __global__ void k_adj_objective(const T* E, long nyz, long gap, double dt, doubl
 }
 
 // gradient term of step n:
-//   gacc[v] += sum_c Ebar_c^{n+1}(v) (E_c^{n+1}(v) - E_c^n(v))   (before the adjoint step)
-// Ebar: the adjoint field set holding Ebar^{n+1} ([x][comp][y][z]).
+//   gacc[v] += sum_c Ubar_c^{n+1}(v) (E_c^{n+1}(v) - E_c^n(v))   (before the adjoint step)
+// With Ubar = Ebar / eps_v this is (1/eps_v) sum_c Ebar_c^{n+1} (E^{n+1} - E^n): the gradient
+// dJ/d eps_v = -(1/eps_v) sum_n sum_c Ebar (E^{n+1} - E^n) is minus the accumulated sum.
+// Ebar: the adjoint field set holding Ubar^{n+1} ([x][comp][y][z]).
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_grad(const StepArgs<T> a, const T* Ebar, const T* ring, long s_next,
                                                   long s_cur, double* gacc) {
@@ -1671,14 +1649,17 @@ __global__ void __launch_bounds__(256) k_adj_grad(const StepArgs<T> a, const T* 
     gacc[v] += acc;
 }
 
-// Ebar^n(gap) += dJ/dE^n = 2 dt E^n(gap)
+// Ebar^n(gap) += dJ/dE^n = 2 dt E^n(gap), on the state Ubar = Ebar / eps: Ubar(gap) gains
+// 2 dt E^n(gap) / eps(gap) per component (ie = 1/eps at the gap's three E points)
 template <typename T>
-__global__ void k_adj_inject(T* Ebar, long nyz, const T* ring, long nd, long slot, long gap, double dt) {
+__global__ void k_adj_inject(T* Ubar, long nyz, const T* ring, long nd, long slot, long gap, double dt, double ie0,
+                             double ie1, double ie2) {
     const int c = threadIdx.x;
     if (blockIdx.x == 0 && c < 3) {
         const double e = (double)ring[slot * (3 * nd + 3) + 3 * nd + c];
+        const double ie = c == 0 ? ie0 : (c == 1 ? ie1 : ie2);
         const long o = field_index(gap, c, nyz);
-        Ebar[o] = (T)((double)Ebar[o] + 2.0 * dt * e);
+        Ubar[o] = (T)((double)Ubar[o] + 2.0 * dt * e * ie);
     }
 }
 
