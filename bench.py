@@ -68,18 +68,24 @@ def cpu_baseline(scns, steps):
                       f"same workload, {steps} steps each after 1 warm-up step, {secs:.1f} s"}
 
 
-def roofline(bytes_step, ms_kernel, workload=None):
-    """Roofline of the fused step kernel: algorithmic bytes per launch over its mean duration.
-    `traffic` is the HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    (stored by `python -m sim_juncs_amd.profsum traffic ... --store`), reported only when it was
-    measured on the kernel sources being run."""
+def roofline(bytes_step, ms_kernel, kernel, threads, groups, workload):
+    """Roofline of the dominant kernel: algorithmic bytes per launch over its mean duration
+    (HIP events on the engine's stream).  `traffic` is the HBM bytes per launch measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (stored by `python -m sim_juncs_amd.profsum
+    traffic ... --store`), reported only when it was measured on the kernel sources being run at
+    the same launch shape."""
     from sim_juncs_amd import profsum
 
     achieved = bytes_step / (ms_kernel * 1e-3) / 1e9
-    rec = profsum.stored_traffic(workload) if workload else None
+    rec = profsum.stored_traffic(workload, kernel, threads, groups)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": rec["traffic_bytes"] if rec else None,
-            "kernel": "k_step", "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step}
+            "kernel": kernel, "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step,
+            "threads": threads, "groups": groups}
+
+
+def step_kernel(info):
+    return "k_step" if info["form"] == "register_rows" else "k_step_rows"
 
 
 def set_tile(a, eng):
@@ -108,9 +114,8 @@ def single(a, scn, tag):
     wall = time.perf_counter() - t
     log(f"timed {steps} steps in {wall:.3f}s")
     _, ms_kernel = eng.timed_steps(a.kernel_steps)
-    # stored traffic was measured with the tuned tile: not reported for a --tile override
-    rl = roofline(eng.step_bytes(), ms_kernel, None if a.tile else scn.name)
     info = eng.launch_info()
+    rl = roofline(eng.step_bytes(), ms_kernel, step_kernel(info), info["threads"], info["groups"], scn.name)
     eng.close()
     cpu = None
     if not a.no_cpu:
@@ -139,6 +144,7 @@ def sweep(a):
     b.synchronize()
     wall = time.perf_counter() - t
     _, ms_kernel = b.timed_steps(a.kernel_steps)
+    threads, groups = b.launch_info()
     b.close()
     for e in engs:
         e.close()
@@ -146,7 +152,8 @@ def sweep(a):
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
                         "launch": "one fused launch per step over all instances, hipGraph replay"},
-                roofline=roofline(bytes_step, ms_kernel, f"sweep_{len(scns)}_instances"), cpu=cpu, dtype="f32")
+                roofline=roofline(bytes_step, ms_kernel, "k_step_rows", threads, groups,
+                                  f"sweep_{len(scns)}_instances"), cpu=cpu, dtype="f32")
 
 
 def adjoint(a, variants=False):
@@ -157,23 +164,23 @@ def adjoint(a, variants=False):
     steps = a.steps if a.steps is not None else scns[0].steps
     cells = 0
     wall = 0.0
+    rev = None
     for s in scns:
         eng = s.make(Engine)
-        set_tile(a, eng)  # the forward and recomputed steps run the tuned tile
+        set_tile(a, eng)  # the forward steps and the reverse steps run the tuned x-chunk
         nd = s.design[2].size
         t = time.perf_counter()
         J, g = eng.adjoint_run(steps, 0, s.meta["gap"], nd)  # record mode (no recomputation)
         wall += time.perf_counter() - t
         cells += s.cells
         log(f"{s.name}: J={J:.4e} |grad|={float((g * g).sum()) ** 0.5:.4e}")
+        if rev is None:
+            # the fused reverse step is the dominant kernel; it moves the forward step's bytes
+            rev = (eng.step_bytes(),) + eng.adjoint_kernel_info()
+            log(f"reverse kernel {rev[1] * 1e3:.1f} us over {rev[3]} workgroups of {rev[2]} threads")
         eng.close()
     s0 = scns[0]
-    eng = s0.make(Engine)
-    set_tile(a, eng)
-    eng.step(a.warmup)
-    _, ms_kernel = eng.timed_steps(a.kernel_steps)
-    rl = roofline(eng.step_bytes(), ms_kernel, None if a.tile else s0.name)
-    eng.close()
+    rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], s0.name)
     cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or 60)
     if cpu:
         # the adjoint costs a forward and a reverse sweep per step (the design fields of every

@@ -216,6 +216,11 @@ int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
  * ends in its final forward state. */
 int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                     double* grad, size_t cap);
+/* The fused reverse-step kernel of the last sjf_adjoint_run: mean duration in ms (HIP events on
+ * the engine's stream around the first min(nsteps, 64) reverse launches), threads per workgroup
+ * and workgroups.  SJF_E_STATE before any adjoint run.  It moves the same algorithmic bytes per
+ * launch as the forward update (sjf_step_bytes).  New (accounting). */
+int sjf_adjoint_kernel_info(const sjf_engine* e, double* ms_mean, int* threads, int* groups);
 
 /* ---- batches ----------------------------------------------------------------------------
  * New (the reference sweeps parameters with one meep process per SLURM task,
@@ -235,6 +240,9 @@ int sjf_batch_synchronize(sjf_batch* b);
 int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean);
 /* Total cells and algorithmic bytes of one fused launch over the batch. */
 int sjf_batch_info(const sjf_batch* b, long* cells, double* bytes_per_step);
+/* Threads per workgroup and workgroups of the batch's fused launch as last built (SJF_E_STATE
+ * before the first step). */
+int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups);
 const char* sjf_batch_last_error(const sjf_batch* b);
 int sjf_batch_destroy(sjf_batch* b);
 

@@ -105,6 +105,10 @@ struct sjf_engine {
     bool tile_set = false;
     int graph_chunk = 32;  // steps per captured graph (0: no graphs)
 
+    // fused reverse-step kernel of the last adjoint run: mean duration and launch shape
+    double adj_ms = 0;
+    int adj_threads = 0, adj_groups = 0;
+
     LaunchSet* own = nullptr;
     sjf_batch* batch = nullptr;
 };
@@ -1146,6 +1150,23 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         HIPCHK(e, hipGetLastError());
         return SJF_OK;
     };
+    // HIP events around the fused reverse kernel of the first kTimed reverse steps (its mean
+    // duration is reported by sjf_adjoint_kernel_info)
+    constexpr int kTimed = 64;
+    std::vector<hipEvent_t> evs;
+    long ntimed = 0;
+    auto free_events = [&]() {
+        for (auto& ev : evs) (void)hipEventDestroy(ev);
+        evs.clear();
+    };
+    for (int q = 0; q < 2 * kTimed; ++q) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreate(&ev) != hipSuccess) {
+            free_events();
+            return bail(fail(e, SJF_E_HIP, "hipEventCreate failed"));
+        }
+        evs.push_back(ev);
+    }
     // reverse steps n1 - 1 .. n0 with the record slots of steps n0 .. n1 at slot (n - base)
     auto reverse = [&](long n0, long n1, long base) -> int {
         for (long n = n1 - 1; n >= n0; --n) {
@@ -1156,32 +1177,50 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
             g.hw = (n & 1) ? 4 : 2;
             g.er = (n & 1) ? 0 : 6;
             g.ew = (n & 1) ? 6 : 0;
+            const bool timed = ntimed < kTimed;
+            if (timed) HIPCHK(e, hipEventRecord(evs[2 * ntimed], e->stream));
             int r = launch_adj_step<T>(e, g, npv, block, tiles, lds);
             if (r) return r;
+            if (timed) HIPCHK(e, hipEventRecord(evs[2 * ntimed++ + 1], e->stream));
             g.rd = 1 - g.rd;
             if (n >= 1 && (r = inject(n - base))) return r;
         }
         return SJF_OK;
     };
+    auto bail_ev = [&](int r) {
+        free_events();
+        return bail(r);
+    };
     if (rec_all) {
-        if ((rc = inject(nsteps))) return bail(rc);
-        if ((rc = reverse(0, nsteps, 0))) return bail(rc);
+        if ((rc = inject(nsteps))) return bail_ev(rc);
+        if ((rc = reverse(0, nsteps, 0))) return bail_ev(rc);
     } else {
         for (long k = nck - 1; k >= 0; --k) {
             const long n0 = k * K, n1 = std::min(n0 + K, nsteps);
-            if ((rc = set_step(e, n0))) return bail(rc);
-            if ((rc = copy_state(e, cks + L.total * k, L, false))) return bail(rc);
-            if ((rc = record(0))) return bail(rc);
+            if ((rc = set_step(e, n0))) return bail_ev(rc);
+            if ((rc = copy_state(e, cks + L.total * k, L, false))) return bail_ev(rc);
+            if ((rc = record(0))) return bail_ev(rc);
             for (long n = n0; n < n1; ++n) {
-                if ((rc = launch_bare_step(s))) return bail(rc);
+                if ((rc = launch_bare_step(s))) return bail_ev(rc);
                 e->step++;
-                if ((rc = record(n + 1 - n0))) return bail(rc);
+                if ((rc = record(n + 1 - n0))) return bail_ev(rc);
             }
-            if (k == nck - 1 && (rc = inject(n1 - n0))) return bail(rc);
-            if ((rc = reverse(n0, n1, n0))) return bail(rc);
+            if (k == nck - 1 && (rc = inject(n1 - n0))) return bail_ev(rc);
+            if ((rc = reverse(n0, n1, n0))) return bail_ev(rc);
         }
     }
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return bail_ev(fail(e, SJF_E_HIP, "hipStreamSynchronize failed"));
+    double msum = 0;
+    for (long q = 0; q < ntimed; ++q) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, evs[2 * q], evs[2 * q + 1]) != hipSuccess)
+            return bail_ev(fail(e, SJF_E_HIP, "hipEventElapsedTime failed"));
+        msum += ms;
+    }
+    free_events();
+    e->adj_ms = ntimed ? msum / ntimed : 0;
+    e->adj_threads = block;
+    e->adj_groups = (int)tiles;
     std::vector<double> gh(nd);
     if ((rc = d2h(e, gh.data(), gacc, sizeof(double) * nd))) return bail(rc);
     // the state is Ubar = Ebar / eps, so the accumulated sum already carries the 1/eps_v factor
@@ -1213,6 +1252,15 @@ extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every,
     const long q = cell_index(e, gap[0], gap[1], gap[2]);
     return e->prec == 4 ? adjoint_run_t<float>(e, nsteps, checkpoint_every, q, J, grad, nd)
                         : adjoint_run_t<double>(e, nsteps, checkpoint_every, q, J, grad, nd);
+}
+
+extern "C" int sjf_adjoint_kernel_info(const sjf_engine* e, double* ms_mean, int* threads, int* groups) {
+    if (!e || !ms_mean) return SJF_E_ARG;
+    if (e->adj_groups == 0) return SJF_E_STATE;
+    *ms_mean = e->adj_ms;
+    if (threads) *threads = e->adj_threads;
+    if (groups) *groups = e->adj_groups;
+    return SJF_OK;
 }
 
 // ============================================================================ batches ====
@@ -1874,6 +1922,14 @@ int sjf_batch_info(const sjf_batch* b, long* cells, double* bytes_per_step) {
     }
     if (cells) *cells = c;
     if (bytes_per_step) *bytes_per_step = by;
+    return SJF_OK;
+}
+
+int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups) {
+    if (!b) return SJF_E_ARG;
+    if (!b->set->built) return SJF_E_STATE;
+    if (threads) *threads = b->set->block;
+    if (groups) *groups = b->set->ntiles;
     return SJF_OK;
 }
 

@@ -260,6 +260,13 @@ class Engine:
                                            C.byref(J), _dp(grad), grad.size), "sjf_adjoint_run")
         return J.value, grad
 
+    def adjoint_kernel_info(self):
+        """(mean ms, threads, workgroups) of the fused reverse-step kernel of the last adjoint run."""
+        ms, th, gr = C.c_double(), C.c_int(), C.c_int()
+        self._chk(self.lib.sjf_adjoint_kernel_info(self.h, C.byref(ms), C.byref(th), C.byref(gr)),
+                  "sjf_adjoint_kernel_info")
+        return ms.value, th.value, gr.value
+
     def set_graph(self, chunk: int):
         """Steps per replayed hipGraph (0: launch every step directly)."""
         self._chk(self.lib.sjf_set_graph(self.h, int(chunk)), "sjf_set_graph")
@@ -307,6 +314,12 @@ class Batch:
         c, b = C.c_long(), C.c_double()
         self._chk(self.lib.sjf_batch_info(self.h, C.byref(c), C.byref(b)), "sjf_batch_info")
         return c.value, b.value
+
+    def launch_info(self):
+        """(threads per workgroup, workgroups) of the fused launch as last built."""
+        th, gr = C.c_int(), C.c_int()
+        self._chk(self.lib.sjf_batch_launch_info(self.h, C.byref(th), C.byref(gr)), "sjf_batch_launch_info")
+        return th.value, gr.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -23,22 +23,39 @@ COPY_BYTES = 1 << 30  # the calibration copy kernel reads and writes 1 GiB per l
 
 
 def kernel_source_hash() -> str:
-    """Hash of the engine sources: a stored traffic figure is valid only for the kernels it was
-    measured on."""
+    """Hash of the device code: a stored traffic figure is valid only for the kernels it was
+    measured on (their launch shape is checked separately)."""
     h = hashlib.sha256()
-    for f in ("sjf_kernels.h", "sjf_engine.hip"):
-        with open(os.path.join(HERE, "csrc", f), "rb") as fh:
-            h.update(fh.read())
+    with open(os.path.join(HERE, "csrc", "sjf_kernels.h"), "rb") as fh:
+        h.update(fh.read())
     return h.hexdigest()[:16]
+
+
+def is_kernel(name: str, kernel: str) -> bool:
+    """True for a dispatch of `kernel` itself (k_step matches sjf::k_step<...>, not k_step_rows)."""
+    base = name.split("(")[0].split("<")[0].strip()
+    return base == kernel or base.endswith("::" + kernel)
 
 
 def counter_mean(db: str, kernel: str, counter: str) -> float:
     con = sqlite3.connect(db)
     v = [cv for name, cn, cv in con.execute("select name, counter_name, counter_value from pmc_events")
-         if kernel in name and cn == counter]
+         if is_kernel(name, kernel) and cn == counter]
     if not v:
         raise SystemExit(f"no {counter} samples of {kernel} in {db}")
     return sum(v) / len(v)
+
+
+def launch_shape(db: str, kernel: str):
+    """(threads per workgroup, workgroups) of the dispatches of `kernel` (one shape expected)."""
+    con = sqlite3.connect(db)
+    shapes = {(wx * wy * wz, (gx // wx) * (gy // wy) * (gz // wz))
+              for name, gx, gy, gz, wx, wy, wz in con.execute(
+                  "select name, grid_x, grid_y, grid_z, workgroup_x, workgroup_y, workgroup_z from kernels")
+              if is_kernel(name, kernel)}
+    if len(shapes) != 1:
+        raise SystemExit(f"{kernel} in {db}: expected one launch shape, found {sorted(shapes)}")
+    return shapes.pop()
 
 
 def traffic(workload, kernel, fetch_db, write_db, copy_fetch_db, copy_write_db, store=False):
@@ -48,28 +65,34 @@ def traffic(workload, kernel, fetch_db, write_db, copy_fetch_db, copy_write_db, 
     fw = COPY_BYTES / (1024.0 * counter_mean(copy_write_db, "k_copy", "WRITE_SIZE"))
     fk = counter_mean(fetch_db, kernel, "FETCH_SIZE")
     wk = counter_mean(write_db, kernel, "WRITE_SIZE")
-    rec = dict(kernel=kernel, fetch_kib=fk, write_kib=wk, fetch_factor=ff, write_factor=fw,
+    threads, groups = launch_shape(fetch_db, kernel)
+    if launch_shape(write_db, kernel) != (threads, groups):
+        raise SystemExit(f"{kernel}: the FETCH_SIZE and WRITE_SIZE runs launched different shapes")
+    rec = dict(fetch_kib=fk, write_kib=wk, fetch_factor=ff, write_factor=fw,
                fetched_bytes=fk * 1024.0 * ff, written_bytes=wk * 1024.0 * fw,
-               traffic_bytes=(fk * ff + wk * fw) * 1024.0, kernel_source=kernel_source_hash())
-    print(json.dumps(dict(workload=workload, **rec), indent=1))
+               traffic_bytes=(fk * ff + wk * fw) * 1024.0, kernel_source=kernel_source_hash(),
+               threads=threads, groups=groups)
+    print(json.dumps(dict(workload=workload, kernel=kernel, **rec), indent=1))
     if store:
         c = {}
         if os.path.exists(TRAFFIC):
             with open(TRAFFIC) as f:
                 c = json.load(f)
-        c[workload] = rec
+        c.setdefault(workload, {})[kernel] = rec
         with open(TRAFFIC, "w") as f:
             json.dump(c, f, indent=1, sort_keys=True)
 
 
-def stored_traffic(workload: str):
-    """Measured HBM bytes per launch for `workload`, or None when absent or measured on other
-    kernel sources."""
+def stored_traffic(workload: str, kernel: str, threads: int, groups: int):
+    """Measured HBM bytes per launch of `kernel` in `workload`, or None when absent, measured on
+    other kernel sources or at another launch shape."""
     if not os.path.exists(TRAFFIC):
         return None
     with open(TRAFFIC) as f:
-        rec = json.load(f).get(workload)
+        rec = json.load(f).get(workload, {}).get(kernel)
     if not rec or rec.get("kernel_source") != kernel_source_hash():
+        return None
+    if (rec.get("threads"), rec.get("groups")) != (threads, groups):
         return None
     return rec
 

@@ -12,6 +12,7 @@ import pytest
 from oracle.oracle import Oracle
 from sim_juncs_amd import materials as M
 from sim_juncs_amd import scenario
+from sim_juncs_amd._lib import EngineError
 from sim_juncs_amd.engine import Engine
 
 pytestmark = pytest.mark.gpu
@@ -111,6 +112,23 @@ def test_adjoint_independent_of_checkpoint_interval_and_leaves_final_state():
         assert np.array_equal(g, res[0][1])
         assert np.array_equal(ex, ref.read_field("Ex"))
     ref.close()
+
+
+def test_adjoint_kernel_info_reports_the_reverse_launch():
+    """The reverse kernel's timing and launch shape: absent before an adjoint run, then one y row
+    per workgroup over x-chunks of the set tile's length."""
+    scn, gap, eps = design_case("poles", precision="fp32", steps=30)
+    eng = scn.make(Engine)
+    eng.set_tile("register_rows", 1, 4)
+    with pytest.raises(EngineError):
+        eng.adjoint_kernel_info()
+    eng.adjoint_run(scn.steps, 0, gap, eps.size)
+    ms, threads, groups = eng.adjoint_kernel_info()
+    nx, ny, nz = scn.shape
+    assert 0 < ms < 100
+    assert threads == -(-nz // 64) * 64
+    assert groups == ny * -(-nx // 4)
+    eng.close()
 
 
 def test_adjoint_fp32_close_to_fp64():

@@ -11,6 +11,7 @@ import pytest
 
 from oracle.oracle import Oracle
 from sim_juncs_amd import scenario
+from sim_juncs_amd._lib import EngineError
 from sim_juncs_amd.engine import Batch, Engine
 
 pytestmark = pytest.mark.gpu
@@ -81,7 +82,15 @@ def test_batch_matches_oracle_fp64(tile):
     b = Batch(eng)
     if tile:
         b.set_tile(*tile)
+    with pytest.raises(EngineError):
+        b.launch_info()  # nothing built before the first step
     b.step(80)
+    threads, groups = b.launch_info()
+    if tile and tile[0] == "register_rows":
+        # one y row per workgroup over x-chunks of the batch's (or each engine's own) length
+        xcs = [min(tile[1], s.shape[0]) if tile[1] else e.launch_info()["xchunk"] for s, e in zip(scns, eng)]
+        assert groups == sum(s.shape[1] * -(-s.shape[0] // xc) for s, xc in zip(scns, xcs))
+        assert threads == max(-(-s.shape[2] // 64) * 64 for s in scns)
     for s, e in zip(scns, eng):
         o = s.make(Oracle)
         o.step(80)
