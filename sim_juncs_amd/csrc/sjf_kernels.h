@@ -1302,11 +1302,77 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         const Rsrc rA = rsrc(AR + (long)pi * FPB);
         const Rsrc rB = rsrc(AW + (long)pi * FPB);
 
+        // ---- the global loads of plane i, issued together (u of the planes ahead is already in
+        // flight): Hbar' of rows -1 and 0, the z auxiliary bars, the owned cell's design 1/eps
+        // and pole state; the y and x auxiliary bars (layers of a few tiles) are loaded in place
+        T hb[2][3];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hb[r][c] = bld<T>(rA, oR[r], (3 + c) * PB);
+        T pzE[2][2] = {{T(0), T(0)}, {T(0), T(0)}}, phz[2] = {T(0), T(0)};
+        if (zany) {  // lanes outside the layers: b = 1 and a zero auxiliary
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                if (!rv[r]) continue;
+                const uint32_t so = (uint32_t)((pi * ny + gj[r]) * Lz) * TS;
+                pzE[r][0] = bld<T>(rpz, oZ, er * SB2 + so);
+                pzE[r][1] = bld<T>(rpz, oZ, (er + 1) * SB2 + so);
+                if (r == 1) {
+                    phz[0] = bld<T>(rpz, oZ, hr * SB2 + so);
+                    phz[1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
+                }
+            }
+        }
+        const bool wr = own_plane && rv[1];
+        // material of the owned cell: pole count per component and the LUT entry (design voxels:
+        // 1/eps from the design array, no poles)
+        int mm[3] = {0, 0, 0}, npl[3] = {0, 0, 0};
+        bool des[3] = {false, false, false};
+        T dv[3];
+        bool desany = false;
+        uint32_t sact = 0u;  // pole slots some lane of the wavefront uses (uniform)
+        T qv[NP > 0 ? NP : 1][3], pv[NP > 0 ? NP : 1][3];
+        if (wr && !uniform) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int m = (wc1 >> (8 * c)) & 0xff;
+                des[c] = m == kDesignMat;
+                mm[c] = des[c] ? 0 : m;
+                npl[c] = des[c] ? 0 : slut[mm[c]].np;
+            }
+            desany = wave_any(des[0] || des[1] || des[2]);
+            if (desany) {
+                const uint32_t dq =
+                    (uint32_t)(((pi - a.dlo[0]) * dey + (gj[1] - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dv[c] = bld<T>(rd, des[c] ? dq : kOOB, 0);
+            }
+            if (NP > 0) {
+                const uint32_t o = oR[1];
+                const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
+                const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
+#pragma unroll
+                for (int s = 0; s < NP; ++s) {
+                    if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+                    sact |= 1u << s;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const bool vq = s < npl[c];
+                        const bool vp = vq && ((slut[mm[c]].lorentz_mask >> s) & 1);
+                        const uint32_t so = (s * 3 + c) * PB;
+                        pv[s][c] = bld<T>(rP, vp ? o : kOOB, so);
+                        qv[s][c] = bld<T>(rQ, vq ? o : kOOB, so);
+                    }
+                }
+            }
+        }
+
         // ---- phase 1: E-terms and Hbar_total on rows -1 and 0 of plane i ----------------
         // E-terms on rows r = 0, 1 (labels: y (Ex->Hz), z (Ex->Hy), z (Ey->Hx), x (Ey->Hz),
         // x (Ez->Hy), y (Ez->Hx)) and the y-terms of row 2
         T eb[3][6];
-        T pey[2] = {T(0), T(0)}, pez[2] = {T(0), T(0)}, pex[2] = {T(0), T(0)};  // owned row's E aux bars
+        T pey[2] = {T(0), T(0)}, pex[2] = {T(0), T(0)};  // owned row's y and x E aux bars
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const bool ok = rv[r];
@@ -1326,15 +1392,9 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 }
             }
             if (r < 2) {
-                if (ok && zany) {  // lanes outside the layers: b = 1 and a zero auxiliary
-                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    const T p0 = bld<T>(rpz, oZ, er * SB2 + so), p1 = bld<T>(rpz, oZ, (er + 1) * SB2 + so);
-                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * p0;
-                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * p1;
-                    if (r == 1) {
-                        pez[0] = p0;
-                        pez[1] = p1;
-                    }
+                if (ok && zany) {
+                    t_ex_z = bEz * t_ex_z + (bEz - T(1)) * pzE[r][0];
+                    t_ey_z = bEz * t_ey_z + (bEz - T(1)) * pzE[r][1];
                 }
                 if (ok && xpml) {
                     const T p0 = bld<T>(rpx, oR[r], er * SB0 + sxo), p1 = bld<T>(rpx, oR[r], (er + 1) * SB0 + sxo);
@@ -1380,31 +1440,22 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
             const T exz_k = kp1 >= 0 ? bufe[(2 * r + 0) * B + kp1] : T(0);
             const T eyz_k = kp1 >= 0 ? bufe[(2 * r + 1) * B + kp1] : T(0);
             // Hx: (Ey, z) and (Ez, y); Hy: (Ex, z) and (Ez, x); Hz: (Ex, y) and (Ey, x)
-            const T hx = bld<T>(rA, oR[r], 3 * PB);
-            const T hy = bld<T>(rA, oR[r], 4 * PB);
-            const T hz = bld<T>(rA, oR[r], 5 * PB);
-            tot[r][0] = ok ? hx + (eb[r][2] - eyz_k) + (eb[r][5] - eb[r + 1][5]) : T(0);
-            tot[r][1] = ok ? hy + (eb[r][1] - exz_k) + (eb[r][4] - ebn[r][1]) : T(0);
-            tot[r][2] = ok ? hz + (eb[r][0] - eb[r + 1][0]) + (eb[r][3] - ebn[r][0]) : T(0);
+            tot[r][0] = ok ? hb[r][0] + (eb[r][2] - eyz_k) + (eb[r][5] - eb[r + 1][5]) : T(0);
+            tot[r][1] = ok ? hb[r][1] + (eb[r][1] - exz_k) + (eb[r][4] - ebn[r][1]) : T(0);
+            tot[r][2] = ok ? hb[r][2] + (eb[r][0] - eb[r + 1][0]) + (eb[r][3] - ebn[r][0]) : T(0);
         }
         // H auxiliary bars of the owned row (read by the update below and by phase 2)
-        T phy[2] = {T(0), T(0)}, phz[2] = {T(0), T(0)}, phx[2] = {T(0), T(0)};
+        T phy[2] = {T(0), T(0)}, phx[2] = {T(0), T(0)};
         if (ypml[1]) {
             const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
             phy[0] = bld<T>(rpy, oK, hr * SB1 + so);
             phy[1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
-        }
-        if (rv[1] && zany) {
-            const uint32_t so = (uint32_t)((pi * ny + gj[1]) * Lz) * TS;
-            phz[0] = bld<T>(rpz, oZ, hr * SB2 + so);
-            phz[1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
         }
         if (rv[1] && xpml) {
             phx[0] = bld<T>(rpx, oR[1], hr * SB0 + sxo);
             phx[1] = bld<T>(rpx, oR[1], (hr + 1) * SB0 + sxo);
         }
         // own row: Hbar at the old time and the new H auxiliary bars
-        const bool wr = own_plane && rv[1];
         if (wr) {
             const int j = gj[1];
             bst(rB, oR[1], 3 * PB, tot[1][0]);
@@ -1478,36 +1529,18 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 const T hxz_k = km1 >= 0 ? bufd[1 * B + km1] : T(0);
                 // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
                 const T u0[3] = {uc[1][0], uc[1][1], uc[1][2]};
-                // material of the owned cell: 1/eps and eps per component (design voxels: 1/eps
-                // from the design array)
-                int mm[3] = {0, 0, 0}, npl[3] = {0, 0, 0};
+                // 1/eps and eps per component of the owned cell
                 T inv[3], epsc[3];
-                if (uniform) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    inv[c] = slut[mm[c]].inv_eps;
+                    epsc[c] = slut[mm[c]].eps_inf;
+                }
+                if (desany) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        inv[c] = slut[0].inv_eps;
-                        epsc[c] = slut[0].eps_inf;
-                    }
-                } else {
-                    bool des[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const int m = (wc1 >> (8 * c)) & 0xff;
-                        des[c] = m == kDesignMat;
-                        mm[c] = des[c] ? 0 : m;
-                        npl[c] = des[c] ? 0 : slut[mm[c]].np;
-                        inv[c] = slut[mm[c]].inv_eps;
-                        epsc[c] = slut[mm[c]].eps_inf;
-                    }
-                    if (wave_any(des[0] || des[1] || des[2])) {
-                        const uint32_t dq =
-                            (uint32_t)(((pi - a.dlo[0]) * dey + (j - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
-                            inv[c] = des[c] ? v : inv[c];
-                            epsc[c] = des[c] ? T(1) / v : epsc[c];
-                        }
+                        inv[c] = des[c] ? dv[c] : inv[c];
+                        epsc[c] = des[c] ? T(1) / dv[c] : epsc[c];
                     }
                 }
                 // Ebar' = eps Ubar' plus the differences
@@ -1520,16 +1553,15 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                     const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
-                        if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+                        if (!((sact >> s) & 1u)) continue;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
                             const MatLut<T>& L = slut[mm[c]];
                             const bool vq = s < npl[c];
                             const bool vp = vq && ((L.lorentz_mask >> s) & 1);
                             const uint32_t so = (s * 3 + c) * PB;
-                            const T pb = bld<T>(rP, vp ? o : kOOB, so);
-                            const T q0 = bld<T>(rQ, vq ? o : kOOB, so);
-                            const T qb = vq ? q0 + pb - u0[c] : T(0);
+                            const T pb = pv[s][c];
+                            const T qb = vq ? qv[s][c] + pb - u0[c] : T(0);
                             bst(rQ, vq ? o : kOOB, so, L.alpha[s] * qb);
                             bst(rP, vp ? o : kOOB, so, pb + L.beta[s] * qb);
                             out[c] += L.c3[s] * qb;
@@ -1545,8 +1577,8 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 }
                 if (zany) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + pez[0]));
-                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + pez[1]));
+                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + pzE[1][0]));
+                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + pzE[1][1]));
                 }
                 if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
                     bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + pex[0]));
@@ -1578,9 +1610,10 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
 }
 
 // Fused reverse step over one engine: workgroup w handles tile w after the XCD-aware remap.
-// The FP32 reverse step fits 80 registers (6 wavefronts per SIMD) without spilling.
+// With every load of a plane issued together, the FP32 three-pole reverse step needs 104
+// registers: 4 wavefronts per SIMD without spilling (5 would spill 32 bytes per lane).
 template <typename T>
-constexpr int adj_waves() { return sizeof(T) == 4 ? 6 : 1; }
+constexpr int adj_waves() { return sizeof(T) == 4 ? 4 : 1; }
 
 template <typename T, int NP, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(adj_waves<T>())))
