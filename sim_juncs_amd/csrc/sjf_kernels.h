@@ -1266,10 +1266,10 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     const int dey = a.dhi[1] - a.dlo[1], dez = a.dhi[2] - a.dlo[2];
 
     const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
-    // u = Ubar' (the state itself) on planes i and i + 1 (rows 0..2) and plane i + 2 in flight;
-    // the owned row's material word of planes i, i + 1 and i + 2 for phase 2
-    T uc[3][3], un[3][3], rn[3][3];
-    uint32_t wc1 = 0u, wn1 = 0u, wnn = 0u;
+    // u = Ubar' (the state itself) on planes i and i + 1 (rows 0..2; plane i + 1 is loaded with
+    // the other loads of iteration i) and the owned row's material word of both planes
+    T uc[3][3], un[3][3];
+    uint32_t wc1 = 0u, wn1 = 0u;
     auto load_u = [&](int p, T (&u)[3][3], uint32_t& w1) {
         const int pp = wrap_or_ghost(p, nx, a.px);
         const long pc = pp >= 0 ? pp : 0;
@@ -1284,15 +1284,13 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     };
     lds_barrier();  // LUT in LDS
     load_u(istart, uc, wc1);
-    load_u(istart + 1, un, wn1);
     T dbx_prev[2] = {T(0), T(0)};  // x H-terms of row 0 on plane i - 1: (Hz -> Ey), (Hy -> Ez)
 
     for (int i = istart; i < x1; ++i) {
         const bool own_plane = i >= x0;
-        const bool has_next = i + 1 < x1;
         const int pi = wrap_or_ghost(i, nx, a.px);
         const int pn = wrap_or_ghost(i + 1, nx, a.px);
-        if (has_next) load_u(i + 2, rn, wnn);
+        load_u(i + 1, un, wn1);
         const bool xpml = in_pml(a, 0, pi, nx);
         const bool xpmln = pn >= 0 && in_pml(a, 0, pn, nx);
         const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
@@ -1599,19 +1597,12 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) uc[r][c] = un[r][c];
         wc1 = wn1;
-        if (has_next) {
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) un[r][c] = rn[r][c];
-            wn1 = wnn;
-        }
     }
 }
 
 // Fused reverse step over one engine: workgroup w handles tile w after the XCD-aware remap.
-// With every load of a plane issued together, the FP32 three-pole reverse step needs 104
-// registers: 4 wavefronts per SIMD without spilling (5 would spill 32 bytes per lane).
+// With the loads of a plane issued together, the FP32 reverse step needs 95 registers with three
+// pole slots (5 wavefronts per SIMD) and 102 with four (4 wavefronts), without spilling.
 template <typename T>
 constexpr int adj_waves() { return sizeof(T) == 4 ? 4 : 1; }
 
