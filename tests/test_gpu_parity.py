@@ -132,6 +132,32 @@ def test_periodic_axes_fp64(periodic, tile):
     eng.close()
 
 
+@pytest.mark.parametrize("shape,periodic,tile", [
+    ((1, 1, 66), (1, 1, 0), None), ((1, 1, 66), (1, 1, 0), ("register_rows", 4, 3)),
+    ((1, 1, 66), (1, 1, 0), ("thread_rows", 5, 3)), ((2, 3, 40), (1, 1, 1), ("register_rows", 4, 5)),
+    ((2, 3, 40), (1, 1, 1), ("thread_rows", 9, 4)), ((5, 2, 70), (0, 1, 0), ("thread_rows", 13, 2)),
+])
+def test_periodic_axes_shorter_than_the_tile_fp64(shape, periodic, tile):
+    """A tile's rows, and the planes ahead of its march, reach more than one period past the
+    edge of a periodic axis shorter than the tile (one-cell axes: 2-D and 1-D runs)."""
+    pml = [0 if periodic[d] else 2 for d in range(3) for _ in range(2)]
+    scn, rng = random_scenario(shape, seed=3, pml=pml, periodic=periodic)
+    eng = scn.make(Engine)
+    if tile:
+        eng.set_tile(*tile)
+    print(f"[case] periodic {shape} {periodic} {tile} {eng.launch_info()}", flush=True)
+    orc = scn.make(Oracle)
+    for c in range(6):
+        f = rng.standard_normal(shape)
+        eng.write_field(c, f)
+        orc.write_field(c, f)
+    eng.step(30)
+    orc.step(30)
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP64_TOL, errs
+    eng.close()
+
+
 def test_bowtie_small_fp32_and_fp64():
     scn = early_pulse(scenario.bowtie_junction(nx=64, ny=64, nz=64, steps=200, pml=6), fwhm_fs=1.5)
     for prec, tol in (("fp64", FP64_TOL), ("fp32", FP32_TOL)):
