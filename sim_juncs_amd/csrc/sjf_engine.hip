@@ -4,7 +4,8 @@
 // polarisation state, sources, monitors, DFT accumulators).  Stepping goes through a launch
 // set: one engine, or a batch of engines, compiled into small device tables so that each
 // time step is a fixed sequence of launches on one stream,
-//   [k_monitor when step % save_span == 0] -> k_step -> k_source -> k_dft -> k_tick,
+//   [k_monitor when step % save_span == 0] -> k_step -> k_source -> k_dft,
+// the last of which advances the step counters (a separate k_tick when neither runs),
 // whatever the number of instances.  The per-step scalars (step index, pulse value, DFT
 // phase) are computed on the device from each engine's step counter, so a run of steps is
 // captured once in a hipGraph and replayed.  This is the sequence bound_geom::run performs
@@ -334,8 +335,9 @@ struct LaunchSet {
     int ndft = 0;
     long dft_total = 0;
     int dft_maxnf = 0;  // most frequencies of any region (the launch's second dimension)
-    // step counters
+    // step counters, and the sign-off word of the last kernel of a step that advances them
     long** d_counters = nullptr;
+    unsigned* d_done = nullptr;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -354,6 +356,7 @@ void set_free_tables(LaunchSet* s) {
     dev_free(s->d_mon);
     dev_free(s->d_dft);
     dev_free(s->d_counters);
+    dev_free(s->d_done);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->gexec = nullptr;
@@ -521,6 +524,8 @@ int build_set_t(LaunchSet* s) {
     for (sjf_engine* e : s->eng) ctr.push_back(e->d_step);
     rc = upload_vec(e0, s->stream, (void**)&s->d_counters, ctr);
     if (rc) return rc;
+    rc = upload_vec(e0, s->stream, (void**)&s->d_done, std::vector<unsigned>{0u});  // armed at zero
+    if (rc) return rc;
 
     s->versions.clear();
     for (sjf_engine* e : s->eng) s->versions.push_back(e->version);
@@ -595,21 +600,26 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
     int rc = launch_step_t<T>(s);
     if (rc) return rc;
     if (ev_b) SETCHK(s, hipEventRecord(ev_b, s->stream));
+    // the last kernel of the step advances the step counters (k_tick when there is none)
+    const int n = (int)s->eng.size();
+    const bool dft = s->ndft && !bare;
+    const sjf::Tick tick{s->d_counters, n, s->d_done}, none{s->d_counters, n, nullptr};
     if (s->nsrcblk) {
         hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream,
                            (const sjf::SrcDev<T>*)s->d_src, s->d_blk_src, s->d_blk_first, s->d_blk_end,
-                           s->d_src_idx, s->d_src_coef);
+                           s->d_src_idx, s->d_src_coef, dft ? none : tick);
         SETCHK(s, hipGetLastError());
     }
-    if (s->ndft && !bare) {
+    if (dft) {
         hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256), (unsigned)s->dft_maxnf),
                            dim3(256), 0,
-                           s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total);
+                           s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total, tick);
         SETCHK(s, hipGetLastError());
     }
-    const int n = (int)s->eng.size();
-    hipLaunchKernelGGL(sjf::k_tick, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->d_counters, n);
-    SETCHK(s, hipGetLastError());
+    if (!s->nsrcblk && !dft) {
+        hipLaunchKernelGGL(sjf::k_tick, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->d_counters, n);
+        SETCHK(s, hipGetLastError());
+    }
     return SJF_OK;
 }
 

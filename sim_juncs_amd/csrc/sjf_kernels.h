@@ -961,6 +961,30 @@ __global__ void k_tick(long* const* counters, int n) {
     if (i < n) counters[i][0] += 1;
 }
 
+// The same, folded into the last kernel of a step (one launch fewer per step): each workgroup
+// signs off on `done` after all of its threads have finished (and so have read the counters);
+// the last one to sign off advances every counter and re-arms `done` for the next step.  No
+// fence is needed: no workgroup reads what another writes within the kernel (the counters
+// were read before signing off, and the kernel boundary publishes the increment).
+// `done` == nullptr: the step ends with k_tick instead.
+struct Tick {
+    long* const* counters;
+    int n;
+    unsigned* done;
+};
+
+__device__ __forceinline__ void tick_tail(const Tick& tk) {
+    if (!tk.done) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+        if (atomicAdd(tk.done, 1u) == nblk - 1) {
+            for (int i = 0; i < tk.n; ++i) tk.counters[i][0] += 1;
+            atomicExch(tk.done, 0u);
+        }
+    }
+}
+
 // ---- source time functions (FP64) ---------------------------------------------------------
 // gaussian_src_time_phase::dipole (reference src/disp.cpp:395-400) and meep's
 // continuous_src_time::dipole, evaluated in FP64 on the device.
@@ -1031,7 +1055,7 @@ __global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ sr
                                                 const long* __restrict__ blk_first,
                                                 const long* __restrict__ blk_end,
                                                 const long* __restrict__ idx,
-                                                const double* __restrict__ coef) {
+                                                const double* __restrict__ coef, const Tick tk) {
     __shared__ double J;
     const int sid = blk_src[blockIdx.x];
     const SrcDev<T>& S = src[sid];
@@ -1039,10 +1063,12 @@ __global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ sr
     if (threadIdx.x == 0) J = source_current(S.p, step);
     __syncthreads();
     const long e = blk_first[blockIdx.x] + threadIdx.x;
-    if (e >= blk_end[blockIdx.x] || J == 0.0) return;
-    T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
-    const long q = idx[e];
-    F[q] = (T)((double)F[q] - coef[e] * J);
+    if (e < blk_end[blockIdx.x] && J != 0.0) {
+        T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
+        const long q = idx[e];
+        F[q] = (T)((double)F[q] - coef[e] * J);
+    }
+    tick_tail(tk);
 }
 
 // ---- point monitors: trilinear gather, FP64 result --------------------------------------
@@ -1098,12 +1124,17 @@ struct DftDev {
     double2* cmp;        // compensation terms, same layout
 };
 
+// accumulation of one (cell, frequency) of region ri
+template <typename T>
+__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int ri, int f, long g, const double* ph,
+                                         int blk_reg);
+
 // One thread per (cell, frequency): blockIdx.y is the frequency index.  The phase factors depend
 // only on the step and the frequency, so one thread per block evaluates them (FP64 sincos) and
 // shares them through LDS; a block's cells all lie in one region when regions start on block
 // boundaries, otherwise each thread evaluates its own.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total) {
+__global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total, const Tick tk) {
     __shared__ double ph[4];  // cos, sin at the E and H time levels for this block's region
     __shared__ int blk_reg;
     const long g0 = (long)blockIdx.x * blockDim.x;
@@ -1130,8 +1161,13 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
         }
     }
     __syncthreads();
-    if (g >= total) return;
-    const int ri = region_of(g);
+    if (g < total) dft_cell(regs, region_of(g), f, g, ph, blk_reg);
+    tick_tail(tk);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, const int ri, const int f, const long g,
+                                         const double* ph, const int blk_reg) {
     const DftDev<T>& a = regs[ri];
     if (f >= a.nf) return;
     const long cell = g - a.first;
