@@ -51,9 +51,17 @@ int sjfh_voxelize(const char* scene_json, double ambient_eps, unsigned smooth_n,
                   double um_scale, double resolution, int n, double dx, unsigned* words, double* mats,
                   int cap_mat, int* n_mat);
 
+/* Runs the .geom scene script geom_path (the reference's scene language: scene::read_file,
+ * src/cgs.cpp:510-570) in the context context_from_settings builds (src/disp.cpp:28-50) from
+ * the given settings values and the user_opts statements ("a=1;b=2"), and writes the scene in
+ * this layer's JSON form (roots, sources, monitor_clusters, context_numbers) to out.
+ * *parse_error = the script's parse_ercode (0 = success; on an error no roots are built). */
+int sjfh_geom_scene_json(const char* geom_path, double pml_thickness, double len, double um_scale, const char* out_dir,
+                         const char* user_opts, char* out, size_t cap, int* parse_error);
+
 /* Whole driver run (BoundGeom): settings from conf_path + argv as sjfh_settings_json with
- * mode 2, scene from scene_json, then run() and save_field_times() into the
- * settings' out_dir.  precision: 4 or 8; complex_fields: two engines (Re, Im).  summary =
+ * mode 2, scene from scene_json (a scene JSON, or a .geom script run with those settings),
+ * then run() and save_field_times() into the settings' out_dir.  precision: 4 or 8; complex_fields: two engines (Re, Im).  summary =
  * {grid_num, dt, ttot, n_t_pts, n_monitors, n_clusters} (optional). */
 int sjfh_run(const char* conf_path, int argc, const char* const* argv, const char* scene_json, int precision,
              int complex_fields, int device, double* summary);

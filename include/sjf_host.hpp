@@ -157,7 +157,22 @@ struct Scene {
      * monitor_clusters, context_numbers). */
     static Scene from_json_file(const std::string& path);
     static Scene from_json_text(const std::string& text);
+    /* Runs a .geom scene script (src/cgs_read.cpp, src/cgs_data.cpp; scene::read_file,
+     * src/cgs.cpp:510-570) in a context holding the settings' values (pi, pml_thickness,
+     * sim_length, length, l_per_um, out_dir, um_to_l, fs_to_t: src/disp.cpp:28-50) and the
+     * statements of s.user_opts.  parse_error receives the reference's parse_ercode value
+     * (0 = success); on an error the statements before it still count (sources, monitors,
+     * numbers) but no roots are built, as in the reference. */
+    static Scene from_geom_file(const std::string& path, const Settings& s, int* parse_error = nullptr);
+    static Scene from_geom_text(const std::string& text, const Settings& s, int* parse_error = nullptr);
+    /* This layer's JSON form (the keys from_json_text reads). */
+    std::string to_json() const;
 };
+
+/* A scene file: path ending in ".json" -> Scene::from_json_file, otherwise a .geom script run
+ * with s (Scene::from_geom_file).  A script error throws, as bound_geom stops on one
+ * ("Scene parsing failed", src/disp.cpp:562-565). */
+Scene load_scene(const std::string& path, const Settings& s);
 
 // ---- materials (src/disp.cpp) ----------------------------------------------------------------
 

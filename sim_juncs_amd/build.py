@@ -37,7 +37,8 @@ def build_engine(verbose: bool = False) -> str:
 
 
 HOST_SRC = [os.path.join(HERE, "host", f) for f in
-            ("csg.cpp", "settings.cpp", "materials.cpp", "fft.cpp", "h5out.cpp", "bound_geom.cpp", "capi.cpp")]
+            ("csg.cpp", "geom.cpp", "settings.cpp", "materials.cpp", "fft.cpp", "h5out.cpp", "bound_geom.cpp",
+             "capi.cpp")]
 HOST_OUT = os.path.join(HERE, "libsjf_host.so")
 GEOM_OUT = os.path.join(HERE, "sjf_geom")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fopenmp", "-fPIC", "-Wall", "-Wno-unused-result"]

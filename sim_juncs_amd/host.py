@@ -3,7 +3,8 @@
 The host layer restates the reference driver around the engine: configuration parsing
 (src/argparse.h), the CSG scene and its material maps (src/cgs.cpp, src/disp.cpp), source
 descriptions, the monitor-series transform (src/data_utils.cpp) and field_samples.h5
-(src/disp.cpp:758-923).  Scenes are JSON files (see sjfh::Scene::from_json_file).
+(src/disp.cpp:758-923).  Scenes are JSON files (see sjfh::Scene::from_json_file) or .geom
+scripts in the reference's scene language (geom_scene; the driver takes either).
 """
 from __future__ import annotations
 
@@ -44,6 +45,8 @@ def lib():
             "sjfh_write_field_samples": (C.c_int, [cp, cp, C.c_double, C.c_double, C.c_uint, C.c_uint, dp, C.c_long]),
             "sjfh_h5_read": (C.c_int, [cp, cp, cpp, C.c_int, dp, C.c_size_t, szp]),
             "sjfh_h5_list": (C.c_int, [cp, cp, C.c_char_p, C.c_size_t]),
+            "sjfh_geom_scene_json": (C.c_int, [cp, C.c_double, C.c_double, C.c_double, cp, cp, C.c_char_p,
+                                               C.c_size_t, ip]),
         }
         for name, (res, args) in protos.items():
             f = getattr(L, name)
@@ -83,6 +86,17 @@ def settings(conf_path, argv=(), mode=2) -> dict:
     _chk(lib().sjfh_settings_json(_b(conf_path), len(args), _argv(args), int(mode), buf, len(buf)),
          "sjfh_settings_json")
     return json.loads(buf.value.decode())
+
+
+def geom_scene(geom_path, pml_thickness, length, um_scale=1.0, out_dir="", user_opts=""):
+    """Run a .geom scene script with these settings values; returns (scene dict in the JSON
+    form: roots, sources, monitor_clusters, context_numbers; parse_error code, 0 = success)."""
+    cap = 1 << 24
+    buf = C.create_string_buffer(cap)
+    er = C.c_int()
+    _chk(lib().sjfh_geom_scene_json(_b(geom_path), float(pml_thickness), float(length), float(um_scale), _b(out_dir),
+                                    _b(user_opts), buf, cap, C.byref(er)), "sjfh_geom_scene_json")
+    return json.loads(buf.value.decode()), er.value
 
 
 def scene_in(scene_json, pts) -> np.ndarray:

@@ -249,7 +249,7 @@ int sjfh_run(const char* conf_path, int argc, const char* const* argv, const cha
             g_err = "settings could not be read";
             return -2;
         }
-        const Scene sc = Scene::from_json_file(scene_json);
+        const Scene sc = load_scene(scene_json, s);
         RunOptions opt;
         opt.precision = precision;
         opt.complex_fields = complex_fields != 0;
@@ -266,6 +266,24 @@ int sjfh_run(const char* conf_path, int argc, const char* const* argv, const cha
             summary[5] = (double)g.get_n_monitor_clusters();
         }
         return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int sjfh_geom_scene_json(const char* geom_path, double pml_thickness, double len, double um_scale, const char* out_dir,
+                         const char* user_opts, char* out, size_t cap, int* parse_error) {
+    try {
+        Settings s;
+        s.pml_thickness = pml_thickness;
+        s.len = len;
+        s.um_scale = um_scale;
+        s.out_dir = out_dir ? out_dir : "";
+        s.user_opts = user_opts ? user_opts : "";
+        int er = 0;
+        const Scene sc = Scene::from_geom_file(geom_path, s, &er);
+        if (parse_error) *parse_error = er;
+        return copy_out(sc.to_json(), out, cap);
     } catch (const std::exception& e) {
         return fail(e);
     }

@@ -1,4 +1,5 @@
 // csg.cpp -- CSG object tree, point inclusion and list assembly (src/cgs.cpp), scene files.
+#include <cstdio>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -249,6 +250,7 @@ void fill_from_list(Node* self, const std::vector<Shape>& list, std::vector<Mat3
     for (size_t i = 0; i < n; ++i) {
         const Shape& s = list[i];
         const bool is_last = (i == n - 1);
+        if (s.type.empty()) continue;  // a list entry that is no object: skipped, but counted
         std::unique_ptr<Node> obj = make_object(s, invert, stack);
         if (!obj) {
             if (s.type == "Complement") {
@@ -385,6 +387,106 @@ Scene Scene::from_json_text(const std::string& text) {
     if (d.has("context_numbers"))
         for (const auto& kv : d.at("context_numbers").a) sc.context_numbers.emplace_back(kv[0].s, kv[1].num());
     return sc;
+}
+
+namespace {
+
+std::string jnum(double x) {
+    char b[40];
+    std::snprintf(b, sizeof b, "%.17g", x);
+    return b;
+}
+
+std::string jstr(const std::string& s) {
+    std::string o = "\"";
+    for (char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        if ((unsigned char)c < 0x20) {
+            o += ' ';
+            continue;
+        }
+        o += c;
+    }
+    return o + "\"";
+}
+
+std::string jvec(const Vec3& v) { return "[" + jnum(v.x) + "," + jnum(v.y) + "," + jnum(v.z) + "]"; }
+
+std::string jvalue(const Value& v) {
+    switch (v.type) {
+        case Value::Num: return jnum(v.x);
+        case Value::Str: return jstr(v.s);
+        case Value::List: {
+            std::string o = "[";
+            for (size_t i = 0; i < v.l.size(); ++i) o += (i ? "," : "") + jvalue(v.l[i]);
+            return o + "]";
+        }
+        default: return "null";
+    }
+}
+
+std::string jnode(const Node* n) {
+    if (!n) return "null";
+    static const char* kinds[5] = {"composite", "sphere", "box", "plane", "cylinder"};
+    std::string o = "{\"kind\":\"" + std::string(kinds[(int)n->kind]) + "\",\"invert\":" + std::to_string(n->invert) +
+                    ",\"trans\":[";
+    for (int q = 0; q < 9; ++q) o += (q ? "," : "") + jnum(n->trans.el[q]);
+    o += "]";
+    switch (n->kind) {
+        case Kind::Composite: {
+            const char* c = n->combine == Combine::Union ? "union" : n->combine == Combine::Intersect ? "intersect"
+                          : n->combine == Combine::Difference ? "difference" : "noop";
+            o += ",\"combine\":\"" + std::string(c) + "\",\"children\":[" + jnode(n->child[0].get()) + "," +
+                 jnode(n->child[1].get()) + "]";
+            break;
+        }
+        case Kind::Sphere: o += ",\"center\":" + jvec(n->center) + ",\"radius\":" + jnum(n->radius); break;
+        case Kind::Box: o += ",\"center\":" + jvec(n->center) + ",\"offset\":" + jvec(n->offset); break;
+        case Kind::Plane: o += ",\"normal\":" + jvec(n->normal) + ",\"offset\":" + jnum(n->plane_offset); break;
+        case Kind::Cylinder:
+            o += ",\"center\":" + jvec(n->center) + ",\"height\":" + jnum(n->height) + ",\"r1_sq\":" + jnum(n->r1_sq) +
+                 ",\"r1_sq_x_h\":" + jnum(n->r1_sq_x_h) + ",\"r2_sq\":" + jnum(n->r2_sq);
+            break;
+    }
+    return o + "}";
+}
+
+}  // namespace
+
+std::string Scene::to_json() const {
+    std::string o = "{\"roots\":[";
+    for (size_t r = 0; r < roots.size(); ++r) {
+        o += std::string(r ? "," : "") + "{\"metadata\":{";
+        bool first = true;
+        for (const auto& kv : roots[r].metadata) {
+            o += (first ? "" : ",") + jstr(kv.first) + ":" + jvalue(kv.second);
+            first = false;
+        }
+        o += "},\"tree\":" + jnode(roots[r].tree.get()) + "}";
+    }
+    o += "],\"sources\":[";
+    static const char* keys[9] = {"component", "wavelength", "amplitude", "width", "phase",
+                                  "cutoff", "start_time", "end_time", "slowness"};
+    for (size_t i = 0; i < sources.size(); ++i) {
+        const SourceSpec& sp = sources[i];
+        const double* val[9] = {&sp.component, &sp.wavelength, &sp.amplitude, &sp.width, &sp.phase,
+                                &sp.cutoff, &sp.start_time, &sp.end_time, &sp.slowness};
+        o += std::string(i ? "," : "") + "{\"type\":" + jstr(sp.gaussian ? "Gaussian_source" : "CW_source");
+        for (int q = 0; q < 9; ++q)
+            if (sp.has[q]) o += "," + jstr(keys[q]) + ":" + jnum(*val[q]);
+        if (sp.has_box) o += ",\"box\":[" + jvec(sp.box[0]) + "," + jvec(sp.box[1]) + "]";
+        o += "}";
+    }
+    o += "],\"monitor_clusters\":[";
+    for (size_t k = 0; k < monitor_clusters.size(); ++k) {
+        o += std::string(k ? "," : "") + "[";
+        for (size_t p = 0; p < monitor_clusters[k].size(); ++p) o += (p ? "," : "") + jvec(monitor_clusters[k][p]);
+        o += "]";
+    }
+    o += "],\"context_numbers\":[";
+    for (size_t k = 0; k < context_numbers.size(); ++k)
+        o += std::string(k ? "," : "") + "[" + jstr(context_numbers[k].first) + "," + jnum(context_numbers[k].second) + "]";
+    return o + "]}";
 }
 
 Scene Scene::from_json_file(const std::string& path) {

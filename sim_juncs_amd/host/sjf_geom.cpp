@@ -1,11 +1,12 @@
 // sjf_geom.cpp -- command-line driver with the reference's sim_geom flags (src/main.cpp).
 //
-//   sjf_geom [--conf-file params.conf] [--geom-file scene.json] [--out-dir DIR] [--grid-res R]
+//   sjf_geom [--conf-file params.conf] [--geom-file junc.geom] [--out-dir DIR] [--grid-res R]
 //            [--grid-num N] [--length L] [--eps1 E] [--save-span S] [-v V] [--opts "a=1"]
 //            [--fp32] [--real] [--device D]
 //
-// The geometry file is a scene JSON (roots, sources, monitor clusters, numeric context
-// entries); the .geom text interpreter is not part of this driver.  Steps, as main():
+// The geometry file is a .geom scene script (run with the settings and --opts, as the
+// reference does) or a scene JSON (roots, sources, monitor clusters, numeric context entries;
+// file name ending in .json).  Steps, as main():
 // parse_args, parse_conf_file (params.conf unless --conf-file), correct_defaults, build the
 // simulation, run, save field_samples.h5, printing the same timing lines.
 #include <chrono>
@@ -40,7 +41,7 @@ int main(int argc, char** argv) {
     correct_defaults(&args);
     try {
         const auto start = std::chrono::steady_clock::now();
-        const Scene scene = Scene::from_json_file(args.geom_fname);
+        const Scene scene = load_scene(args.geom_fname, args);
         BoundGeom geom(args, scene, opt);
         const auto end_init = std::chrono::steady_clock::now();
         std::cout << "initialization completed in "

@@ -47,6 +47,10 @@ SCENES = [
     ("sidetie", "junctions/Au_SiO2_sidetie", "params.conf", ["--opts", RUN_OPTS, "--grid-res", "12", "--npts", "2500"]),
     ("bowtie_inf", "junctions/Au_SiO2_bowtie", "params.conf",
      ["--opts", RUN_OPTS.replace("inf_thick=0", "inf_thick=1"), "--grid-res", "12", "--npts", "3000"]),
+    # a script of this repository (geom/quirks.geom) that exercises the evaluation rules of the
+    # scene language; "@" marks a directory next to this file
+    ("quirks", "@geom", "quirks.conf", ["--opts", "extra=4;twice=extra*2", "--npts", "800"]),
+    ("small_run", "@geom", "small_run.conf", ["--npts", "500"]),
 ]
 
 CONF_CASES = [
@@ -71,18 +75,25 @@ def run(harness, cwd, args):
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--reference", default="/root/reference")
+    p.add_argument("--only", default=None, help="write only the scene fixture of this name")
     a = p.parse_args()
     harness = build_ref.build(a.reference)
     if harness is None:
         raise SystemExit("reference tree not found")
     with tempfile.TemporaryDirectory() as tmp:
         for name, cwd, conf, extra in SCENES:
+            if a.only and name != a.only:
+                continue
             out = os.path.join(tmp, f"{name}.json")
-            run(harness, os.path.join(a.reference, cwd), ["scene", conf, out, *extra])
+            where = os.path.join(HERE, cwd[1:]) if cwd.startswith("@") else os.path.join(a.reference, cwd)
+            run(harness, where, ["scene", conf, out, *extra])
             d = json.load(open(out))
             d["case"] = {"conf": conf, "args": extra, "dir": cwd}
             with open(os.path.join(HERE, f"scene_{name}.json"), "w") as f:
                 json.dump(d, f, separators=(",", ":"))
+        if a.only:
+            print("wrote", a.only)
+            return
         confs = {}
         for name, cwd, conf, extra in CONF_CASES:
             out = os.path.join(tmp, f"{name}.json")

@@ -104,6 +104,26 @@ def test_driver_matches_oracle_with_materials(tmp_path):
             assert np.abs(series[pt][:, part] - ref[part][:, pt]).max() <= 1e-10 * scale, (pt, part)
 
 
+def test_driver_runs_geom_script_like_its_scene_json(tmp_path):
+    """The driver given the .geom script itself (run with the settings of the conf file) writes
+    the same field_samples.h5 series as given the scene the reference parser made of it."""
+    conf = os.path.join(G, "geom", "small_run.conf")
+    out = {}
+    for kind, scene in (("geom", os.path.join(G, "geom", "small_run.geom")),
+                        ("json", os.path.join(G, "scene_small_run.json"))):
+        d = tmp_path / kind
+        d.mkdir()
+        argv = ["--conf-file", conf, "--grid-num", "17", "--out-dir", str(d)]
+        print(f"[case] driver small_run.{kind} 17^3 fp32 complex", flush=True)
+        s = host.run(conf, scene, argv, precision=4, complex_fields=True)
+        assert s["grid_num"] == 17 and s["n_monitors"] == 3 and s["n_clusters"] == 1
+        h5 = str(d / "field_samples.h5")
+        out[kind] = [host.h5_read(h5, f"cluster_0/point_{p}/time", ["Re", "Im"]) for p in range(3)]
+    for a, b in zip(out["geom"], out["json"]):
+        assert np.array_equal(a, b)
+    assert max(np.abs(a).max() for a in out["geom"]) > 0
+
+
 def test_geom_binary_bowtie_junction(tmp_path):
     exe = host.GEOM_BIN
     assert os.path.exists(exe), "sjf_geom is not built"
