@@ -180,7 +180,8 @@ def adjoint(a, variants=False):
             log(f"reverse kernel {rev[1] * 1e3:.1f} us over {rev[3]} workgroups of {rev[2]} threads")
         eng.close()
     s0 = scns[0]
-    rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], s0.name)
+    workload = ("variants_" if variants else "") + s0.name
+    rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], workload)
     cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or 60)
     if cpu:
         # the adjoint costs a forward and a reverse sweep per step (the design fields of every
@@ -188,7 +189,7 @@ def adjoint(a, variants=False):
         cpu["value"] /= 2.0
         cpu["sample"] += "; divided by 2 for the forward and reverse sweeps"
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
-                config={"workload": ("variants_" if variants else "") + s0.name, "cells": cells,
+                config={"workload": workload, "cells": cells,
                         "precision": "fp32", "checkpoint_every": 0,
                         "note": "value counts forward steps per second of a full gradient computation"},
                 roofline=rl, cpu=cpu, dtype="f32")

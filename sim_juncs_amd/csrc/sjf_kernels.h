@@ -961,28 +961,41 @@ __global__ void k_tick(long* const* counters, int n) {
     if (i < n) counters[i][0] += 1;
 }
 
-// The same, folded into the last kernel of a step (one launch fewer per step): each workgroup
-// signs off on `done` after all of its threads have finished (and so have read the counters);
-// the last one to sign off advances every counter and re-arms `done` for the next step.  No
-// fence is needed: no workgroup reads what another writes within the kernel (the counters
-// were read before signing off, and the kernel boundary publishes the increment).
+// The same, folded into the DFT kernel when a step has one (one launch fewer): each workgroup
+// signs off after all of its threads have finished (and so have read the counters); the last
+// one to sign off advances every counter.  Sign-offs go to eight group words (workgroup index
+// mod 8) and the last of each group signs off on a top word, so no word takes more than an
+// eighth of the atomics; the words re-arm themselves for the next step.  No fence is needed:
+// no workgroup reads what another writes within the kernel (the counters were read before
+// signing off, and the kernel boundary publishes the increment).
 // `done` == nullptr: the step ends with k_tick instead.
 struct Tick {
     long* const* counters;
     int n;
-    unsigned* done;
+    unsigned* done;  // [0]: top word, [1 + g]: group g
 };
 
 __device__ __forceinline__ void tick_tail(const Tick& tk) {
     if (!tk.done) return;
+    __shared__ int last;
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
-        if (atomicAdd(tk.done, 1u) == nblk - 1) {
-            for (int i = 0; i < tk.n; ++i) tk.counters[i][0] += 1;
-            atomicExch(tk.done, 0u);
+        const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const unsigned G = nblk < 8u ? nblk : 8u;
+        const unsigned g = b % G;
+        const unsigned members = (nblk - g + G - 1) / G;  // workgroups of group g
+        bool fin = atomicAdd(tk.done + 1 + g, 1u) == members - 1;
+        if (fin) {
+            atomicExch(tk.done + 1 + g, 0u);
+            fin = atomicAdd(tk.done, 1u) == G - 1;
+            if (fin) atomicExch(tk.done, 0u);
         }
+        last = fin ? 1 : 0;
     }
+    __syncthreads();
+    if (last)
+        for (int i = threadIdx.x; i < tk.n; i += blockDim.x) tk.counters[i][0] += 1;
 }
 
 // ---- source time functions (FP64) ---------------------------------------------------------
@@ -1055,7 +1068,7 @@ __global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ sr
                                                 const long* __restrict__ blk_first,
                                                 const long* __restrict__ blk_end,
                                                 const long* __restrict__ idx,
-                                                const double* __restrict__ coef, const Tick tk) {
+                                                const double* __restrict__ coef) {
     __shared__ double J;
     const int sid = blk_src[blockIdx.x];
     const SrcDev<T>& S = src[sid];
@@ -1063,12 +1076,10 @@ __global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ sr
     if (threadIdx.x == 0) J = source_current(S.p, step);
     __syncthreads();
     const long e = blk_first[blockIdx.x] + threadIdx.x;
-    if (e < blk_end[blockIdx.x] && J != 0.0) {
-        T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
-        const long q = idx[e];
-        F[q] = (T)((double)F[q] - coef[e] * J);
-    }
-    tick_tail(tk);
+    if (e >= blk_end[blockIdx.x] || J == 0.0) return;
+    T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
+    const long q = idx[e];
+    F[q] = (T)((double)F[q] - coef[e] * J);
 }
 
 // ---- point monitors: trilinear gather, FP64 result --------------------------------------
