@@ -119,7 +119,7 @@ def single(a, scn, tag):
     eng.close()
     cpu = None
     if not a.no_cpu:
-        n_cpu = a.cpu_steps or max(3, min(steps, int(7e9 / scn.cells)))
+        n_cpu = a.cpu_steps or max(3, int(7e9 / scn.cells))  # about 7e9 cell-updates (~10 s)
         cpu = cpu_baseline([scn], n_cpu)
     return dict(value=scn.cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": scn.name, "cells": scn.cells, "precision": scn.precision,
@@ -148,7 +148,8 @@ def sweep(a):
     b.close()
     for e in engs:
         e.close()
-    cpu = None if a.no_cpu else cpu_baseline(scns[:4], a.cpu_steps or 20)
+    sample = scns[:4]
+    cpu = None if a.no_cpu else cpu_baseline(sample, a.cpu_steps or max(3, int(7e9 / sum(s.cells for s in sample))))
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
                         "launch": "one fused launch per step over all instances, hipGraph replay"},
@@ -182,7 +183,7 @@ def adjoint(a, variants=False):
     s0 = scns[0]
     workload = ("variants_" if variants else "") + s0.name
     rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], workload)
-    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or 60)
+    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or max(3, int(5e9 / s0.cells)))
     if cpu:
         # the adjoint costs a forward and a reverse sweep per step (the design fields of every
         # step are recorded, as a CPU implementation could record them too)

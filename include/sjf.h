@@ -197,7 +197,8 @@ int sjf_step_bytes(const sjf_engine* e, double* bytes);
  * workgroup), x-chunk length, threads per workgroup, workgroups. */
 int sjf_launch_info(const sjf_engine* e, int* form, int* rows, int* xchunk, int* threads, int* groups);
 /* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x). The
- * x-chunk also applies to the reverse step of sjf_adjoint_run (one y row per workgroup). */
+ * x-chunk of a REGISTER_ROWS tile also applies to the reverse step of sjf_adjoint_run (one y
+ * row per workgroup); with a THREAD_ROWS tile the reverse step keeps its own default. */
 int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
 
 /* ---- adjoint -----------------------------------------------------------------------------

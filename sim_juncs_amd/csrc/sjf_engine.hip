@@ -1023,8 +1023,10 @@ int set_step(sjf_engine* e, long n) {
 // x-chunk of the fused reverse step: the one sjf_set_tile fixed, else about 16384 workgroups
 // (one y row each); measured on MI355X, 4-6 planes beat 8 and more (256x256x128: 1.019 ms per
 // gradient step at 4 planes, 1.041 at 8, 1.13 at 16)
+// x-chunk of the reverse step (one y row per workgroup): that of a register-rows tile set on the
+// engine (the same tiling), else enough chunks for about 16384 workgroups
 int adj_xchunk(const sjf_engine* e) {
-    if (e->tile_set) return std::min(e->xc, e->g.nx);
+    if (e->tile_set && e->form == SJF_TILE_REGISTER_ROWS) return std::min(e->xc, e->g.nx);
     const long target = 16384;
     long nxc = std::max(1L, (target + e->g.ny - 1) / e->g.ny);
     nxc = std::min<long>(nxc, e->g.nx);

@@ -129,6 +129,14 @@ def test_adjoint_kernel_info_reports_the_reverse_launch():
     assert threads == -(-nz // 64) * 64
     assert groups == ny * -(-nx // 4)
     eng.close()
+    # a row-of-threads tile leaves the reverse step its own x-chunks (about 16384 workgroups)
+    eng = scn.make(Engine)
+    eng.set_tile("thread_rows", 1, 4)
+    eng.adjoint_run(scn.steps, 0, gap, eps.size)
+    _, _, groups = eng.adjoint_kernel_info()
+    nxc = min(-(-16384 // ny), nx)
+    assert groups == ny * -(-nx // -(-nx // nxc))
+    eng.close()
 
 
 def test_adjoint_fp32_close_to_fp64():
