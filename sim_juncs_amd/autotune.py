@@ -31,7 +31,7 @@ def candidates(shape, xchunks=XCHUNKS):
         if rows == 1 or rows <= ny:
             out += [("register_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
     max_rows = 1024 // nz - 1
-    for rows in sorted({r for r in (1, 2, 3, 5, 7, 11, 15) if r <= max_rows} | ({max_rows} if max_rows >= 1 else set())):
+    for rows in sorted({r for r in (1, 2, 3, 4, 5, 7, 11, 15) if r <= max_rows} | ({max_rows} if max_rows >= 1 else set())):
         if rows > ny and rows != 1:
             continue
         out += [("thread_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
