@@ -103,3 +103,21 @@ def test_reference_scripts_match_reference_parser(name):
 def test_missing_script_raises():
     with pytest.raises(host.HostError):
         host.geom_scene("/nonexistent/scene.geom", 1.0, 2.0)
+
+
+def test_error_stops_the_script_and_drops_the_roots(tmp_path):
+    """An unknown function is error 6 (E_BAD_TYPE, as the reference's sidetie script shows):
+    what ran before it stays (numbers, sources), nothing after it runs and no roots are built."""
+    geom = tmp_path / "broken.geom"
+    geom.write_text(
+        "a = 2\n"
+        "Gaussian_source(\"Ey\", 0.8, 1.0, 2.0, 0.0, Box([0,0,1], [length,length,1]))\n"
+        "Composite(eps = 3.0, [Box([0,0,0], [1,1,1])])\n"
+        "b = not_a_function(1)\n"
+        "c = 3\n")
+    scene, er = host.geom_scene(str(geom), 1.0, 2.0)
+    assert er == 6
+    names = [n for n, _ in scene["context_numbers"]]
+    assert "a" in names and "b" not in names and "c" not in names
+    assert len(scene["sources"]) == 1 and scene["sources"][0]["component"] == 1
+    assert scene["roots"] == []
