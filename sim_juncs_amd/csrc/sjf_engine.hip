@@ -5,7 +5,8 @@
 // set: one engine, or a batch of engines, compiled into small device tables so that each
 // time step is a fixed sequence of launches on one stream,
 //   [k_monitor when step % save_span == 0] -> k_step -> k_source -> k_dft (or k_tick),
-// the last of which advances the step counters, whatever the number of instances.  The per-step scalars (step index, pulse value, DFT
+// the last of which advances the step counters, whatever the number of instances; source and
+// DFT share one launch (k_source_dft) when no DFT region holds a source cell.  The per-step scalars (step index, pulse value, DFT
 // phase) are computed on the device from each engine's step counter, so a run of steps is
 // captured once in a hipGraph and replayed.  This is the sequence bound_geom::run performs
 // around fields.step() (reference src/disp.cpp:719-741).
@@ -334,6 +335,7 @@ struct LaunchSet {
     int ndft = 0;
     long dft_total = 0;
     int dft_maxnf = 0;  // most frequencies of any region (the launch's second dimension)
+    bool src_dft_one_launch = false;  // no DFT region holds a source cell: one launch for both
     // step counters, and the sign-off word of the last kernel of a step that advances them
     long** d_counters = nullptr;
     unsigned* d_done = nullptr;
@@ -512,6 +514,19 @@ int build_set_t(LaunchSet* s) {
     }
     s->ndft = (int)regs.size();
     s->dft_total = tot;
+    // sources and DFT can share a launch when no DFT region of an engine holds one of its
+    // source cells
+    s->src_dft_one_launch = s->nsrcblk > 0 && s->ndft > 0;
+    for (sjf_engine* e : s->eng) {
+        const long nyz = (long)e->g.ny * e->g.nz;
+        for (const Source& src : e->src)
+            for (const long q : src.idx) {
+                const int i = (int)(q / nyz), j = (int)((q % nyz) / e->g.nz), k = (int)(q % e->g.nz);
+                for (const Dft& d : e->dft)
+                    if (i >= d.lo[0] && i < d.hi[0] && j >= d.lo[1] && j < d.hi[1] && k >= d.lo[2] && k < d.hi[2])
+                        s->src_dft_one_launch = false;
+            }
+    }
     s->dft_maxnf = 0;
     for (const auto& r : regs) s->dft_maxnf = std::max(s->dft_maxnf, r.nf);
     if (s->ndft) {
@@ -604,17 +619,24 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
     // 256 source workgroups of 256 threads, 10.9 us against 4.7 + 4.7 us.)
     const int n = (int)s->eng.size();
     const bool dft = s->ndft && !bare;
+    const sjf::SrcTables<T> st{(const sjf::SrcDev<T>*)s->d_src, s->d_blk_src, s->d_blk_first, s->d_blk_end,
+                               s->d_src_idx, s->d_src_coef};
+    const sjf::Tick tick{s->d_counters, n, s->d_done};
+    const int dftx = (int)((s->dft_total + 255) / 256);
+    if (dft && s->src_dft_one_launch) {
+        hipLaunchKernelGGL((sjf::k_source_dft<T>), dim3((unsigned)(s->nsrcblk + dftx * s->dft_maxnf)), dim3(256), 0,
+                           s->stream, st, s->nsrcblk, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total, dftx,
+                           tick);
+        SETCHK(s, hipGetLastError());
+        return SJF_OK;
+    }
     if (s->nsrcblk) {
-        hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream,
-                           (const sjf::SrcDev<T>*)s->d_src, s->d_blk_src, s->d_blk_first, s->d_blk_end,
-                           s->d_src_idx, s->d_src_coef);
+        hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream, st);
         SETCHK(s, hipGetLastError());
     }
     if (dft) {
-        const sjf::Tick tick{s->d_counters, n, s->d_done};
-        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)((s->dft_total + 255) / 256), (unsigned)s->dft_maxnf),
-                           dim3(256), 0,
-                           s->stream, (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total, tick);
+        hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)dftx, (unsigned)s->dft_maxnf), dim3(256), 0, s->stream,
+                           (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total, tick);
         SETCHK(s, hipGetLastError());
     } else {
         hipLaunchKernelGGL(sjf::k_tick, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->d_counters, n);

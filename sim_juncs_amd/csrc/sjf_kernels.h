@@ -1062,24 +1062,37 @@ struct SrcDev {
     const long* d_step;  // the owning engine's step counter
 };
 
+// the tables of the source launch: per workgroup its source and range of entries; per entry
+// the layout index and FP64 coefficient
 template <typename T>
-__global__ void __launch_bounds__(256) k_source(const SrcDev<T>* __restrict__ src,
-                                                const int* __restrict__ blk_src,
-                                                const long* __restrict__ blk_first,
-                                                const long* __restrict__ blk_end,
-                                                const long* __restrict__ idx,
-                                                const double* __restrict__ coef) {
+struct SrcTables {
+    const SrcDev<T>* src;
+    const int* blk_src;
+    const long* blk_first;
+    const long* blk_end;
+    const long* idx;
+    const double* coef;
+};
+
+// the cells of source workgroup blk
+template <typename T>
+__device__ __forceinline__ void source_block(const SrcTables<T>& t, const int blk) {
     __shared__ double J;
-    const int sid = blk_src[blockIdx.x];
-    const SrcDev<T>& S = src[sid];
+    const int sid = t.blk_src[blk];
+    const SrcDev<T>& S = t.src[sid];
     const long step = *S.d_step;
     if (threadIdx.x == 0) J = source_current(S.p, step);
     __syncthreads();
-    const long e = blk_first[blockIdx.x] + threadIdx.x;
-    if (e >= blk_end[blockIdx.x] || J == 0.0) return;
+    const long e = t.blk_first[blk] + threadIdx.x;
+    if (e >= t.blk_end[blk] || J == 0.0) return;
     T* F = S.F[(step + 1) & 1];  // the step just completed wrote the other buffer
-    const long q = idx[e];
-    F[q] = (T)((double)F[q] - coef[e] * J);
+    const long q = t.idx[e];
+    F[q] = (T)((double)F[q] - t.coef[e] * J);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_source(const SrcTables<T> t) {
+    source_block(t, blockIdx.x);
 }
 
 // ---- point monitors: trilinear gather, FP64 result --------------------------------------
@@ -1145,10 +1158,11 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int
 // shares them through LDS; a block's cells all lie in one region when regions start on block
 // boundaries, otherwise each thread evaluates its own.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total, const Tick tk) {
+__device__ __forceinline__ void dft_block(const DftDev<T>* __restrict__ regs, const int nreg, const long total,
+                                          const int bx, const int f) {
     __shared__ double ph[4];  // cos, sin at the E and H time levels for this block's region
     __shared__ int blk_reg;
-    const long g0 = (long)blockIdx.x * blockDim.x;
+    const long g0 = (long)bx * blockDim.x;
     const long g = g0 + threadIdx.x;
     auto region_of = [&](long x) {
         int lo = 0, hi = nreg - 1;
@@ -1159,7 +1173,6 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
         }
         return lo;
     };
-    const int f = blockIdx.y;
     if (threadIdx.x == 0) {
         const int r0 = region_of(g0);
         blk_reg = r0;
@@ -1173,6 +1186,27 @@ __global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs,
     }
     __syncthreads();
     if (g < total) dft_cell(regs, region_of(g), f, g, ph, blk_reg);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dft(const DftDev<T>* __restrict__ regs, int nreg, long total, const Tick tk) {
+    dft_block(regs, nreg, total, blockIdx.x, blockIdx.y);
+    tick_tail(tk);
+}
+
+// Source injection and DFT in one launch, for steps whose DFT regions hold no source cell (so
+// neither part reads what the other writes): workgroups [0, nsrc) inject, the rest accumulate
+// (dftx workgroups per frequency); the last workgroup advances the step counters.
+template <typename T>
+__global__ void __launch_bounds__(256) k_source_dft(const SrcTables<T> t, int nsrc, const DftDev<T>* __restrict__ regs,
+                                                    int nreg, long total, int dftx, const Tick tk) {
+    const int b = blockIdx.x;
+    if (b < nsrc) {
+        source_block(t, b);
+    } else {
+        const int d = b - nsrc;
+        dft_block(regs, nreg, total, d % dftx, d / dftx);
+    }
     tick_tail(tk);
 }
 

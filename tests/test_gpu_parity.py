@@ -77,6 +77,23 @@ def test_plain_box_fp64_matches_oracle(n):
     eng.close()
 
 
+@pytest.mark.parametrize("through_source", [False, True])
+def test_dft_region_at_and_away_from_the_source_plane_fp64(through_source):
+    """Source injection and DFT accumulation share one launch only when no DFT region holds a
+    source cell; a region through the source plane keeps them in two launches (the DFT must
+    see E after the injection).  Both match the oracle."""
+    scn = early_pulse(scenario.plain_box(n=24, steps=90, precision="fp64", pml=4))
+    z = scn.meta["src_z"] if through_source else scn.meta["src_z"] + 3
+    scn.dfts = list(scn.dfts) + [((0, 0, z), (24, 24, z + 1), ("Ex", "Ey", "Hx"), np.array([0.8, 1.3]))]
+    eng, orc = run_pair(scn)
+    assert max(field_errors(eng, orc)) < FP64_TOL
+    for i in range(len(scn.dfts)):
+        d_e, d_o = eng.read_dft(i), orc.read_dft(i)
+        assert np.abs(d_o).max() > 0
+        assert rel(d_e, d_o) < FP64_TOL, i
+    eng.close()
+
+
 def test_plain_box_fp32_within_tolerance():
     scn = early_pulse(scenario.plain_box(n=32, steps=300, precision="fp32", pml=6))
     eng, orc = run_pair(scn)
