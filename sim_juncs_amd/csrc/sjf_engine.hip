@@ -4,10 +4,11 @@
 // polarisation state, sources, monitors, DFT accumulators).  Stepping goes through a launch
 // set: one engine, or a batch of engines, compiled into small device tables so that each
 // time step is a fixed sequence of launches on one stream,
-//   [k_monitor when step % save_span == 0] -> k_step -> k_source -> k_dft (or k_tick),
+//   [k_monitor when step % save_span == 0] -> k_step -> [k_source] -> [k_dft] (or k_tick),
 // the last of which advances the step counters, whatever the number of instances; source and
-// DFT share one launch (k_source_dft) when no DFT region holds a source cell.  The per-step scalars (step index, pulse value, DFT
-// phase) are computed on the device from each engine's step counter, so a run of steps is
+// DFT share one launch (k_source_dft) when no DFT region holds a source cell.  The per-step
+// scalars (step index, pulse value, DFT phase) are computed on the device from each engine's
+// step counter, so a run of steps is
 // captured once in a hipGraph and replayed.  This is the sequence bound_geom::run performs
 // around fields.step() (reference src/disp.cpp:719-741).
 #include <hip/hip_runtime.h>
@@ -538,7 +539,7 @@ int build_set_t(LaunchSet* s) {
     for (sjf_engine* e : s->eng) ctr.push_back(e->d_step);
     rc = upload_vec(e0, s->stream, (void**)&s->d_counters, ctr);
     if (rc) return rc;
-    rc = upload_vec(e0, s->stream, (void**)&s->d_done, std::vector<unsigned>(9, 0u));  // armed at zero
+    rc = upload_vec(e0, s->stream, (void**)&s->d_done, std::vector<unsigned>(sjf::kDoneWords, 0u));  // armed at zero
     if (rc) return rc;
 
     s->versions.clear();
@@ -614,9 +615,8 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
     int rc = launch_step_t<T>(s);
     if (rc) return rc;
     if (ev_b) SETCHK(s, hipEventRecord(ev_b, s->stream));
-    // the DFT kernel, when it runs, also advances the step counters; otherwise k_tick does.
-    // (Folded into the source kernel instead, the sign-off costs more than the launch it saves:
-    // 256 source workgroups of 256 threads, 10.9 us against 4.7 + 4.7 us.)
+    // the last kernel of the step advances the step counters: the DFT kernel when there is
+    // one, else the source kernel, else k_tick
     const int n = (int)s->eng.size();
     const bool dft = s->ndft && !bare;
     const sjf::SrcTables<T> st{(const sjf::SrcDev<T>*)s->d_src, s->d_blk_src, s->d_blk_first, s->d_blk_end,
@@ -631,14 +631,15 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
         return SJF_OK;
     }
     if (s->nsrcblk) {
-        hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream, st);
+        const sjf::Tick src_tick{s->d_counters, n, dft ? nullptr : s->d_done};
+        hipLaunchKernelGGL((sjf::k_source<T>), dim3(s->nsrcblk), dim3(256), 0, s->stream, st, src_tick);
         SETCHK(s, hipGetLastError());
     }
     if (dft) {
         hipLaunchKernelGGL((sjf::k_dft<T>), dim3((unsigned)dftx, (unsigned)s->dft_maxnf), dim3(256), 0, s->stream,
                            (const sjf::DftDev<T>*)s->d_dft, s->ndft, s->dft_total, tick);
         SETCHK(s, hipGetLastError());
-    } else {
+    } else if (!s->nsrcblk) {
         hipLaunchKernelGGL(sjf::k_tick, dim3((n + 63) / 64), dim3(64), 0, s->stream, s->d_counters, n);
         SETCHK(s, hipGetLastError());
     }
