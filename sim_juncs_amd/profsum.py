@@ -109,10 +109,11 @@ def trace(db: str):
     for name, dur in rows:
         agg[name].append(dur)
     tot = sum(sum(v) for v in agg.values())
-    print(f"| kernel | calls | total ms | mean us | min us | max us | % time |")
-    print("|---|---|---|---|---|---|---|")
+    print(f"| kernel | calls | total ms | mean us | median us | min us | max us | % time |")
+    print("|---|---|---|---|---|---|---|---|")
     for name, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
-        print(f"| `{short(name)}` | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e3:.2f} | "
+        med = sorted(v)[len(v) // 2]
+        print(f"| `{short(name)}` | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e3:.2f} | {med / 1e3:.2f} | "
               f"{min(v) / 1e3:.2f} | {max(v) / 1e3:.2f} | {100.0 * sum(v) / tot:.1f} |")
 
 
