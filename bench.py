@@ -8,7 +8,9 @@ source + DFT + monitor sampling every 20 steps).  Fields are resident in HBM bef
 
 Other workloads (--workload): plain (96^3 FP64), sweep (64 uneven instances batched),
 adjoint (256x256x128 with a 64x64x16 design region: forward + discrete adjoint), large
-(1024x768x512 FP32 plain run), variants (16 adjoint design variants).
+(1024x768x512 FP32 plain run), variants (16 adjoint design variants as one batch: one fused
+forward and one fused reverse launch per step over all of them), variants-sequential (the same
+16 variants, one adjoint run after another).
 
 Prints one JSON line: metric Yee cell-updates/s (cells including PML x steps / wall time),
 the dominant kernel's HBM roofline, and the FP64 CPU oracle timed on this host's cores.
@@ -33,7 +35,7 @@ def parse():
     p.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="bowtie",
-                   choices=["bowtie", "plain", "sweep", "adjoint", "large", "variants"])
+                   choices=["bowtie", "plain", "sweep", "adjoint", "large", "variants", "variants-sequential"])
     p.add_argument("--cpu-steps", type=int, default=None,
                    help="oracle steps timed for the CPU baseline (default: about 10 s of work)")
     p.add_argument("--no-cpu", action="store_true")
@@ -157,6 +159,45 @@ def sweep(a):
                                   f"sweep_{len(scns)}_instances"), cpu=cpu, dtype="f32")
 
 
+def adjoint_batched(a):
+    """The 16 design variants as one batch: per step one fused forward launch and one fused
+    reverse launch over all of them (sjf_batch_adjoint_run, record mode)."""
+    from sim_juncs_amd import scenario
+    from sim_juncs_amd.engine import Batch, Engine
+
+    scns = scenario.adjoint_variants()
+    steps = a.steps if a.steps is not None else scns[0].steps
+    engs = [s.make(Engine) for s in scns]
+    for e in engs:
+        set_tile(a, e)  # a register-rows tile also sets the reverse step's x-chunk
+    b = Batch(engs)
+    cells, _ = b.info()
+    bytes_rev = sum(e.step_bytes() for e in engs)
+    t = time.perf_counter()
+    J, grads = b.adjoint_run(steps, [s.meta["gap"] for s in scns], [s.design[2].size for s in scns])
+    wall = time.perf_counter() - t
+    ms, threads, groups = b.adjoint_kernel_info()
+    log(f"{len(scns)} variants: J {J.min():.4e}..{J.max():.4e}; reverse launch {ms * 1e3:.1f} us over "
+        f"{groups} workgroups of {threads} threads")
+    fthreads, fgroups = b.launch_info()
+    b.close()
+    for e in engs:
+        e.close()
+    s0 = scns[0]
+    workload = f"variants_{len(scns)}_batched"
+    rl = roofline(bytes_rev, ms, "k_adj_step_batch", threads, groups, workload)
+    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or max(3, int(5e9 / s0.cells)))
+    if cpu:
+        cpu["value"] /= 2.0
+        cpu["sample"] += "; divided by 2 for the forward and reverse sweeps"
+    return dict(value=cells * steps / wall, steps=steps, wall=wall,
+                config={"workload": workload, "cells": cells, "precision": "fp32", "checkpoint_every": 0,
+                        "launch": f"per step one fused forward launch ({fgroups} workgroups of {fthreads}) and "
+                                  f"one fused reverse launch over all {len(scns)} variants",
+                        "note": "value counts forward steps per second of a full gradient computation"},
+                roofline=rl, cpu=cpu, dtype="f32")
+
+
 def adjoint(a, variants=False):
     from sim_juncs_amd import scenario
     from sim_juncs_amd.engine import Engine
@@ -215,8 +256,10 @@ def main():
         r = sweep(a)
     elif a.workload == "adjoint":
         r = adjoint(a)
-    else:
+    elif a.workload == "variants-sequential":
         r = adjoint(a, variants=True)
+    else:
+        r = adjoint_batched(a)
     log(f"done in {time.time() - t0:.1f}s")
     out = {
         "metric": "Yee cell-updates/s",

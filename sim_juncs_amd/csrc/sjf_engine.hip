@@ -1270,6 +1270,215 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     return SJF_OK;
 }
 
+template <typename T, int NP>
+void launch_adj_batch_np(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* first, int n, int block,
+                         unsigned tiles, size_t lds) {
+    if (block <= 256)
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 256>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+    else if (block <= 512)
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 512>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+    else
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 1024>), dim3(tiles), dim3(block), lds, s->stream, args, first,
+                           n);
+}
+
+// Adjoint gradients of every engine of a launch set in one pass, in record mode: the forward
+// steps are the set's own fused launches; after each of them one launch records the design
+// fields of all engines and adds their objectives; each reverse step is one gradient launch, one
+// fused reverse launch over all engines' tiles and one injection launch.  The arithmetic of every
+// cell is that of adjoint_run_t, so each engine's J and gradient are those of its own run.
+// gaps[i]: objective cell index of engine i; nds[i]: its design voxels.
+template <typename T>
+int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long* nds, double* J_out,
+                        double* const* grads, double* ms_out, int* threads_out, int* groups_out) {
+    sjf_engine* e0 = s->eng[0];
+    const int n = (int)s->eng.size();
+    int rc = prepare_set(s, nsteps);
+    if (rc) return rc;
+    double rec_bytes = 0;
+    for (int i = 0; i < n; ++i) rec_bytes += (double)sizeof(T) * (3 * nds[i] + 3) * (nsteps + 1);
+    size_t fr = 0, tot = 0;
+    HIPCHK(e0, hipMemGetInfo(&fr, &tot));
+    if (rec_bytes > 0.25 * (double)fr)
+        return fail(e0, SJF_E_NOMEM, "the design-field record of the batch (" + std::to_string(rec_bytes / 1e9) +
+                                         " GB) exceeds a quarter of the free device memory");
+    std::vector<void*> mem;
+    auto cleanup = [&]() {
+        for (void*& p : mem) dev_free(p);
+        mem.clear();
+    };
+    auto bail = [&](int r) {
+        cleanup();
+        return r;
+    };
+    auto alloc = [&](void** p, size_t bytes) -> int {
+        *p = nullptr;
+        const int r = dev_alloc(e0, p, bytes);
+        if (!r) mem.push_back(*p);
+        return r;
+    };
+    std::vector<sjf::AdjBatchDev<T>> tab(n);
+    std::vector<sjf::AdjArgs<T>> args(n);
+    std::vector<int> first;
+    int t = 0, nmat = 1;
+    long maxv = 0;
+    for (int i = 0; i < n; ++i) {
+        sjf_engine* e = s->eng[i];
+        const StateLayout L = state_layout(e);
+        const long nd = nds[i];
+        void *A = nullptr, *Aq = nullptr, *Ap = nullptr, *ring = nullptr, *gacc = nullptr, *dJ = nullptr;
+        void* Apsi[3] = {nullptr, nullptr, nullptr};
+        if (alloc(&A, sizeof(T) * 12 * e->N) || (L.q && alloc(&Aq, L.q)) || (L.p && alloc(&Ap, L.p)) ||
+            alloc(&ring, sizeof(T) * (3 * nd + 3) * (nsteps + 1)) || alloc(&gacc, sizeof(double) * nd) ||
+            alloc(&dJ, sizeof(double)))
+            return bail(SJF_E_NOMEM);
+        for (int d = 0; d < 3; ++d) {
+            if ((double)L.psi[d] / 6 * 8 >= 2147483648.0)
+                return bail(fail(e0, SJF_E_UNSUPPORTED, "adjoint PML auxiliaries exceed 2 GiB on one axis"));
+            if (L.psi[d] && alloc(&Apsi[d], L.psi[d] / 6 * 8)) return bail(SJF_E_NOMEM);
+        }
+        sjf::AdjBatchDev<T>& d = tab[i];
+        std::memset(&d, 0, sizeof(d));
+        d.F[0] = (const T*)e->Fbuf[0];
+        d.F[1] = (const T*)e->Fbuf[1];
+        d.A[0] = (T*)A;
+        d.A[1] = (T*)A + (size_t)6 * e->N;
+        d.ring = (T*)ring;
+        d.gacc = (double*)gacc;
+        d.J = (double*)dJ;
+        d.nyz = (long)e->g.ny * e->g.nz;
+        d.ny = e->g.ny;
+        d.nz = e->g.nz;
+        for (int q = 0; q < 3; ++q) {
+            d.dlo[q] = e->dlo[q];
+            d.de[q] = e->dhi[q] - e->dlo[q];
+            d.ie[q] = inv_eps_at(e, gaps[i], q);
+        }
+        d.nd = nd;
+        d.gap = gaps[i];
+        d.dt = e->dt;
+        sjf::AdjArgs<T>& g = args[i];
+        std::memset(&g, 0, sizeof(g));
+        g.f = step_args<T>(e, 1, adj_xchunk(e));  // one y row per workgroup, as adjoint_run_t
+        g.f.nty = e->g.ny;
+        g.A[0] = d.A[0];
+        g.A[1] = d.A[1];
+        g.Aq = (T*)Aq;
+        g.Ap = (T*)Ap;
+        for (int q = 0; q < 3; ++q) g.Apsi[q] = (T*)Apsi[q];
+        first.push_back(t);
+        t += (int)((long)e->g.ny * ((e->g.nx + g.f.xc - 1) / g.f.xc));
+        nmat = std::max(nmat, e->nmat);
+        maxv = std::max(maxv, nd + 3);
+    }
+    first.push_back(t);
+    // the reverse step's buffer roles alternate with the parity p of the step index n: read set
+    // rd = (nsteps - 1 - n) & 1, H auxiliary pair read 2 / 4 and E pair read 0 / 6
+    std::vector<sjf::AdjArgs<T>> argp[2] = {args, args};
+    for (int p = 0; p < 2; ++p)
+        for (sjf::AdjArgs<T>& g : argp[p]) {
+            g.rd = (int)((nsteps - 1 - p) & 1);
+            g.hr = p ? 2 : 4;
+            g.hw = p ? 4 : 2;
+            g.er = p ? 0 : 6;
+            g.ew = p ? 6 : 0;
+        }
+    void *d_tab = nullptr, *d_args0 = nullptr, *d_args1 = nullptr, *d_first = nullptr;
+    if ((rc = upload_vec(e0, s->stream, &d_tab, tab))) return bail(rc);
+    mem.push_back(d_tab);
+    if ((rc = upload_vec(e0, s->stream, &d_args0, argp[0]))) return bail(rc);
+    mem.push_back(d_args0);
+    if ((rc = upload_vec(e0, s->stream, &d_args1, argp[1]))) return bail(rc);
+    mem.push_back(d_args1);
+    if ((rc = upload_vec(e0, s->stream, &d_first, first))) return bail(rc);
+    mem.push_back(d_first);
+    const auto* tabp = (const sjf::AdjBatchDev<T>*)d_tab;
+    const sjf::AdjArgs<T>* argd[2] = {(const sjf::AdjArgs<T>*)d_args0, (const sjf::AdjArgs<T>*)d_args1};
+    const int block = ((e0->g.nz + 63) / 64) * 64;
+    const size_t lds = sizeof(T) * 12 * block + sizeof(sjf::MatLut<T>) * nmat;
+    if (lds > 65536) return bail(fail(e0, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
+    const int npv = np_variant_of(e0->maxp, e0->uniform);
+    const dim3 grid_v((unsigned)((maxv + 255) / 256), (unsigned)n);
+    auto record = [&](long sl, int objective) -> int {
+        hipLaunchKernelGGL((sjf::k_adj_record_batch<T>), grid_v, dim3(256), 0, s->stream, tabp,
+                           (int)(e0->step & 1), sl, objective);
+        SETCHK(s, hipGetLastError());
+        return SJF_OK;
+    };
+    // ---- forward pass, recording after every step
+    if ((rc = record(0, 0))) return bail(rc);
+    for (long q = 0; q < nsteps; ++q) {
+        if ((rc = launch_one_step(s, samples_at(s, e0->step)))) return bail(rc);
+        advance_hosts(s, 1);
+        if ((rc = record(q + 1, 1))) return bail(rc);
+    }
+    // ---- reverse pass
+    constexpr int kTimed = 64;
+    std::vector<hipEvent_t> evs;
+    auto free_events = [&]() {
+        for (auto& ev : evs) (void)hipEventDestroy(ev);
+        evs.clear();
+    };
+    auto bail_ev = [&](int r) {
+        free_events();
+        return bail(r);
+    };
+    for (int q = 0; q < 2 * kTimed; ++q) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreate(&ev) != hipSuccess) return bail_ev(fail(e0, SJF_E_HIP, "hipEventCreate failed"));
+        evs.push_back(ev);
+    }
+    int rd = 0;
+    auto inject = [&](long sl) -> int {
+        hipLaunchKernelGGL((sjf::k_adj_inject_batch<T>), dim3(1, (unsigned)n), dim3(64), 0, s->stream, tabp, rd, sl);
+        SETCHK(s, hipGetLastError());
+        return SJF_OK;
+    };
+    const dim3 grid_g((unsigned)((maxv - 3 + 255) / 256), (unsigned)n);
+    long ntimed = 0;
+    if ((rc = inject(nsteps))) return bail_ev(rc);
+    for (long q = nsteps - 1; q >= 0; --q) {
+        hipLaunchKernelGGL((sjf::k_adj_grad_batch<T>), grid_g, dim3(256), 0, s->stream, tabp, rd, q + 1, q);
+        SETCHK(s, hipGetLastError());
+        const bool timed = ntimed < kTimed;
+        if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed], s->stream));
+        const sjf::AdjArgs<T>* ad = argd[q & 1];
+        switch (npv) {
+            case 0: launch_adj_batch_np<T, 0>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
+            case 1: launch_adj_batch_np<T, 1>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
+            case 3: launch_adj_batch_np<T, 3>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
+            default: launch_adj_batch_np<T, 4>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
+        }
+        SETCHK(s, hipGetLastError());
+        if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed++ + 1], s->stream));
+        rd = 1 - rd;
+        if (q >= 1 && (rc = inject(q))) return bail_ev(rc);
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess)
+        return bail_ev(fail(e0, SJF_E_HIP, "hipStreamSynchronize failed"));
+    double msum = 0;
+    for (long q = 0; q < ntimed; ++q) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, evs[2 * q], evs[2 * q + 1]) != hipSuccess)
+            return bail_ev(fail(e0, SJF_E_HIP, "hipEventElapsedTime failed"));
+        msum += ms;
+    }
+    free_events();
+    *ms_out = ntimed ? msum / ntimed : 0;
+    *threads_out = block;
+    *groups_out = t;
+    for (int i = 0; i < n; ++i) {
+        std::vector<double> gh(nds[i]);
+        if ((rc = d2h(e0, gh.data(), tab[i].gacc, sizeof(double) * nds[i])) ||
+            (rc = d2h(e0, &J_out[i], tab[i].J, sizeof(double))))
+            return bail(rc);
+        // the state is Ubar = Ebar / eps, so the sums already carry the 1/eps_v factor
+        for (long v = 0; v < nds[i]; ++v) grads[i][v] = -gh[v];
+    }
+    cleanup();
+    return SJF_OK;
+}
+
 }  // namespace
 
 extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
@@ -1303,6 +1512,8 @@ extern "C" int sjf_adjoint_kernel_info(const sjf_engine* e, double* ms_mean, int
 struct sjf_batch {
     LaunchSet* set = nullptr;
     std::string err;
+    double adj_ms = 0;  // fused reverse launch of the last batched adjoint run
+    int adj_threads = 0, adj_groups = 0;
 };
 
 namespace {
@@ -1965,6 +2176,60 @@ int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups) {
     if (!b->set->built) return SJF_E_STATE;
     if (threads) *threads = b->set->block;
     if (groups) *groups = b->set->ntiles;
+    return SJF_OK;
+}
+
+int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, const int* gaps, double* J, double* const* grads,
+                          const size_t* caps) {
+    if (!b || nsteps < 1 || !gaps || !J || !grads || !caps) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    sjf_engine* e0 = s->eng[0];
+    const int n = (int)s->eng.size();
+    std::vector<long> q(n, 0), nd(n, 0);
+    std::string why;
+    for (int i = 0; i < n && why.empty(); ++i) {
+        const sjf_engine* e = s->eng[i];
+        const int ext[3] = {e->g.nx, e->g.ny, e->g.nz};
+        if (e->step != 0) why = "the adjoint run starts from step 0";
+        else if (e->deps_h.empty()) why = "an engine has no design region (sjf_set_design_eps)";
+        else if (e->g.nz != e0->g.nz) why = "engines of a batched adjoint run must share nz";
+        else if (np_variant_of(e->maxp, e->uniform) != np_variant_of(e0->maxp, e0->uniform))
+            why = "engines of a batched adjoint run must share the pole-slot variant";
+        else if (!grads[i]) why = "null gradient buffer";
+        for (int d = 0; d < 3 && why.empty(); ++d)
+            if (gaps[3 * i + d] < 0 || gaps[3 * i + d] >= ext[d]) why = "objective cell outside the grid";
+        if (why.empty()) {
+            nd[i] = (long)e->deps_h.size();
+            if (caps[i] < (size_t)nd[i]) why = "gradient buffer smaller than the design region";
+            q[i] = cell_index(e, gaps[3 * i], gaps[3 * i + 1], gaps[3 * i + 2]);
+        }
+    }
+    if (!why.empty()) {
+        b->err = why;
+        return SJF_E_ARG;
+    }
+    if (hipSetDevice(e0->g.device) != hipSuccess) return SJF_E_HIP;
+    double ms = 0;
+    int th = 0, gr = 0;
+    const int rc = s->prec == 4
+                       ? batch_adjoint_run_t<float>(s, nsteps, q.data(), nd.data(), J, grads, &ms, &th, &gr)
+                       : batch_adjoint_run_t<double>(s, nsteps, q.data(), nd.data(), J, grads, &ms, &th, &gr);
+    if (rc) {
+        b->err = e0->err;
+        return rc;
+    }
+    b->adj_ms = ms;
+    b->adj_threads = th;
+    b->adj_groups = gr;
+    return SJF_OK;
+}
+
+int sjf_batch_adjoint_kernel_info(const sjf_batch* b, double* ms_mean, int* threads, int* groups) {
+    if (!b || !ms_mean) return SJF_E_ARG;
+    if (b->adj_groups == 0) return SJF_E_STATE;
+    *ms_mean = b->adj_ms;
+    if (threads) *threads = b->adj_threads;
+    if (groups) *groups = b->adj_groups;
     return SJF_OK;
 }
 

@@ -1779,6 +1779,101 @@ __global__ void k_adj_inject(T* Ubar, long nyz, const T* ring, long nd, long slo
     }
 }
 
+// ---- adjoint over a batch: the kernels above, one launch for every engine of the set --------
+// Per-engine table of the batched record, gradient and injection kernels (blockIdx.y is the
+// engine); the fused reverse step maps workgroups to engines as k_step_rows does.
+template <typename T>
+struct AdjBatchDev {
+    const T* F[2];   // forward field sets 0 / 1
+    T* A[2];         // adjoint field sets (Ubar, Hbar)
+    T* ring;         // record: per step E of the design box (3 nd) and of the objective cell (3)
+    double* gacc;    // nd gradient sums
+    double* J;       // objective
+    long nyz;
+    int ny, nz;
+    int dlo[3], de[3];
+    long nd;
+    long gap;        // objective cell
+    double dt;
+    double ie[3];    // 1/eps at the objective cell's three E points
+};
+
+// k_adj_record for every engine (E = field set `par`), and with `objective` set also
+// k_adj_objective: the thread of the objective cell's first component adds dt sum_c E_c(gap)^2
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_record_batch(const AdjBatchDev<T>* __restrict__ tab, int par,
+                                                          long slot, int objective) {
+    const AdjBatchDev<T>& a = tab[blockIdx.y];
+    const long nd = a.nd;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nd + 3) return;
+    const T* E = a.F[par];
+    T* out = a.ring + slot * (3 * nd + 3);
+    if (v < nd) {
+        const int kk = (int)(v % a.de[2]);
+        const long r = v / a.de[2];
+        const int jj = (int)(r % a.de[1]), ii = (int)(r / a.de[1]);
+        const long q = ((long)(a.dlo[0] + ii) * a.ny + (a.dlo[1] + jj)) * a.nz + (a.dlo[2] + kk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c * nd + v] = E[field_index(q, c, a.nyz)];
+    } else {
+        const int c = (int)(v - nd);
+        out[3 * nd + c] = E[field_index(a.gap, c, a.nyz)];
+        if (objective && c == 0) {
+            double sum = 0;
+            for (int k = 0; k < 3; ++k) {
+                const double x = (double)E[field_index(a.gap, k, a.nyz)];
+                sum += x * x;
+            }
+            *a.J += sum * a.dt;
+        }
+    }
+}
+
+// k_adj_grad for every engine (Ubar^{n+1} in adjoint set `rd`)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_grad_batch(const AdjBatchDev<T>* __restrict__ tab, int rd,
+                                                        long s_next, long s_cur) {
+    const AdjBatchDev<T>& a = tab[blockIdx.y];
+    const long nd = a.nd;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nd) return;
+    const int kk = (int)(v % a.de[2]);
+    const long r = v / a.de[2];
+    const int jj = (int)(r % a.de[1]), ii = (int)(r / a.de[1]);
+    const long q = ((long)(a.dlo[0] + ii) * a.ny + (a.dlo[1] + jj)) * a.nz + (a.dlo[2] + kk);
+    const T* Ebar = a.A[rd];
+    const T* e1 = a.ring + s_next * (3 * nd + 3);
+    const T* e0 = a.ring + s_cur * (3 * nd + 3);
+    double acc = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        acc += (double)Ebar[field_index(q, c, a.nyz)] * ((double)e1[c * nd + v] - (double)e0[c * nd + v]);
+    a.gacc[v] += acc;
+}
+
+// k_adj_inject for every engine (into adjoint set `rd`, record slot `slot`)
+template <typename T>
+__global__ void k_adj_inject_batch(const AdjBatchDev<T>* __restrict__ tab, int rd, long slot) {
+    const AdjBatchDev<T>& a = tab[blockIdx.y];
+    const int c = threadIdx.x;
+    if (blockIdx.x == 0 && c < 3) {
+        const double e = (double)a.ring[slot * (3 * a.nd + 3) + 3 * a.nd + c];
+        const long o = field_index(a.gap, c, a.nyz);
+        T* U = a.A[rd];
+        U[o] = (T)((double)U[o] + 2.0 * a.dt * e * a.ie[c]);
+    }
+}
+
+// Fused reverse step over every engine of a batch (tile t of engine b after the XCD-aware remap)
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(adj_waves<T>())))
+k_adj_step_batch(const AdjArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
+    int t;
+    const int b = tile_instance(tile_first, ninst, t);
+    adj_tile<T, NP>(args[b], t);
+}
+
 // ---- plain streaming copy (achievable-bandwidth probe and counter calibration) -----------
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                               long n) {

@@ -321,6 +321,29 @@ class Batch:
         self._chk(self.lib.sjf_batch_launch_info(self.h, C.byref(th), C.byref(gr)), "sjf_batch_launch_info")
         return th.value, gr.value
 
+    def adjoint_run(self, nsteps: int, gaps, n_design):
+        """Engine.adjoint_run (record mode) for every engine in one pass: one fused reverse launch
+        per step over all engines.  gaps: one (i, j, k) per engine; n_design: design voxels per
+        engine.  Returns (J per engine, list of gradients)."""
+        n = len(self.engines)
+        if len(gaps) != n or len(n_design) != n:
+            raise ValueError("one gap cell and one design size per engine")
+        g = np.ascontiguousarray(np.asarray(gaps, dtype=np.int32).reshape(n, 3))
+        J = np.zeros(n)
+        grads = [np.zeros(int(m)) for m in n_design]
+        gp = (C.POINTER(C.c_double) * n)(*[_dp(x) for x in grads])
+        caps = (C.c_size_t * n)(*[x.size for x in grads])
+        self._chk(self.lib.sjf_batch_adjoint_run(self.h, int(nsteps), _ip(g.ravel()), _dp(J), gp, caps),
+                  "sjf_batch_adjoint_run")
+        return J, grads
+
+    def adjoint_kernel_info(self):
+        """(mean ms, threads, workgroups) of the fused reverse launch of the last batched adjoint run."""
+        ms, th, gr = C.c_double(), C.c_int(), C.c_int()
+        self._chk(self.lib.sjf_batch_adjoint_kernel_info(self.h, C.byref(ms), C.byref(th), C.byref(gr)),
+                  "sjf_batch_adjoint_kernel_info")
+        return ms.value, th.value, gr.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.sjf_batch_destroy(self.h)

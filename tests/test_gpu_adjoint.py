@@ -194,3 +194,103 @@ def test_adjoint_beyond_4gib_matches_small_grid():
     assert res[0][0] > 0 and np.abs(res[0][1]).max() > 0
     assert res[0][0] == res[1][0]
     assert np.array_equal(res[0][1], res[1][1])
+
+
+def batch_member(shape, periodic, lo, hi, gap, seed, precision, steps):
+    """An adjoint case of its own shape, design box and objective cell (Au electrodes and a SiO2
+    slab as in design_case, so every member runs the same pole-slot variant)."""
+    rng = np.random.default_rng(seed)
+    dx = 0.02
+    mat = np.zeros(tuple(shape) + (3,), np.uint8)
+    lut = [M.VACUUM, M.AU.scaled(1.0), M.SIO2.scaled(1.0)]
+    scenario.paint(mat, shape, scenario.box_inside((-1, -1, 12.5), (5.5, 40, 15.5)), 1)
+    scenario.paint(mat, shape, scenario.box_inside((shape[0] - 5.5, -1, 12.5), (40, 40, 15.5)), 1)
+    scenario.paint(mat, shape, scenario.box_inside((-1, -1, 15.5), (40, 40, 21)), 2)
+    pml_cells = [0, 0, 0, 0, 3, 3] if any(periodic) else [3] * 6
+    eps = rng.uniform(1, 4, size=tuple(h - l for l, h in zip(lo, hi)))
+    scn = scenario.Scenario(name=f"batch_adj_{seed}", shape=tuple(shape), dx=dx, precision=precision,
+                            periodic=periodic, matw=scenario.pack(mat), lut=lut, pml=(pml_cells, 1e-15, 2.0),
+                            steps=steps, design=(lo, hi, eps))
+    scn.sources = [scenario.plane_source(tuple(shape), dx, 4,
+                                         scenario.gaussian_pulse(0.8, 1.5, 0.4 + 0.1 * seed, start_fs=0.0))]
+    return scn, gap, eps
+
+
+BATCH_MEMBERS = [
+    # shape, periodic, design lo, hi, objective cell, x-chunk of the register-rows mode
+    ((14, 12, 20), (0, 0, 0), (5, 4, 9), (9, 8, 12), (7, 6, 13), 5),
+    ((18, 10, 20), (0, 0, 0), (6, 3, 8), (11, 6, 13), (9, 5, 12), 4),
+    ((13, 16, 20), (1, 1, 0), (3, 5, 10), (7, 12, 12), (6, 9, 13), 3),
+]
+
+
+@pytest.mark.parametrize("mode", ["register_rows", "default"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
+    """sjf_batch_adjoint_run over engines of different shapes, design boxes, objective cells and
+    periodic axes gives every engine the J and gradient of its own sjf_adjoint_run, bit for bit
+    (the per-engine run is pinned to finite differences of the oracle above), and leaves each
+    engine in its final forward state.  The batch's forward launch runs one tile form for all
+    engines, and the forms round differently in the last bit, so each mode runs the same form
+    alone and in the batch: register rows with each engine's x-chunk (which the reverse step
+    then takes too), or the default row-of-threads form (the reverse step's default chunks)."""
+    from sim_juncs_amd.engine import Batch
+
+    steps = 60
+    cases = [batch_member(sh, per, lo, hi, gap, i, precision, steps)
+             for i, (sh, per, lo, hi, gap, _) in enumerate(BATCH_MEMBERS)]
+
+    def make(scn, m):
+        eng = scn.make(Engine)
+        if mode == "register_rows":
+            eng.set_tile("register_rows", 1, m[5])
+        return eng
+
+    own = []
+    for (scn, gap, eps), m in zip(cases, BATCH_MEMBERS):
+        print(f"[case] own adjoint {precision} {mode} {scn.shape}", flush=True)
+        eng = make(scn, m)
+        J, g = eng.adjoint_run(steps, 0, gap, eps.size)
+        own.append((J, g, eng.read_field("Ey")))
+        eng.close()
+    print(f"[case] batched adjoint {precision} {mode}", flush=True)
+    engs = [make(scn, m) for (scn, _, _), m in zip(cases, BATCH_MEMBERS)]
+    b = Batch(engs)
+    if mode == "register_rows":
+        b.set_tile("register_rows", 0)
+    with pytest.raises(EngineError):
+        b.adjoint_kernel_info()
+    J, grads = b.adjoint_run(steps, [gap for _, gap, _ in cases], [eps.size for _, _, eps in cases])
+    ms, threads, groups = b.adjoint_kernel_info()
+    b.close()
+    assert 0 < ms < 100 and threads == 64
+    tiles = 0
+    for (scn, _, _), m in zip(cases, BATCH_MEMBERS):
+        nx, ny, _ = scn.shape
+        if mode == "register_rows":
+            xc = m[5]
+        else:
+            nxc = min(-(-16384 // ny), nx)
+            xc = -(-nx // nxc)
+        tiles += ny * -(-nx // xc)
+    assert groups == tiles
+    for i, eng in enumerate(engs):
+        assert J[i] > 0 and J[i] == own[i][0], i
+        assert np.array_equal(grads[i], own[i][1]), i
+        assert eng.step_count() == steps
+        assert np.array_equal(eng.read_field("Ey"), own[i][2]), i
+        eng.close()
+
+
+def test_batched_adjoint_rejects_members_it_cannot_run_together():
+    from sim_juncs_amd.engine import Batch
+
+    a, gap_a, eps_a = batch_member((14, 12, 20), (0, 0, 0), (5, 4, 9), (9, 8, 12), (7, 6, 13), 0, "fp32", 10)
+    c, gap_c, eps_c = batch_member((14, 12, 24), (0, 0, 0), (5, 4, 9), (9, 8, 12), (7, 6, 13), 1, "fp32", 10)
+    engs = [a.make(Engine), c.make(Engine)]
+    b = Batch(engs)
+    with pytest.raises(EngineError, match="nz"):
+        b.adjoint_run(10, [gap_a, gap_c], [eps_a.size, eps_c.size])
+    b.close()
+    for e in engs:
+        e.close()
