@@ -43,6 +43,8 @@ def parse():
                    help="steps timed with per-launch HIP events for the kernel duration")
     p.add_argument("--tile", default=None,
                    help="fused-kernel tile as form,rows,xchunk (default: tuned cache or engine default)")
+    p.add_argument("--batch-tile", default=None,
+                   help="tile of a batch's fused forward launch as form,xchunk (sweep, variants)")
     return p.parse_args()
 
 
@@ -84,6 +86,12 @@ def roofline(bytes_step, ms_kernel, kernel, threads, groups, workload):
             "frac": achieved / HBM_PEAK_GBS, "traffic": rec["traffic_bytes"] if rec else None,
             "kernel": kernel, "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step,
             "threads": threads, "groups": groups}
+
+
+def set_batch_tile(a, b):
+    if a.batch_tile:
+        form, xc = a.batch_tile.split(",")
+        b.set_tile(form, int(xc))
 
 
 def step_kernel(info):
@@ -137,6 +145,7 @@ def sweep(a):
     steps = a.steps if a.steps is not None else scns[0].steps
     engs = [s.make(Engine) for s in scns]
     b = Batch(engs)
+    set_batch_tile(a, b)
     cells, bytes_step = b.info()
     log(f"batch of {len(engs)} engines, {cells} cells")
     b.step(a.warmup)
@@ -171,6 +180,7 @@ def adjoint_batched(a):
     for e in engs:
         set_tile(a, e)  # a register-rows tile also sets the reverse step's x-chunk
     b = Batch(engs)
+    set_batch_tile(a, b)
     cells, _ = b.info()
     bytes_rev = sum(e.step_bytes() for e in engs)
     t = time.perf_counter()

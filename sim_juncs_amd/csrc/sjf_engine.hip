@@ -1043,18 +1043,23 @@ int set_step(sjf_engine* e, long n) {
     return SJF_OK;
 }
 
-// x-chunk of the fused reverse step: the one sjf_set_tile fixed, else about 16384 workgroups
-// (one y row each); measured on MI355X, 4-6 planes beat 8 and more (256x256x128: 1.019 ms per
-// gradient step at 4 planes, 1.041 at 8, 1.13 at 16)
-// x-chunk of the reverse step (one y row per workgroup): that of a register-rows tile set on the
-// engine (the same tiling), else enough chunks for about 16384 workgroups
-int adj_xchunk(const sjf_engine* e) {
+// x-chunk of the fused reverse step (one y row per workgroup): that of a register-rows tile set
+// on the engine (the same tiling), else enough chunks for about `target` workgroups over `rows`
+// y rows (the engine's own, or those of every engine of a batch).  Measured on MI355X: alone,
+// 256x256x128 runs 1.019 ms per gradient step at 4 planes (16384 workgroups), 1.041 at 8 and
+// 1.13 at 16; sixteen 192x192x96 engines in one batch run the reverse launch in 1.96 ms at 3
+// planes (196608 workgroups), 1.82 ms at 6 and 1.79 ms at 12 on one box, and in 1.61 ms at 18
+// planes (33792 workgroups) against 1.64-1.68 ms at 12 and 1.69-1.71 ms at 32 on another.
+int adj_xchunk(const sjf_engine* e, long target = 16384, long rows = 0) {
     if (e->tile_set && e->form == SJF_TILE_REGISTER_ROWS) return std::min(e->xc, e->g.nx);
-    const long target = 16384;
-    long nxc = std::max(1L, (target + e->g.ny - 1) / e->g.ny);
+    if (rows <= 0) rows = e->g.ny;
+    long nxc = std::max(1L, (target + rows - 1) / rows);
     nxc = std::min<long>(nxc, e->g.nx);
     return std::max(1, (int)((e->g.nx + nxc - 1) / nxc));
 }
+
+// workgroups the reverse launch of a batch aims at (x-chunks common to all engines)
+constexpr long kBatchAdjGroups = 32768;
 
 template <typename T, int NP>
 void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
@@ -1321,7 +1326,8 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
     std::vector<sjf::AdjArgs<T>> args(n);
     std::vector<int> first;
     int t = 0, nmat = 1;
-    long maxv = 0;
+    long maxv = 0, rows = 0;
+    for (const sjf_engine* e : s->eng) rows += e->g.ny;
     for (int i = 0; i < n; ++i) {
         sjf_engine* e = s->eng[i];
         const StateLayout L = state_layout(e);
@@ -1359,7 +1365,7 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
         d.dt = e->dt;
         sjf::AdjArgs<T>& g = args[i];
         std::memset(&g, 0, sizeof(g));
-        g.f = step_args<T>(e, 1, adj_xchunk(e));  // one y row per workgroup, as adjoint_run_t
+        g.f = step_args<T>(e, 1, adj_xchunk(e, kBatchAdjGroups, rows));  // one y row per workgroup
         g.f.nty = e->g.ny;
         g.A[0] = d.A[0];
         g.A[1] = d.A[1];

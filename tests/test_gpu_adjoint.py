@@ -265,12 +265,13 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     b.close()
     assert 0 < ms < 100 and threads == 64
     tiles = 0
+    rows = sum(scn.shape[1] for scn, _, _ in cases)
     for (scn, _, _), m in zip(cases, BATCH_MEMBERS):
         nx, ny, _ = scn.shape
         if mode == "register_rows":
             xc = m[5]
-        else:
-            nxc = min(-(-16384 // ny), nx)
+        else:  # x-chunks common to all engines, for about 32768 workgroups in all
+            nxc = min(-(-32768 // rows), nx)
             xc = -(-nx // nxc)
         tiles += ny * -(-nx // xc)
     assert groups == tiles
