@@ -43,6 +43,9 @@ def parse():
                    help="steps timed with per-launch HIP events for the kernel duration")
     p.add_argument("--tile", default=None,
                    help="fused-kernel tile as form,rows,xchunk (default: tuned cache or engine default)")
+    p.add_argument("--checkpoint-every", type=int, default=0,
+                   help="adjoint workload: checkpoint every K steps and recompute (0: record the design "
+                        "fields of every step, no recomputation; both give the same bits)")
     p.add_argument("--batch-tile", default=None,
                    help="tile of a batch's fused forward launch as form,xchunk (sweep, variants)")
     return p.parse_args()
@@ -222,7 +225,7 @@ def adjoint(a, variants=False):
         set_tile(a, eng)  # the forward steps and the reverse steps run the tuned x-chunk
         nd = s.design[2].size
         t = time.perf_counter()
-        J, g = eng.adjoint_run(steps, 0, s.meta["gap"], nd)  # record mode (no recomputation)
+        J, g = eng.adjoint_run(steps, a.checkpoint_every, s.meta["gap"], nd)
         wall += time.perf_counter() - t
         cells += s.cells
         log(f"{s.name}: J={J:.4e} |grad|={float((g * g).sum()) ** 0.5:.4e}")
@@ -242,7 +245,7 @@ def adjoint(a, variants=False):
         cpu["sample"] += "; divided by 2 for the forward and reverse sweeps"
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": workload, "cells": cells,
-                        "precision": "fp32", "checkpoint_every": 0,
+                        "precision": "fp32", "checkpoint_every": a.checkpoint_every,
                         "note": "value counts forward steps per second of a full gradient computation"},
                 roofline=rl, cpu=cpu, dtype="f32")
 
