@@ -510,7 +510,7 @@ int build_set_t(LaunchSet* s) {
             r.acc = d.acc;
             r.cmp = d.cmp;
             regs.push_back(r);
-            tot += d.ncell;
+            tot += d.ncell * d.ncomp;  // one thread per (component, cell) of each frequency
         }
     }
     s->ndft = (int)regs.size();

@@ -1145,7 +1145,7 @@ struct DftDev {
     long N, nyz, nz;
     int lo[3], ext[3];
     long ncell;
-    long first;          // first global cell index of this region in the launch
+    long first;          // first global index of this region's (component, cell) pairs
     int comps[6];
     int ncomp;
     int nf;
@@ -1155,15 +1155,16 @@ struct DftDev {
     double2* cmp;        // compensation terms, same layout
 };
 
-// accumulation of one (cell, frequency) of region ri
+// accumulation of one (component, cell, frequency) of region ri
 template <typename T>
 __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int ri, int f, long g, const double* ph,
                                          int blk_reg);
 
-// One thread per (cell, frequency): blockIdx.y is the frequency index.  The phase factors depend
-// only on the step and the frequency, so one thread per block evaluates them (FP64 sincos) and
-// shares them through LDS; a block's cells all lie in one region when regions start on block
-// boundaries, otherwise each thread evaluates its own.
+// One thread per (component, cell, frequency): blockIdx.y is the frequency index, and the
+// threads of a region run over its components' cells one component after another.  The phase
+// factors depend only on the step and the frequency, so one thread per block evaluates them
+// (FP64 sincos) and shares them through LDS; a block's threads all lie in one region when
+// regions start on block boundaries, otherwise each thread evaluates its own.
 template <typename T>
 __device__ __forceinline__ void dft_block(const DftDev<T>* __restrict__ regs, const int nreg, const long total,
                                           const int bx, const int f) {
@@ -1222,7 +1223,9 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, con
                                          const double* ph, const int blk_reg) {
     const DftDev<T>& a = regs[ri];
     if (f >= a.nf) return;
-    const long cell = g - a.first;
+    const long local = g - a.first;
+    const int s = (int)(local / a.ncell);  // component slot
+    const long cell = local - (long)s * a.ncell;
     const long step = *a.d_step;
     const T* F = a.F[(step + 1) & 1];
     const int kk = (int)(cell % a.ext[2]);
@@ -1241,16 +1244,14 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, con
         sincos(w * ((step + 1) * a.dt), &sE, &cE);
         sincos(w * ((step + 0.5) * a.dt), &sH, &cH);
     }
-    for (int s = 0; s < a.ncomp; ++s) {
-        const int c = a.comps[s];
-        const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
-        const long o = ((long)s * a.nf + f) * a.ncell + cell;
-        double2 sm = a.acc[o], cp = a.cmp[o];
-        neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
-        neumaier(sm.y, cp.y, v * (c < 3 ? sE : sH));
-        a.acc[o] = sm;
-        a.cmp[o] = cp;
-    }
+    const int c = a.comps[s];
+    const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
+    const long o = ((long)s * a.nf + f) * a.ncell + cell;
+    double2 sm = a.acc[o], cp = a.cmp[o];
+    neumaier(sm.x, cp.x, v * (c < 3 ? cE : cH));
+    neumaier(sm.y, cp.y, v * (c < 3 ? sE : sH));
+    a.acc[o] = sm;
+    a.cmp[o] = cp;
 }
 
 // ===================================================== adjoint (reverse-mode) kernels ======
