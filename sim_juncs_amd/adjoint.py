@@ -4,6 +4,9 @@ design-region permittivity.
     J = GapFieldObjective.apply(eps, scn, steps, checkpoint_every, gap)
     J.backward()          # eps.grad = dJ/d eps from the discrete adjoint (sjf_adjoint_run)
 
+    J = BatchedGapFieldObjective.apply(eps_stack, scns, steps, gaps)   # one J per variant
+    J.sum().backward()    # eps_stack.grad from one batched adjoint run (sjf_batch_adjoint_run)
+
 `eps` is a tensor shaped like the scenario's design box (any device; values move through host
 memory into the engine).  The forward runs the whole simulation once on the GPU with state
 checkpoints and the backward pass of the adjoint in the same call; autograd's backward only
@@ -13,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .engine import Engine
+from .engine import Batch, Engine
 from .scenario import Scenario
 
 
@@ -29,6 +32,27 @@ def run_adjoint(scn: Scenario, eps: np.ndarray, steps: int, checkpoint_every: in
     finally:
         eng.close()
     return J, g.reshape(np.asarray(eps).shape)
+
+
+def run_adjoint_batch(scns, eps, steps: int, gaps, precision: str | None = None):
+    """(J per scenario, list of dJ/d eps) for scenario i with design permittivities eps[i], all
+    in one batched adjoint run (record mode; the scenarios share nz and their materials' pole
+    count)."""
+    engs = []
+    try:
+        for s0, e in zip(scns, eps):
+            s = Scenario(**{**s0.__dict__})
+            s.design = (s0.design[0], s0.design[1], np.asarray(e, dtype=np.float64))
+            engs.append(s.make(Engine, precision=precision or s0.precision))
+        b = Batch(engs)
+        try:
+            J, g = b.adjoint_run(steps, gaps, [np.asarray(e).size for e in eps])
+        finally:
+            b.close()
+    finally:
+        for e in engs:
+            e.close()
+    return J, [gi.reshape(np.asarray(e).shape) for gi, e in zip(g, eps)]
 
 
 try:
@@ -48,5 +72,22 @@ try:
             (g,) = ctx.saved_tensors
             return grad_out.to(g.dtype) * g, None, None, None, None
 
+    class BatchedGapFieldObjective(torch.autograd.Function):
+        """J_i = dt * sum_n |E^n(gap_i)|^2 of design variant i, for a stack of design
+        permittivity tensors eps[i] (all of one shape), from one batched adjoint run."""
+
+        @staticmethod
+        def forward(ctx, eps, scns, steps, gaps):
+            J, g = run_adjoint_batch(scns, list(eps.detach().cpu().double().numpy()), steps, gaps)
+            ctx.save_for_backward(torch.as_tensor(np.stack(g), dtype=eps.dtype, device=eps.device))
+            return torch.as_tensor(J, dtype=torch.float64, device=eps.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            w = grad_out.to(g.dtype).reshape((-1,) + (1,) * (g.dim() - 1))
+            return w * g, None, None, None
+
 except ImportError:  # torch is optional for the engine itself
     GapFieldObjective = None
+    BatchedGapFieldObjective = None
