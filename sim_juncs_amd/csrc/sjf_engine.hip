@@ -1390,14 +1390,15 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
             g.ew = p ? 6 : 0;
         }
     void *d_tab = nullptr, *d_args0 = nullptr, *d_args1 = nullptr, *d_first = nullptr;
-    if ((rc = upload_vec(e0, s->stream, &d_tab, tab))) return bail(rc);
-    mem.push_back(d_tab);
-    if ((rc = upload_vec(e0, s->stream, &d_args0, argp[0]))) return bail(rc);
-    mem.push_back(d_args0);
-    if ((rc = upload_vec(e0, s->stream, &d_args1, argp[1]))) return bail(rc);
-    mem.push_back(d_args1);
-    if ((rc = upload_vec(e0, s->stream, &d_first, first))) return bail(rc);
-    mem.push_back(d_first);
+    auto upload = [&](void** dst, auto const& v) -> int {
+        *dst = nullptr;
+        const int r = upload_vec(e0, s->stream, dst, v);
+        if (*dst) mem.push_back(*dst);  // freed with the rest, also when the copy failed
+        return r;
+    };
+    if ((rc = upload(&d_tab, tab)) || (rc = upload(&d_args0, argp[0])) || (rc = upload(&d_args1, argp[1])) ||
+        (rc = upload(&d_first, first)))
+        return bail(rc);
     const auto* tabp = (const sjf::AdjBatchDev<T>*)d_tab;
     const sjf::AdjArgs<T>* argd[2] = {(const sjf::AdjArgs<T>*)d_args0, (const sjf::AdjArgs<T>*)d_args1};
     const int block = ((e0->g.nz + 63) / 64) * 64;
@@ -1442,8 +1443,8 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
     };
     const dim3 grid_g((unsigned)((maxv - 3 + 255) / 256), (unsigned)n);
     long ntimed = 0;
-    if ((rc = inject(nsteps))) return bail_ev(rc);
-    for (long q = nsteps - 1; q >= 0; --q) {
+    // one reverse step: gradient term, fused reverse launch (timed for the first kTimed steps)
+    auto reverse_step = [&](long q) -> int {
         hipLaunchKernelGGL((sjf::k_adj_grad_batch<T>), grid_g, dim3(256), 0, s->stream, tabp, rd, q + 1, q);
         SETCHK(s, hipGetLastError());
         const bool timed = ntimed < kTimed;
@@ -1457,6 +1458,11 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
         }
         SETCHK(s, hipGetLastError());
         if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed++ + 1], s->stream));
+        return SJF_OK;
+    };
+    if ((rc = inject(nsteps))) return bail_ev(rc);
+    for (long q = nsteps - 1; q >= 0; --q) {
+        if ((rc = reverse_step(q))) return bail_ev(rc);
         rd = 1 - rd;
         if (q >= 1 && (rc = inject(q))) return bail_ev(rc);
     }
