@@ -158,6 +158,13 @@ int sjf_add_dft(sjf_engine* e, const int lo[3], const int hi[3], int comp_mask,
                 const double* freqs, int nf, int* id);
 /* out[c][f][cell][2] for the masked components in ascending order. */
 int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap);
+/* Flux through DFT region `id`, a plane one cell thick along `axis` that holds both tangential
+ * E and both tangential H components: flux[f] = dx^2 * Re sum_cells (E_a conj(H_b) - E_b conj(H_a))
+ * with (axis, a, b) in cyclic order, the Poynting flux along +axis of the accumulated spectra
+ * (each component at its own Yee point, not interpolated), reduced on the device per frequency
+ * in a fixed order; cap >= the region's frequencies.  New (the reference takes no flux: the
+ * derived monitor of meep's add_flux / get_fluxes, for the DFT planes of this engine). */
+int sjf_dft_flux(sjf_engine* e, int id, int axis, double* flux, size_t cap);
 
 /* ---- time stepping ---------------------------------------------------------------------- */
 

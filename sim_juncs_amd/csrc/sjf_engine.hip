@@ -1984,6 +1984,29 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap) {
     return SJF_OK;
 }
 
+int sjf_dft_flux(sjf_engine* e, int id, int axis, double* flux, size_t cap) {
+    if (!e || !flux || id < 0 || id >= (int)e->dft.size() || axis < 0 || axis > 2) return SJF_E_ARG;
+    const Dft& d = e->dft[id];
+    if (d.hi[axis] - d.lo[axis] != 1) return fail(e, SJF_E_ARG, "a flux plane is one cell thick along its normal");
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    int slot[6] = {-1, -1, -1, -1, -1, -1};
+    for (int q = 0; q < d.ncomp; ++q) slot[d.comps[q]] = q;
+    if (slot[a] < 0 || slot[b] < 0 || slot[3 + a] < 0 || slot[3 + b] < 0)
+        return fail(e, SJF_E_ARG, "a flux plane needs both tangential E and both tangential H components");
+    const int nf = (int)d.freqs.size();
+    if (cap < (size_t)nf) return fail(e, SJF_E_ARG, "output buffer too small");
+    if (int rc = sync_engine(e)) return rc;
+    double* dout = nullptr;
+    if (int rc = dev_alloc(e, (void**)&dout, sizeof(double) * nf)) return rc;
+    hipLaunchKernelGGL(sjf::k_dft_flux, dim3(nf), dim3(256), 0, e->stream, (const double2*)d.acc,
+                       (const double2*)d.cmp, d.ncell, nf, slot[a], slot[b], slot[3 + a], slot[3 + b],
+                       e->g.dx * e->g.dx, dout);
+    int rc = hipGetLastError() == hipSuccess ? SJF_OK : fail(e, SJF_E_HIP, "k_dft_flux launch failed");
+    if (!rc) rc = d2h(e, flux, dout, sizeof(double) * nf);
+    dev_free(dout);
+    return rc;
+}
+
 int sjf_step(sjf_engine* e, long n) {
     if (!e || n < 0) return SJF_E_ARG;
     if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");

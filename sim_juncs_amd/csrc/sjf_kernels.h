@@ -1254,6 +1254,37 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, con
     a.cmp[o] = cp;
 }
 
+// ---- flux through a DFT plane ------------------------------------------------------------
+// out[f] = dA * Re sum_cells (E_a conj(H_b) - E_b conj(H_a)) for frequency f = blockIdx.x, with
+// each spectrum value acc + cmp of the region's [slot][f][cell] accumulators (sa, sb, ha, hb:
+// the slots of E_a, E_b, H_a, H_b).  Each thread sums a strided share of the cells in FP64; the
+// wavefronts reduce by shuffles and the first thread adds the wavefront sums in order, so the
+// result does not vary from run to run.
+__global__ void __launch_bounds__(256) k_dft_flux(const double2* __restrict__ acc, const double2* __restrict__ cmp,
+                                                  long ncell, int nf, int sa, int sb, int ha, int hb, double dA,
+                                                  double* __restrict__ out) {
+    const int f = blockIdx.x;
+    auto value = [&](int slot, long c) {
+        const long o = ((long)slot * nf + f) * ncell + c;
+        const double2 a = acc[o], b = cmp[o];
+        return make_double2(a.x + b.x, a.y + b.y);
+    };
+    double sum = 0;
+    for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const double2 ea = value(sa, c), eb = value(sb, c), Ha = value(ha, c), Hb = value(hb, c);
+        sum += (ea.x * Hb.x + ea.y * Hb.y) - (eb.x * Ha.x + eb.y * Ha.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += part[w];
+        out[f] = t * dA;
+    }
+}
+
 // ===================================================== adjoint (reverse-mode) kernels ======
 // Exact transpose of one k_step update (Yee curls, CPML recursions, (P, Q) pole recursions),
 // applied backwards in time to the adjoint state (Ebar, Hbar, Qbar, Pbar, psiEbar, psiHbar).

@@ -205,6 +205,13 @@ class Engine:
         self._chk(self.lib.sjf_read_dft(self.h, int(i), _dp(out), out.size), "sjf_read_dft")
         return out[..., 0] + 1j * out[..., 1]
 
+    def dft_flux(self, i: int, axis: int) -> np.ndarray:
+        """Flux along +axis through DFT plane i, one value per frequency (sjf_dft_flux)."""
+        nf = self._dft[i][1]
+        out = np.zeros(nf)
+        self._chk(self.lib.sjf_dft_flux(self.h, int(i), int(axis), _dp(out), out.size), "sjf_dft_flux")
+        return out
+
     # ------------------------------------------------------------------ stepping ----------
     def step(self, n: int = 1):
         self._chk(self.lib.sjf_step(self.h, int(n)), "sjf_step")
