@@ -187,7 +187,8 @@ def adjoint_batched(a):
     cells, _ = b.info()
     bytes_rev = sum(e.step_bytes() for e in engs)
     t = time.perf_counter()
-    J, grads = b.adjoint_run(steps, [s.meta["gap"] for s in scns], [s.design[2].size for s in scns])
+    J, grads = b.adjoint_run(steps, [s.meta["gap"] for s in scns], [s.design[2].size for s in scns],
+                             a.checkpoint_every)
     wall = time.perf_counter() - t
     ms, threads, groups = b.adjoint_kernel_info()
     log(f"{len(scns)} variants: J {J.min():.4e}..{J.max():.4e}; reverse launch {ms * 1e3:.1f} us over "
@@ -204,7 +205,8 @@ def adjoint_batched(a):
         cpu["value"] /= 2.0
         cpu["sample"] += "; divided by 2 for the forward and reverse sweeps"
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
-                config={"workload": workload, "cells": cells, "precision": "fp32", "checkpoint_every": 0,
+                config={"workload": workload, "cells": cells, "precision": "fp32",
+                        "checkpoint_every": a.checkpoint_every,
                         "launch": f"per step one fused forward launch ({fgroups} workgroups of {fthreads}) and "
                                   f"one fused reverse launch over all {len(scns)} variants",
                         "note": "value counts forward steps per second of a full gradient computation"},

@@ -251,17 +251,18 @@ int sjf_batch_info(const sjf_batch* b, long* cells, double* bytes_per_step);
 /* Threads per workgroup and workgroups of the batch's fused launch as last built (SJF_E_STATE
  * before the first step). */
 int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups);
-/* sjf_adjoint_run for every engine of the batch in one pass, record mode only: the forward
- * steps are the batch's fused launches, and each reverse step is one fused reverse launch over
- * every engine's tiles (plus one gradient and one injection launch for all engines).  Engine i
- * gets J[i] and its gradient in grads[i] (caps[i] >= its design voxels), with objective cell
- * gaps[3i..3i+2]; each result equals that engine's own sjf_adjoint_run.  Every engine needs a
- * design region and step 0, and all share nz and the pole-slot variant (the same materials
- * with poles, or none); SJF_E_ARG otherwise.  SJF_E_NOMEM when the record of all engines
- * exceeds a quarter of the free device memory (sjf_adjoint_run can checkpoint instead).
+/* sjf_adjoint_run for every engine of the batch in one pass: the forward steps are the batch's
+ * fused launches, and each reverse step is one fused reverse launch over every engine's tiles
+ * (plus one gradient and one injection launch for all engines).  checkpoint_every as in
+ * sjf_adjoint_run (0: record mode when the record of all engines fits a quarter of the free
+ * device memory, else checkpoints every ~sqrt(nsteps) steps).  Engine i gets J[i] and its
+ * gradient in grads[i] (caps[i] >= its design voxels), with objective cell gaps[3i..3i+2]; each
+ * result equals that engine's own sjf_adjoint_run, and every engine ends in its final forward
+ * state.  Every engine needs a design region and step 0, and all share nz and the pole-slot
+ * variant (the same materials with poles, or none); SJF_E_ARG otherwise.
  * New (the batched design-variant gradients). */
-int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, const int* gaps, double* J, double* const* grads,
-                          const size_t* caps);
+int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, int checkpoint_every, const int* gaps, double* J,
+                          double* const* grads, const size_t* caps);
 /* The fused reverse launch of the last sjf_batch_adjoint_run, as sjf_adjoint_kernel_info. */
 int sjf_batch_adjoint_kernel_info(const sjf_batch* b, double* ms_mean, int* threads, int* groups);
 const char* sjf_batch_last_error(const sjf_batch* b);

@@ -125,7 +125,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, dp, C.c_size_t]),
         "sjf_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),
         "sjf_batch_launch_info": (C.c_int, [vp, ip, ip]),
-        "sjf_batch_adjoint_run": (C.c_int, [vp, C.c_long, ip, dp, C.POINTER(dp), C.POINTER(C.c_size_t)]),
+        "sjf_batch_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, C.POINTER(dp), C.POINTER(C.c_size_t)]),
         "sjf_batch_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),
         "sjf_dft_flux": (C.c_int, [vp, C.c_int, C.c_int, dp, C.c_size_t]),
     }

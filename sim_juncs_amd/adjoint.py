@@ -34,10 +34,9 @@ def run_adjoint(scn: Scenario, eps: np.ndarray, steps: int, checkpoint_every: in
     return J, g.reshape(np.asarray(eps).shape)
 
 
-def run_adjoint_batch(scns, eps, steps: int, gaps, precision: str | None = None):
+def run_adjoint_batch(scns, eps, steps: int, gaps, precision: str | None = None, checkpoint_every: int = 0):
     """(J per scenario, list of dJ/d eps) for scenario i with design permittivities eps[i], all
-    in one batched adjoint run (record mode; the scenarios share nz and their materials' pole
-    count)."""
+    in one batched adjoint run (the scenarios share nz and their materials' pole count)."""
     engs = []
     try:
         for s0, e in zip(scns, eps):
@@ -46,7 +45,7 @@ def run_adjoint_batch(scns, eps, steps: int, gaps, precision: str | None = None)
             engs.append(s.make(Engine, precision=precision or s0.precision))
         b = Batch(engs)
         try:
-            J, g = b.adjoint_run(steps, gaps, [np.asarray(e).size for e in eps])
+            J, g = b.adjoint_run(steps, gaps, [np.asarray(e).size for e in eps], checkpoint_every)
         finally:
             b.close()
     finally:

@@ -1016,11 +1016,13 @@ StateLayout state_layout(const sjf_engine* e) {
     return L;
 }
 
-int copy_state(sjf_engine* e, char* ck, const StateLayout& L, bool save) {
+// (on the engine's stream, or on `st` when given: a batch's stream)
+int copy_state(sjf_engine* e, char* ck, const StateLayout& L, bool save, hipStream_t st = nullptr) {
+    if (!st) st = e->stream;
     auto cp = [&](void* dev, size_t off, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
-        return save ? hipMemcpyAsync(ck + off, dev, bytes, hipMemcpyDeviceToDevice, e->stream)
-                    : hipMemcpyAsync(dev, ck + off, bytes, hipMemcpyDeviceToDevice, e->stream);
+        return save ? hipMemcpyAsync(ck + off, dev, bytes, hipMemcpyDeviceToDevice, st)
+                    : hipMemcpyAsync(dev, ck + off, bytes, hipMemcpyDeviceToDevice, st);
     };
     size_t off = 0;
     HIPCHK(e, cp(e->Fbuf[e->step & 1], off, L.fields));
@@ -1287,26 +1289,36 @@ void launch_adj_batch_np(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* f
                            n);
 }
 
-// Adjoint gradients of every engine of a launch set in one pass, in record mode: the forward
-// steps are the set's own fused launches; after each of them one launch records the design
-// fields of all engines and adds their objectives; each reverse step is one gradient launch, one
-// fused reverse launch over all engines' tiles and one injection launch.  The arithmetic of every
-// cell is that of adjoint_run_t, so each engine's J and gradient are those of its own run.
-// gaps[i]: objective cell index of engine i; nds[i]: its design voxels.
+// Adjoint gradients of every engine of a launch set in one pass: the forward steps are the set's
+// own fused launches; after each of them one launch adds the objectives of all engines (and, in
+// record mode, records their design fields); each reverse step is one gradient launch, one fused
+// reverse launch over all engines' tiles and one injection launch.  With checkpoints (K > 0, or
+// when the record does not fit a quarter of the free memory) every engine's state is saved every
+// K steps and each segment is recomputed with bare batch steps, as adjoint_run_t does for one
+// engine.  The arithmetic of every cell is that of adjoint_run_t, so each engine's J and gradient
+// are those of its own run.  gaps[i]: objective cell index of engine i; nds[i]: its design voxels.
 template <typename T>
-int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long* nds, double* J_out,
+int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, const long* nds, double* J_out,
                         double* const* grads, double* ms_out, int* threads_out, int* groups_out) {
     sjf_engine* e0 = s->eng[0];
     const int n = (int)s->eng.size();
     int rc = prepare_set(s, nsteps);
     if (rc) return rc;
-    double rec_bytes = 0;
-    for (int i = 0; i < n; ++i) rec_bytes += (double)sizeof(T) * (3 * nds[i] + 3) * (nsteps + 1);
-    size_t fr = 0, tot = 0;
-    HIPCHK(e0, hipMemGetInfo(&fr, &tot));
-    if (rec_bytes > 0.25 * (double)fr)
-        return fail(e0, SJF_E_NOMEM, "the design-field record of the batch (" + std::to_string(rec_bytes / 1e9) +
-                                         " GB) exceeds a quarter of the free device memory");
+    bool rec_all = K == 0;
+    if (rec_all) {
+        double rec_bytes = 0;
+        for (int i = 0; i < n; ++i) rec_bytes += (double)sizeof(T) * (3 * nds[i] + 3) * (nsteps + 1);
+        size_t fr = 0, tot = 0;
+        HIPCHK(e0, hipMemGetInfo(&fr, &tot));
+        rec_all = rec_bytes <= 0.25 * (double)fr;
+        if (!rec_all) K = std::max(1L, std::lround(std::sqrt((double)nsteps)));
+    }
+    const long nck = rec_all ? 0 : (nsteps + K - 1) / K;
+    // record slots per engine; with checkpoints slot K + 1 takes the forward pass's writes, which
+    // nothing reads (that pass only needs the objectives)
+    const long nslots = rec_all ? nsteps + 1 : K + 2;
+    std::vector<StateLayout> Ls(n);
+    std::vector<char*> cks(n, nullptr);
     std::vector<void*> mem;
     auto cleanup = [&]() {
         for (void*& p : mem) dev_free(p);
@@ -1335,9 +1347,13 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
         void *A = nullptr, *Aq = nullptr, *Ap = nullptr, *ring = nullptr, *gacc = nullptr, *dJ = nullptr;
         void* Apsi[3] = {nullptr, nullptr, nullptr};
         if (alloc(&A, sizeof(T) * 12 * e->N) || (L.q && alloc(&Aq, L.q)) || (L.p && alloc(&Ap, L.p)) ||
-            alloc(&ring, sizeof(T) * (3 * nd + 3) * (nsteps + 1)) || alloc(&gacc, sizeof(double) * nd) ||
+            alloc(&ring, sizeof(T) * (3 * nd + 3) * nslots) || alloc(&gacc, sizeof(double) * nd) ||
             alloc(&dJ, sizeof(double)))
             return bail(SJF_E_NOMEM);
+        Ls[i] = L;
+        if (!rec_all && alloc((void**)&cks[i], L.total * (nck + 1)))
+            return bail(fail(e0, SJF_E_NOMEM, "checkpoint storage allocation failed (" +
+                                                  std::to_string((double)L.total * (nck + 1) / 1e9) + " GB)"));
         for (int d = 0; d < 3; ++d) {
             if ((double)L.psi[d] / 6 * 8 >= 2147483648.0)
                 return bail(fail(e0, SJF_E_UNSUPPORTED, "adjoint PML auxiliaries exceed 2 GiB on one axis"));
@@ -1412,13 +1428,22 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
         SETCHK(s, hipGetLastError());
         return SJF_OK;
     };
-    // ---- forward pass, recording after every step
-    if ((rc = record(0, 0))) return bail(rc);
+    auto save_all = [&](long k, bool save) -> int {
+        for (int i = 0; i < n; ++i) {
+            const int r = copy_state(s->eng[i], cks[i] + Ls[i].total * k, Ls[i], save, s->stream);
+            if (r) return r;
+        }
+        return SJF_OK;
+    };
+    // ---- forward pass: the objectives after every step; the record or checkpoints every K steps
+    if (rec_all && (rc = record(0, 0))) return bail(rc);
     for (long q = 0; q < nsteps; ++q) {
+        if (!rec_all && q % K == 0 && (rc = save_all(q / K, true))) return bail(rc);
         if ((rc = launch_one_step(s, samples_at(s, e0->step)))) return bail(rc);
         advance_hosts(s, 1);
-        if ((rc = record(q + 1, 1))) return bail(rc);
+        if ((rc = record(rec_all ? q + 1 : K + 1, 1))) return bail(rc);
     }
+    if (!rec_all && (rc = save_all(nck, true))) return bail(rc);
     // ---- reverse pass
     constexpr int kTimed = 64;
     std::vector<hipEvent_t> evs;
@@ -1444,8 +1469,9 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
     const dim3 grid_g((unsigned)((maxv - 3 + 255) / 256), (unsigned)n);
     long ntimed = 0;
     // one reverse step: gradient term, fused reverse launch (timed for the first kTimed steps)
-    auto reverse_step = [&](long q) -> int {
-        hipLaunchKernelGGL((sjf::k_adj_grad_batch<T>), grid_g, dim3(256), 0, s->stream, tabp, rd, q + 1, q);
+    auto reverse_step = [&](long q, long base) -> int {
+        hipLaunchKernelGGL((sjf::k_adj_grad_batch<T>), grid_g, dim3(256), 0, s->stream, tabp, rd, q + 1 - base,
+                           q - base);
         SETCHK(s, hipGetLastError());
         const bool timed = ntimed < kTimed;
         if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed], s->stream));
@@ -1460,11 +1486,38 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
         if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed++ + 1], s->stream));
         return SJF_OK;
     };
-    if ((rc = inject(nsteps))) return bail_ev(rc);
-    for (long q = nsteps - 1; q >= 0; --q) {
-        if ((rc = reverse_step(q))) return bail_ev(rc);
-        rd = 1 - rd;
-        if (q >= 1 && (rc = inject(q))) return bail_ev(rc);
+    // reverse steps q1 - 1 .. q0 with the record slots of steps q0 .. q1 at slot (q - base)
+    auto reverse = [&](long q0, long q1, long base) -> int {
+        for (long q = q1 - 1; q >= q0; --q) {
+            int r = reverse_step(q, base);
+            if (r) return r;
+            rd = 1 - rd;
+            if (q >= 1 && (r = inject(q - base))) return r;
+        }
+        return SJF_OK;
+    };
+    auto set_steps = [&](long step) -> int {
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(e0, SJF_E_HIP, "hipStreamSynchronize failed");
+        for (sjf_engine* e : s->eng) {
+            const int r = set_step(e, step);
+            if (r) return r;
+        }
+        return SJF_OK;
+    };
+    if (rec_all) {
+        if ((rc = inject(nsteps)) || (rc = reverse(0, nsteps, 0))) return bail_ev(rc);
+    } else {
+        for (long k = nck - 1; k >= 0; --k) {
+            const long q0 = k * K, q1 = std::min(q0 + K, nsteps);
+            if ((rc = set_steps(q0)) || (rc = save_all(k, false)) || (rc = record(0, 0))) return bail_ev(rc);
+            for (long q = q0; q < q1; ++q) {
+                if ((rc = launch_bare_step(s))) return bail_ev(rc);
+                for (sjf_engine* e : s->eng) e->step++;
+                if ((rc = record(q + 1 - q0, 0))) return bail_ev(rc);
+            }
+            if (k == nck - 1 && (rc = inject(q1 - q0))) return bail_ev(rc);
+            if ((rc = reverse(q0, q1, q0))) return bail_ev(rc);
+        }
     }
     if (hipStreamSynchronize(s->stream) != hipSuccess)
         return bail_ev(fail(e0, SJF_E_HIP, "hipStreamSynchronize failed"));
@@ -1486,6 +1539,11 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, const long* gaps, const long*
             return bail(rc);
         // the state is Ubar = Ebar / eps, so the sums already carry the 1/eps_v factor
         for (long v = 0; v < nds[i]; ++v) grads[i][v] = -gh[v];
+    }
+    // leave every engine in its final forward state (record mode never left it)
+    if (!rec_all) {
+        if ((rc = set_steps(nsteps)) || (rc = save_all(nck, false))) return bail(rc);
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(e0, SJF_E_HIP, "hipStreamSynchronize failed"));
     }
     cleanup();
     return SJF_OK;
@@ -2214,9 +2272,9 @@ int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups) {
     return SJF_OK;
 }
 
-int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, const int* gaps, double* J, double* const* grads,
-                          const size_t* caps) {
-    if (!b || nsteps < 1 || !gaps || !J || !grads || !caps) return SJF_E_ARG;
+int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, int checkpoint_every, const int* gaps, double* J,
+                          double* const* grads, const size_t* caps) {
+    if (!b || nsteps < 1 || checkpoint_every < 0 || !gaps || !J || !grads || !caps) return SJF_E_ARG;
     LaunchSet* s = b->set;
     sjf_engine* e0 = s->eng[0];
     const int n = (int)s->eng.size();
@@ -2247,8 +2305,10 @@ int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, const int* gaps, double* J,
     double ms = 0;
     int th = 0, gr = 0;
     const int rc = s->prec == 4
-                       ? batch_adjoint_run_t<float>(s, nsteps, q.data(), nd.data(), J, grads, &ms, &th, &gr)
-                       : batch_adjoint_run_t<double>(s, nsteps, q.data(), nd.data(), J, grads, &ms, &th, &gr);
+                       ? batch_adjoint_run_t<float>(s, nsteps, checkpoint_every, q.data(), nd.data(), J, grads, &ms,
+                                                    &th, &gr)
+                       : batch_adjoint_run_t<double>(s, nsteps, checkpoint_every, q.data(), nd.data(), J, grads, &ms,
+                                                     &th, &gr);
     if (rc) {
         b->err = e0->err;
         return rc;

@@ -328,10 +328,10 @@ class Batch:
         self._chk(self.lib.sjf_batch_launch_info(self.h, C.byref(th), C.byref(gr)), "sjf_batch_launch_info")
         return th.value, gr.value
 
-    def adjoint_run(self, nsteps: int, gaps, n_design):
-        """Engine.adjoint_run (record mode) for every engine in one pass: one fused reverse launch
-        per step over all engines.  gaps: one (i, j, k) per engine; n_design: design voxels per
-        engine.  Returns (J per engine, list of gradients)."""
+    def adjoint_run(self, nsteps: int, gaps, n_design, checkpoint_every: int = 0):
+        """Engine.adjoint_run for every engine in one pass: one fused reverse launch per step over
+        all engines.  gaps: one (i, j, k) per engine; n_design: design voxels per engine;
+        checkpoint_every as in Engine.adjoint_run.  Returns (J per engine, list of gradients)."""
         n = len(self.engines)
         if len(gaps) != n or len(n_design) != n:
             raise ValueError("one gap cell and one design size per engine")
@@ -340,7 +340,8 @@ class Batch:
         grads = [np.zeros(int(m)) for m in n_design]
         gp = (C.POINTER(C.c_double) * n)(*[_dp(x) for x in grads])
         caps = (C.c_size_t * n)(*[x.size for x in grads])
-        self._chk(self.lib.sjf_batch_adjoint_run(self.h, int(nsteps), _ip(g.ravel()), _dp(J), gp, caps),
+        self._chk(self.lib.sjf_batch_adjoint_run(self.h, int(nsteps), int(checkpoint_every), _ip(g.ravel()), _dp(J),
+                                                 gp, caps),
                   "sjf_batch_adjoint_run")
         return J, grads
 

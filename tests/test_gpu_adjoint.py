@@ -245,7 +245,7 @@ BATCH_MEMBERS = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["register_rows", "default"])
+@pytest.mark.parametrize("mode", ["register_rows", "default", "checkpoints"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     """sjf_batch_adjoint_run over engines of different shapes, design boxes, objective cells and
@@ -254,7 +254,8 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     engine in its final forward state.  The batch's forward launch runs one tile form for all
     engines, and the forms round differently in the last bit, so each mode runs the same form
     alone and in the batch: register rows with each engine's x-chunk (which the reverse step
-    then takes too), or the default row-of-threads form (the reverse step's default chunks)."""
+    then takes too), or the default row-of-threads form (the reverse step's default chunks),
+    also with the batch checkpointing every 13 steps and recomputing (the engines alone record)."""
     from sim_juncs_amd.engine import Batch
 
     steps = 60
@@ -281,7 +282,8 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
         b.set_tile("register_rows", 0)
     with pytest.raises(EngineError):
         b.adjoint_kernel_info()
-    J, grads = b.adjoint_run(steps, [gap for _, gap, _ in cases], [eps.size for _, _, eps in cases])
+    J, grads = b.adjoint_run(steps, [gap for _, gap, _ in cases], [eps.size for _, _, eps in cases],
+                             13 if mode == "checkpoints" else 0)
     ms, threads, groups = b.adjoint_kernel_info()
     b.close()
     assert 0 < ms < 100 and threads == 64
