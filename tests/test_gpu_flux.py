@@ -65,3 +65,36 @@ def test_flux_needs_a_plane_with_the_tangential_components():
         eng.dft_flux(ok, 0)  # along x the plane is 8 cells thick
     assert eng.dft_flux(ok, 2).shape == (len(FREQS),)
     eng.close()
+
+
+def test_flux_through_a_metal_slab_matches_the_oracle():
+    """A 2-cell Au slab (Drude + Lorentz) between two planes: the flux spectra equal the
+    oracle's (its own D/E-split stepper and DFT, the flux summed from its spectra) to 1e-7, and
+    the slab is passive, 0 < transmitted < net incident."""
+    from oracle.oracle import Oracle
+
+    mat = np.zeros(SHAPE + (3,), np.uint8)
+    scenario.paint(mat, SHAPE, scenario.box_inside((-1, -1, 64.5), (40, 40, 66.5)), 1)
+    lut = [M.VACUUM, M.AU.scaled(1.0)]
+    src = scenario.plane_source(SHAPE, DX, 30, scenario.gaussian_pulse(0.8, 1.5, 0.0, start_fs=0.0))
+    res = []
+    for kind in (Engine, Oracle):
+        sim = kind(SHAPE, DX, precision="fp64", periodic=(1, 1, 0))
+        sim.set_materials(scenario.pack(mat), lut)
+        sim.set_pml([0, 0, 0, 0, 12, 12])
+        sim.add_source(**src)
+        planes = [sim.add_dft((0, 0, k), (SHAPE[0], SHAPE[1], k + 1), TANGENTIAL_Z, FREQS) for k in (50, 90)]
+        dt = sim.dt() if callable(sim.dt) else sim.dt
+        sim.step(int((sim.last_source_time() + 3.0) / dt) + 1)
+        if kind is Engine:
+            res.append([sim.dft_flux(i, 2) for i in planes])
+        else:
+            fl = []
+            for i in planes:
+                d = sim.read_dft(i)
+                fl.append(DX * DX * np.real(d[0] * np.conj(d[3]) - d[1] * np.conj(d[2])).sum(axis=(1, 2, 3)))
+            res.append(fl)
+        sim.close()
+    (e50, e90), (o50, o90) = res
+    assert np.allclose(e50, o50, rtol=1e-7, atol=0) and np.allclose(e90, o90, rtol=1e-7, atol=0)
+    assert np.all(e90 > 0) and np.all(e90 < e50)
