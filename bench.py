@@ -1,16 +1,17 @@
 """Throughput of the FDTD time step on one MI355X.
 
-Default workload: the single large junction instance -- 512x512x256 cells of 5 nm, 12-cell
-PML, Au bowtie (Drude + 2 Lorentz) on a SiO2 Lorentz substrate with a graphene Drude sheet,
-FP32 fields, FP64 compensated DFT accumulators at 4 points x 8 frequencies, 800 nm Gaussian
-pulse, 6000 steps.  A "step" is one time step over the whole grid (fused update kernel +
-source + DFT + monitor sampling every 20 steps).  Fields are resident in HBM before timing.
+Default workload (`large`): the largest single-GPU configuration, one 1024x768x512 instance of
+the bowtie junction -- cells of 5 nm, 12-cell PML, Au bowtie (Drude + 2 Lorentz) on a SiO2
+Lorentz substrate with a graphene Drude sheet, FP32 fields, FP64 compensated DFT accumulators
+at 4 points x 8 frequencies, 800 nm Gaussian pulse, 10000 steps.  A "step" is one time step
+over the whole grid (fused update kernel + source + DFT + monitor sampling every 20 steps).
+Fields are resident in HBM before timing.
 
-Other workloads (--workload): plain (96^3 FP64), sweep (64 uneven instances batched),
-adjoint (256x256x128 with a 64x64x16 design region: forward + discrete adjoint), large
-(1024x768x512 FP32 plain run), variants (16 adjoint design variants as one batch: one fused
-forward and one fused reverse launch per step over all of them), variants-sequential (the same
-16 variants, one adjoint run after another).
+Other workloads (--workload): bowtie (the same junction at 512x512x256, 6000 steps), plain
+(96^3 FP64), sweep (64 uneven instances batched), adjoint (256x256x128 with a 64x64x16 design
+region: forward + discrete adjoint), variants (16 adjoint design variants as one batch: one
+fused forward and one fused reverse launch per step over all of them), variants-sequential (the
+same 16 variants, one adjoint run after another).
 
 Prints one JSON line: metric Yee cell-updates/s (cells including PML x steps / wall time),
 the dominant kernel's HBM roofline, and the FP64 CPU oracle timed on this host's cores.
@@ -34,10 +35,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     p.add_argument("--warmup", type=int, default=50)
-    p.add_argument("--workload", default="bowtie",
+    p.add_argument("--workload", default="large",
                    choices=["bowtie", "plain", "sweep", "adjoint", "large", "variants", "variants-sequential"])
     p.add_argument("--cpu-steps", type=int, default=None,
-                   help="oracle steps timed for the CPU baseline (default: about 10 s of work)")
+                   help="oracle steps timed for the CPU baseline (default: about 20 s of work)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-steps", type=int, default=100,
                    help="steps timed with per-launch HIP events for the kernel duration")
@@ -55,8 +56,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scns, steps):
-    """FP64 oracle on this host's cores over `steps` steps of the given instances."""
+def cpu_budget_steps(cells, budget=1.4e10):
+    """Oracle steps of a bounded CPU sample: about 1.4e10 cell-updates (~20 s on 16 cores)."""
+    return max(3, int(budget / cells))
+
+
+def cpu_baseline(scns, steps, of_steps):
+    """FP64 oracle on this host's cores over `steps` steps of the given instances (the workload
+    runs `of_steps`; the rate is per step, so the sample is extrapolated by steps)."""
     from oracle import oracle as orc
 
     cells = 0
@@ -71,8 +78,10 @@ def cpu_baseline(scns, steps):
         o.close()
     return {"value": cells * steps / secs, "unit": "cell-updates/s", "cores": orc.num_threads(),
             "kind": "port",
-            "sample": f"FP64 oracle (build's CPU stepper, not meep) on {len(scns)} instance(s) of the "
-                      f"same workload, {steps} steps each after 1 warm-up step, {secs:.1f} s"}
+            "sample": f"FP64 oracle (build's CPU stepper, not meep) on {len(scns)} full-size instance(s) "
+                      f"of the workload, {steps} of its {of_steps} steps each after 1 warm-up step "
+                      f"({secs:.1f} s, rate extrapolated by steps; the sample is bounded to ~20 s of CPU "
+                      f"work, not the 200 steps a longer run would time)"}
 
 
 def roofline(bytes_step, ms_kernel, kernel, threads, groups, workload):
@@ -82,13 +91,16 @@ def roofline(bytes_step, ms_kernel, kernel, threads, groups, workload):
     traffic ... --store`), reported only when it was measured on the kernel sources being run at
     the same launch shape."""
     from sim_juncs_amd import profsum
+    from sim_juncs_amd.engine import copy_bandwidth
 
     achieved = bytes_step / (ms_kernel * 1e-3) / 1e9
     rec = profsum.stored_traffic(workload, kernel, threads, groups)
+    copy = copy_bandwidth(1 << 30, 20)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": rec["traffic_bytes"] if rec else None,
             "kernel": kernel, "kernel_ms": ms_kernel, "algorithmic_bytes_per_launch": bytes_step,
-            "threads": threads, "groups": groups}
+            "threads": threads, "groups": groups,
+            "measured_copy_gbs": copy, "frac_of_copy": achieved / copy}
 
 
 def set_batch_tile(a, b):
@@ -118,6 +130,7 @@ def single(a, scn, tag):
     steps = a.steps if a.steps is not None else scn.steps
     eng = scn.make(Engine)
     set_tile(a, eng)
+    eng.reserve_steps(a.warmup + steps + a.kernel_steps)  # no allocation in the timed region
     log(f"engine ready {eng.launch_info()}")
     eng.step(a.warmup)
     eng.synchronize()
@@ -132,8 +145,7 @@ def single(a, scn, tag):
     eng.close()
     cpu = None
     if not a.no_cpu:
-        n_cpu = a.cpu_steps or max(3, int(7e9 / scn.cells))  # about 7e9 cell-updates (~10 s)
-        cpu = cpu_baseline([scn], n_cpu)
+        cpu = cpu_baseline([scn], a.cpu_steps or cpu_budget_steps(scn.cells), scn.steps)
     return dict(value=scn.cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": scn.name, "cells": scn.cells, "precision": scn.precision,
                         "dft": "fp64 compensated", "tile": info},
@@ -151,6 +163,7 @@ def sweep(a):
     set_batch_tile(a, b)
     cells, bytes_step = b.info()
     log(f"batch of {len(engs)} engines, {cells} cells")
+    b.reserve_steps(a.warmup + steps + a.kernel_steps)
     b.step(a.warmup)
     b.synchronize()
     t = time.perf_counter()
@@ -163,7 +176,8 @@ def sweep(a):
     for e in engs:
         e.close()
     sample = scns[:4]
-    cpu = None if a.no_cpu else cpu_baseline(sample, a.cpu_steps or max(3, int(7e9 / sum(s.cells for s in sample))))
+    cpu = None if a.no_cpu else cpu_baseline(sample, a.cpu_steps or cpu_budget_steps(sum(s.cells for s in sample)),
+                                             scns[0].steps)
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
                         "launch": "one fused launch per step over all instances, hipGraph replay"},
@@ -173,7 +187,8 @@ def sweep(a):
 
 def adjoint_batched(a):
     """The 16 design variants as one batch: per step one fused forward launch and one fused
-    reverse launch over all of them (sjf_batch_adjoint_run, record mode)."""
+    reverse launch over all of them (sjf_batch_adjoint_run; record mode unless
+    --checkpoint-every K > 0)."""
     from sim_juncs_amd import scenario
     from sim_juncs_amd.engine import Batch, Engine
 
@@ -200,7 +215,7 @@ def adjoint_batched(a):
     s0 = scns[0]
     workload = f"variants_{len(scns)}_batched"
     rl = roofline(bytes_rev, ms, "k_adj_step_batch", threads, groups, workload)
-    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or max(3, int(5e9 / s0.cells)))
+    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or cpu_budget_steps(s0.cells), s0.steps)
     if cpu:
         cpu["value"] /= 2.0
         cpu["sample"] += "; divided by 2 for the forward and reverse sweeps"
@@ -239,7 +254,7 @@ def adjoint(a, variants=False):
     s0 = scns[0]
     workload = ("variants_" if variants else "") + s0.name
     rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], workload)
-    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or max(3, int(5e9 / s0.cells)))
+    cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or cpu_budget_steps(s0.cells), s0.steps)
     if cpu:
         # the adjoint costs a forward and a reverse sweep per step (the design fields of every
         # step are recorded, as a CPU implementation could record them too)

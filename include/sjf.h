@@ -125,8 +125,11 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power);
 /* ---- sources, monitors, DFT ------------------------------------------------------------ */
 
 /* Replaces fields::add_volume_source(c, src, volume, amp) (src/disp.cpp:603,614,619,680) and
- * add_point_source (src/disp.cpp:659, zero-extent box).  comp in Ex..Ez.  box = {x0,y0,z0,
- * x1,y1,z1} in meep units; an axis of zero extent is a delta function (linear weights / dx),
+ * add_point_source (src/disp.cpp:659, zero-extent box).  comp in Ex..Hz: Ex..Ez are electric
+ * currents J added to the E update, E^{n+1} -= dt J^{n+1/2} w / eps, J^{n+1/2} = (dipole(t_{n+1}) -
+ * dipole(t_n)) / dt; Hx..Hz are magnetic currents K added to the H update, H^{n+1/2} -= dt K^n w,
+ * K^n = (dipole(t_n + dt/2) - dipole(t_n - dt/2)) / dt (dB/dt = -curl E - K, mu = 1).  box = {x0,
+ * y0,z0,x1,y1,z1} in meep units; an axis of zero extent is a delta function (linear weights / dx),
  * otherwise cells weigh by their overlap with the box. */
 int sjf_add_source(sjf_engine* e, int comp, const double box[6], const sjf_pulse* p,
                    double amp_re, double amp_im);
@@ -173,6 +176,10 @@ int sjf_dft_flux(sjf_engine* e, int id, int axis, double* flux, size_t cap);
 int sjf_step(sjf_engine* e, long n);
 /* Waits for all work enqueued on the engine's stream. */
 int sjf_synchronize(sjf_engine* e);
+/* Makes the next n steps allocation-free: sizes the monitor sample buffer for them and builds the
+ * launch tables now (sjf_step otherwise does both on demand, which allocates and synchronises).
+ * New (callers that time or capture a run of steps). */
+int sjf_reserve_steps(sjf_engine* e, long n);
 /* Steps per captured hipGraph replayed by sjf_step (rounded up to a multiple of the save span);
  * 0 launches every step directly.  Results are identical either way. */
 int sjf_set_graph(sjf_engine* e, int chunk);
@@ -190,8 +197,11 @@ int sjf_write_field(sjf_engine* e, int comp, const double* in);
 
 /* ---- accounting -------------------------------------------------------------------------- */
 
-/* Algorithmic HBM bytes one launch of the fused update kernel must move (E/H read+write,
- * material word, polarisation state, PML auxiliaries), computed from the actual masks. */
+/* Algorithmic HBM bytes one launch of the fused update kernel must move, from the actual masks:
+ * N (12 p + 4) for E and H read and written and the material word (no word for a uniform
+ * medium), 3 p per pole and dispersive E component (P^n, P^{n-1} read, P^{n+1} written), 8 p per
+ * PML cell and axis (two E and two H auxiliaries read and written) and p per design voxel (its
+ * 1/eps); p = 4 (FP32) or 8 (FP64). */
 int sjf_step_bytes(const sjf_engine* e, double* bytes);
 /* Forms of the fused update kernel.  REGISTER_ROWS: one workgroup per z line, each thread
  * updates `rows` (1, 2 or 4) consecutive y rows and recomputes H on the row below them.
@@ -244,6 +254,8 @@ int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out);
  * (0: each engine's own).  New (tuning of batched runs). */
 int sjf_batch_set_tile(sjf_batch* b, int form, int xchunk);
 int sjf_batch_step(sjf_batch* b, long n);
+/* sjf_reserve_steps for every engine of the batch and the batch's launch tables. */
+int sjf_batch_reserve_steps(sjf_batch* b, long n);
 int sjf_batch_synchronize(sjf_batch* b);
 int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean);
 /* Total cells and algorithmic bytes of one fused launch over the batch. */
@@ -268,9 +280,10 @@ int sjf_batch_adjoint_kernel_info(const sjf_batch* b, double* ms_mean, int* thre
 const char* sjf_batch_last_error(const sjf_batch* b);
 int sjf_batch_destroy(sjf_batch* b);
 
-/* Achievable HBM bandwidth: times `iters` launches of a plain streaming copy of `bytes` bytes
- * on `device` and reports (read + write bytes) / time in GB/s.  New; used as the measured
- * bandwidth ceiling next to the nominal peak and to calibrate traffic counters. */
+/* Achievable HBM bandwidth: times `iters` launches of a plain copy of `bytes` bytes (one 16-byte
+ * load and store per thread) on `device` and reports (read + write bytes) / time in GB/s.  New;
+ * used as the measured bandwidth ceiling next to the nominal peak and to calibrate traffic
+ * counters. */
 int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs);
 
 #ifdef __cplusplus

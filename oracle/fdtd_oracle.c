@@ -282,6 +282,20 @@ static void src_dipole(const orc_src* s, double t, double* re, double* im) {
     else cw_dipole(s, t, re, im);
 }
 
+/* Exposed for tests: dipole(t) of a continuous source (f, width, t_start, t_end, slowness). */
+void orc_cw_dipole(double f, double w, double st, double et, double slow, double t, double* re,
+                   double* im) {
+    orc_src s;
+    memset(&s, 0, sizeof(s));
+    s.kind = 1;
+    s.freq = f;
+    s.cw_width = w;
+    s.t_start = st;
+    s.t_end = et;
+    s.slow = slow;
+    cw_dipole(&s, t, re, im);
+}
+
 /* Exposed for tests: dipole(t) of a gaussian source (f, w, phase, st, et). */
 void orc_gaussian_dipole(double f, double w, double phase, double st, double et, double t,
                          double* re, double* im, double* cutoff_out) {
@@ -339,10 +353,12 @@ static const double COMP_OFF[6][3] = {
 
 /* kind 0 gaussian: params = {f, w, phase, t_start, t_end}
  * kind 1 continuous: params = {f, width, t_start, t_end, slowness}
- * box = {x0, y0, z0, x1, y1, z1} in meep units.  Only E components (0..2). */
+ * box = {x0, y0, z0, x1, y1, z1} in meep units.  comp 0..2: electric current J added to the
+ * D update; comp 3..5: magnetic current K added to the B (= H) update (the reference passes any
+ * component to fields::add_volume_source, src/disp.cpp:614,619). */
 int orc_add_source(orc* o, int comp, const double* box, int kind, const double* prm,
                    double amp_re, double amp_im) {
-    if (comp < 0 || comp > 2) return -1;
+    if (comp < 0 || comp > 5) return -1;
     o->src = (orc_src*)realloc(o->src, sizeof(orc_src) * (o->nsrc + 1));
     orc_src* s = &o->src[o->nsrc++];
     memset(s, 0, sizeof(*s));
@@ -393,8 +409,9 @@ int orc_add_source(orc* o, int comp, const double* box, int kind, const double* 
 double orc_last_source_time(const orc* o) {
     double t = 0;
     for (int s = 0; s < o->nsrc; s++) {
+        /* src_time::last_time rounds to float (gaussian_src_time_phase: src/disp.hpp:119) */
         double lt = o->src[s].kind == 0 ? (double)(float)(o->src[s].peak + o->src[s].cutoff)
-                                          : o->src[s].t_end;
+                                          : (double)(float)o->src[s].t_end;
         if (lt > t) t = lt;
     }
     return t;
@@ -604,10 +621,12 @@ static void update_D(orc* o) {
             }
 }
 
+/* Electric currents: D^{n+1} -= dt J^{n+1/2}, J^{n+1/2} = (dipole(t_{n+1}) - dipole(t_n)) / dt. */
 static void add_sources(orc* o) {
     double t0 = o->step * o->dt, t1 = (o->step + 1) * o->dt;
     for (int s = 0; s < o->nsrc; s++) {
         const orc_src* src = &o->src[s];
+        if (src->comp > 2) continue;
         double r0, i0, r1, i1;
         src_dipole(src, t0, &r0, &i0);
         src_dipole(src, t1, &r1, &i1);
@@ -618,6 +637,25 @@ static void add_sources(orc* o) {
         double J = complex_re_im(ar, ai, o->part);
         double* D = o->D[src->comp];
         for (long q = 0; q < src->n; q++) D[src->idx[q]] -= o->dt * J * src->wt[q];
+    }
+}
+
+/* Magnetic currents, after the H update: H^{n+1/2} -= dt K^n, K^n = (dipole(t_n + dt/2) -
+ * dipole(t_n - dt/2)) / dt (mu = 1; dB/dt = -curl E - K). */
+static void add_h_sources(orc* o) {
+    double t0 = (o->step - 0.5) * o->dt, t1 = (o->step + 0.5) * o->dt;
+    for (int s = 0; s < o->nsrc; s++) {
+        const orc_src* src = &o->src[s];
+        if (src->comp < 3) continue;
+        double r0, i0, r1, i1;
+        src_dipole(src, t0, &r0, &i0);
+        src_dipole(src, t1, &r1, &i1);
+        double kr = (r1 - r0) / o->dt, ki = (i1 - i0) / o->dt;
+        double ar = src->amp_re * kr - src->amp_im * ki;
+        double ai = src->amp_re * ki + src->amp_im * kr;
+        double K = complex_re_im(ar, ai, o->part);
+        double* H = o->F[src->comp];
+        for (long q = 0; q < src->n; q++) H[src->idx[q]] -= o->dt * K * src->wt[q];
     }
 }
 
@@ -660,6 +698,7 @@ void orc_step(orc* o, long nsteps) {
     for (long s = 0; s < nsteps; s++) {
         if (o->step % o->save_span == 0) sample_monitors(o);
         update_H(o);
+        add_h_sources(o);
         update_D(o);
         add_sources(o);
         update_PE(o);

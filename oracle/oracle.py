@@ -56,6 +56,7 @@ def load():
         "orc_set_field": (None, [vp, C.c_int, dp]),
         "orc_step": (None, [vp, C.c_long]),
         "orc_gaussian_dipole": (None, [C.c_double] * 6 + [dp, dp, dp]),
+        "orc_cw_dipole": (None, [C.c_double] * 6 + [dp, dp]),
         "orc_num_threads": (C.c_int, []),
     }
     for name, (res, args) in proto.items():
@@ -84,6 +85,14 @@ def gaussian_dipole(f, w, phase, st, et, t):
     re, im, cut = C.c_double(), C.c_double(), C.c_double()
     lib.orc_gaussian_dipole(f, w, phase, st, et, t, C.byref(re), C.byref(im), C.byref(cut))
     return complex(re.value, im.value), cut.value
+
+
+def cw_dipole(f, w, st, et, slowness, t):
+    """dipole(t) of a continuous source (meep continuous_src_time) as the oracle evaluates it."""
+    lib = load()
+    re, im = C.c_double(), C.c_double()
+    lib.orc_cw_dipole(f, w, st, et, slowness, t, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
 
 
 def num_threads() -> int:

@@ -35,7 +35,7 @@ EXPORTS = [
     "sjf_batch_create", "sjf_batch_set_tile", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
     "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
     "sjf_adjoint_kernel_info", "sjf_batch_launch_info", "sjf_batch_adjoint_run", "sjf_batch_adjoint_kernel_info",
-    "sjf_dft_flux",
+    "sjf_dft_flux", "sjf_reserve_steps", "sjf_batch_reserve_steps",
 ]
 
 
@@ -106,6 +106,8 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_read_dft": (C.c_int, [vp, C.c_int, dp, C.c_size_t]),
         "sjf_step": (C.c_int, [vp, C.c_long]),
         "sjf_synchronize": (C.c_int, [vp]),
+        "sjf_reserve_steps": (C.c_int, [vp, C.c_long]),
+        "sjf_batch_reserve_steps": (C.c_int, [vp, C.c_long]),
         "sjf_timed_steps": (C.c_int, [vp, C.c_long, dp, dp]),
         "sjf_read_field": (C.c_int, [vp, C.c_int, dp]),
         "sjf_write_field": (C.c_int, [vp, C.c_int, dp]),

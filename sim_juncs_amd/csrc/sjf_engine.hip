@@ -328,6 +328,14 @@ struct LaunchSet {
     long* d_src_idx = nullptr;
     double* d_src_coef = nullptr;
     int nsrcblk = 0;
+    // magnetic sources (injected into H before the fused update), same table layout
+    void* d_hsrc = nullptr;
+    int* d_hblk_src = nullptr;
+    long* d_hblk_first = nullptr;
+    long* d_hblk_end = nullptr;
+    long* d_hsrc_idx = nullptr;
+    double* d_hsrc_coef = nullptr;
+    int nhsrcblk = 0;
     // monitors
     void* d_mon = nullptr;
     int nmon = 0;
@@ -355,6 +363,13 @@ void set_free_tables(LaunchSet* s) {
     dev_free(s->d_blk_end);
     dev_free(s->d_src_idx);
     dev_free(s->d_src_coef);
+    dev_free(s->d_hsrc);
+    dev_free(s->d_hblk_src);
+    dev_free(s->d_hblk_first);
+    dev_free(s->d_hblk_end);
+    dev_free(s->d_hsrc_idx);
+    dev_free(s->d_hsrc_coef);
+    s->nsrcblk = s->nhsrcblk = 0;
     dev_free(s->d_mon);
     dev_free(s->d_dft);
     dev_free(s->d_counters);
@@ -416,44 +431,54 @@ int build_set_t(LaunchSet* s) {
     if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_tile_first, first);
     if (rc) return rc;
 
-    // sources: entries of every source of every engine, in blocks of 256 of one source
-    std::vector<sjf::SrcDev<T>> srcs;
-    std::vector<int> blk_src;
-    std::vector<long> blk_first, blk_end, idx;
-    std::vector<double> coef;
-    for (sjf_engine* e : s->eng) {
-        for (const Source& src : e->src) {
-            if (src.idx.empty()) continue;
-            sjf::SrcDev<T> d;
-            std::memset(&d, 0, sizeof(d));
-            d.p = src.dev;
-            d.p.part = e->g.part;
-            d.p.dt = e->dt;
-            for (int b = 0; b < 2; ++b) d.F[b] = (T*)e->Fbuf[b];
-            d.d_step = e->d_step;
-            const int sid = (int)srcs.size();
-            srcs.push_back(d);
-            const long base = (long)idx.size();
-            for (size_t q = 0; q < src.idx.size(); ++q) {
-                idx.push_back(sjf::field_index(src.idx[q], src.comp, (long)e->g.ny * e->g.nz));
-                coef.push_back(e->dt * src.wt[q] * inv_eps_at(e, src.idx[q], src.comp));
-            }
-            const long n = (long)src.idx.size();
-            for (long b = 0; b < n; b += 256) {
-                blk_src.push_back(sid);
-                blk_first.push_back(base + b);
-                blk_end.push_back(base + std::min(n, b + 256));
+    // sources: entries of every source of every engine, in blocks of 256 of one source; electric
+    // sources (coefficient dt * weight / eps) and magnetic ones (dt * weight, mu = 1) in separate
+    // tables
+    for (int mag = 0; mag < 2; ++mag) {
+        std::vector<sjf::SrcDev<T>> srcs;
+        std::vector<int> blk_src;
+        std::vector<long> blk_first, blk_end, idx;
+        std::vector<double> coef;
+        for (sjf_engine* e : s->eng) {
+            for (const Source& src : e->src) {
+                if (src.idx.empty() || (src.comp >= 3) != (mag == 1)) continue;
+                sjf::SrcDev<T> d;
+                std::memset(&d, 0, sizeof(d));
+                d.p = src.dev;
+                d.p.part = e->g.part;
+                d.p.dt = e->dt;
+                for (int b = 0; b < 2; ++b) d.F[b] = (T*)e->Fbuf[b];
+                d.d_step = e->d_step;
+                const int sid = (int)srcs.size();
+                srcs.push_back(d);
+                const long base = (long)idx.size();
+                for (size_t q = 0; q < src.idx.size(); ++q) {
+                    idx.push_back(sjf::field_index(src.idx[q], src.comp, (long)e->g.ny * e->g.nz));
+                    coef.push_back(e->dt * src.wt[q] * (mag ? 1.0 : inv_eps_at(e, src.idx[q], src.comp)));
+                }
+                const long n = (long)src.idx.size();
+                for (long b = 0; b < n; b += 256) {
+                    blk_src.push_back(sid);
+                    blk_first.push_back(base + b);
+                    blk_end.push_back(base + std::min(n, b + 256));
+                }
             }
         }
-    }
-    s->nsrcblk = (int)blk_src.size();
-    if (s->nsrcblk) {
-        rc = upload_vec(e0, s->stream, &s->d_src, srcs);
-        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_src, blk_src);
-        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_first, blk_first);
-        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_blk_end, blk_end);
-        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_src_idx, idx);
-        if (!rc) rc = upload_vec(e0, s->stream, (void**)&s->d_src_coef, coef);
+        const int nblk = (int)blk_src.size();
+        if (!nblk) continue;
+        void** tsrc = mag ? &s->d_hsrc : &s->d_src;
+        int** tbs = mag ? &s->d_hblk_src : &s->d_blk_src;
+        long** tbf = mag ? &s->d_hblk_first : &s->d_blk_first;
+        long** tbe = mag ? &s->d_hblk_end : &s->d_blk_end;
+        long** tidx = mag ? &s->d_hsrc_idx : &s->d_src_idx;
+        double** tcoef = mag ? &s->d_hsrc_coef : &s->d_src_coef;
+        (mag ? s->nhsrcblk : s->nsrcblk) = nblk;
+        rc = upload_vec(e0, s->stream, tsrc, srcs);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)tbs, blk_src);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)tbf, blk_first);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)tbe, blk_end);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)tidx, idx);
+        if (!rc) rc = upload_vec(e0, s->stream, (void**)tcoef, coef);
         if (rc) return rc;
     }
 
@@ -521,7 +546,7 @@ int build_set_t(LaunchSet* s) {
     for (sjf_engine* e : s->eng) {
         const long nyz = (long)e->g.ny * e->g.nz;
         for (const Source& src : e->src)
-            for (const long q : src.idx) {
+            for (const long q : (src.comp < 3 ? src.idx : std::vector<long>())) {
                 const int i = (int)(q / nyz), j = (int)((q % nyz) / e->g.nz), k = (int)(q % e->g.nz);
                 for (const Dft& d : e->dft)
                     if (i >= d.lo[0] && i < d.hi[0] && j >= d.lo[1] && j < d.hi[1] && k >= d.lo[2] && k < d.hi[2])
@@ -609,6 +634,12 @@ int launch_one_step_t(LaunchSet* s, bool sample, hipEvent_t ev_a, hipEvent_t ev_
     if (sample && s->nmon && !bare) {
         hipLaunchKernelGGL((sjf::k_monitor<T>), dim3((s->nmon + 255) / 256), dim3(256), 0, s->stream,
                            (const sjf::MonDev<T>*)s->d_mon, s->nmon);
+        SETCHK(s, hipGetLastError());
+    }
+    if (s->nhsrcblk) {
+        const sjf::SrcTables<T> ht{(const sjf::SrcDev<T>*)s->d_hsrc, s->d_hblk_src, s->d_hblk_first,
+                                   s->d_hblk_end, s->d_hsrc_idx, s->d_hsrc_coef};
+        hipLaunchKernelGGL((sjf::k_source_h<T>), dim3(s->nhsrcblk), dim3(256), 0, s->stream, ht);
         SETCHK(s, hipGetLastError());
     }
     if (ev_a) SETCHK(s, hipEventRecord(ev_a, s->stream));
@@ -760,9 +791,9 @@ int run_set(LaunchSet* s, long n) {
         }
         return SJF_OK;
     };
-    if (K > 0 && n >= 2 * K) {
+    const long lead = K > 0 ? (span - s->eng[0]->step % span) % span : 0;
+    if (K > 0 && n - lead >= K) {
         // align the first replay to the save span, then replay whole chunks
-        const long lead = (span - s->eng[0]->step % span) % span;
         rc = eager(lead);
         if (rc) return rc;
         done += lead;
@@ -972,15 +1003,16 @@ void trilinear(const sjf_engine* e, int comp, const double* xyz, long* idx, doub
 double engine_step_bytes(const sjf_engine* e) {
     const double p = e->prec;
     double b = e->N * (12 * p + (e->uniform ? 0 : 4));
-    // polarisation: Drude pole = Q read+write, Lorentz pole = P and Q read+write
+    // polarisation: per pole and dispersive component 3p, the compulsory state of the textbook
+    // recurrence (read P^n and P^{n-1}, write P^{n+1})
     if (e->have_mat && e->maxp > 0) {
+        long pc = 0;
         for (long q = 0; q < e->N; ++q)
             for (int c = 0; c < 3; ++c) {
                 const int m = (e->matw_h[q] >> (8 * c)) & 0xff;
-                if (m == sjf::kDesignMat) continue;
-                const sjf_material& M = e->lut_h[m];
-                for (int s = 0; s < M.npoles; ++s) b += (M.is_drude[s] ? 2 : 4) * p;
+                if (m != sjf::kDesignMat) pc += e->lut_h[m].npoles;
             }
+        b += 3 * p * (double)pc;
     }
     if (!e->deps_h.empty()) b += p * (double)e->deps_h.size();
     // PML: per cell and PML axis, 2 E + 2 H auxiliaries, each read + written
@@ -1100,6 +1132,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
         rec_all = (double)sizeof(T) * slot * (nsteps + 1) <= 0.25 * (double)fr;
         if (!rec_all) K = std::max(1L, std::lround(std::sqrt((double)nsteps)));
     }
+    K = std::min(K, nsteps);  // a longer interval holds one segment of nsteps all the same
     const long nck = rec_all ? 0 : (nsteps + K - 1) / K;
     const long nslots = rec_all ? nsteps + 1 : K + 1;
     char* cks = nullptr;
@@ -1313,6 +1346,7 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
         rec_all = rec_bytes <= 0.25 * (double)fr;
         if (!rec_all) K = std::max(1L, std::lround(std::sqrt((double)nsteps)));
     }
+    K = std::min(K, nsteps);  // a longer interval holds one segment of nsteps all the same
     const long nck = rec_all ? 0 : (nsteps + K - 1) / K;
     // record slots per engine; with checkpoints slot K + 1 takes the forward pass's writes, which
     // nothing reads (that pass only needs the objectives)
@@ -1852,7 +1886,7 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
 int sjf_add_source(sjf_engine* e, int comp, const double box[6], const sjf_pulse* p, double amp_re,
                    double amp_im) {
     if (!e || !box || !p) return SJF_E_ARG;
-    if (comp < 0 || comp > 2) return fail(e, SJF_E_UNSUPPORTED, "only electric sources are supported");
+    if (comp < 0 || comp > 5) return fail(e, SJF_E_ARG, "source component must be Ex..Hz");
     if (p->kind != SJF_SRC_GAUSSIAN && p->kind != SJF_SRC_CONTINUOUS) return fail(e, SJF_E_ARG, "bad pulse kind");
     if (!(p->freq > 0)) return fail(e, SJF_E_ARG, "pulse frequency must be positive");
     if (p->kind == SJF_SRC_GAUSSIAN && !(p->width > 0)) return fail(e, SJF_E_ARG, "pulse width must be positive");
@@ -1898,7 +1932,9 @@ int sjf_last_source_time(const sjf_engine* e, double* t) {
     if (!e || !t) return SJF_E_ARG;
     double m = 0;
     for (const auto& s : e->src) {
-        const double lt = s.p.kind == SJF_SRC_GAUSSIAN ? (double)(float)(s.dev.peak + s.dev.cutoff) : s.p.t_end;
+        // src_time::last_time rounds to float (gaussian_src_time_phase: src/disp.hpp:119)
+        const double lt = s.p.kind == SJF_SRC_GAUSSIAN ? (double)(float)(s.dev.peak + s.dev.cutoff)
+                                                       : (double)(float)s.p.t_end;
         m = std::max(m, lt);
     }
     *t = m;
@@ -2072,6 +2108,13 @@ int sjf_step(sjf_engine* e, long n) {
     return run_set(own_set(e), n);
 }
 
+int sjf_reserve_steps(sjf_engine* e, long n) {
+    if (!e || n < 0) return SJF_E_ARG;
+    if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
+    HIPCHK(e, hipSetDevice(e->g.device));
+    return prepare_set(own_set(e), n);
+}
+
 int sjf_synchronize(sjf_engine* e) {
     if (!e) return SJF_E_ARG;
     return sync_engine(e);
@@ -2236,6 +2279,15 @@ int sjf_batch_step(sjf_batch* b, long n) {
     return rc;
 }
 
+int sjf_batch_reserve_steps(sjf_batch* b, long n) {
+    if (!b || n < 0) return SJF_E_ARG;
+    LaunchSet* s = b->set;
+    if (hipSetDevice(s->eng[0]->g.device) != hipSuccess) return SJF_E_HIP;
+    const int rc = prepare_set(s, n);
+    if (rc) b->err = s->eng[0]->err;
+    return rc;
+}
+
 int sjf_batch_timed_steps(sjf_batch* b, long n, double* ms_total, double* ms_kernel_mean) {
     if (!b || n < 1 || !ms_total || !ms_kernel_mean) return SJF_E_ARG;
     LaunchSet* s = b->set;
@@ -2358,7 +2410,7 @@ int sjf_copy_probe(int device, size_t bytes, int iters, double* gbs) {
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
-    const int blocks = 256 * 8 * 4;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
     (void)hipEventRecord(t0, 0);
     for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(sjf::k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);

@@ -219,6 +219,11 @@ class Engine:
     def synchronize(self):
         self._chk(self.lib.sjf_synchronize(self.h), "sjf_synchronize")
 
+    def reserve_steps(self, n: int):
+        """Size the sample buffer and build the launch tables for the next n steps now, so that
+        stepping them allocates nothing (sjf_reserve_steps)."""
+        self._chk(self.lib.sjf_reserve_steps(self.h, int(n)), "sjf_reserve_steps")
+
     def timed_steps(self, n: int):
         """(ms_total, ms_kernel_mean) from HIP events on the engine's stream."""
         a, b = C.c_double(), C.c_double()
@@ -310,6 +315,9 @@ class Batch:
 
     def synchronize(self):
         self._chk(self.lib.sjf_batch_synchronize(self.h), "sjf_batch_synchronize")
+
+    def reserve_steps(self, n: int):
+        self._chk(self.lib.sjf_batch_reserve_steps(self.h, int(n)), "sjf_batch_reserve_steps")
 
     def timed_steps(self, n: int):
         a, b = C.c_double(), C.c_double()

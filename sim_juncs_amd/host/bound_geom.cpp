@@ -44,6 +44,10 @@ BoundGeom::BoundGeom(const Settings& s, const Scene& scene, const RunOptions& op
     : scene_(&scene), s_(s), opt_(opt) {
     if (s.grid_num <= 0 || s.resolution <= 0)
         throw std::invalid_argument("grid_num and resolution must be positive (call correct_defaults)");
+    // the reference builds vol1d / vol2d for dimensions 1 / 2 (src/disp.cpp:504-510); this driver
+    // runs the 3-D volume only and refuses the others rather than run different physics
+    if (s.n_dims != 3)
+        throw std::invalid_argument("dimensions = " + std::to_string(s.n_dims) + " is not supported (3-D only)");
     n_ = (int)s.grid_num;
     um_scale_ = s.um_scale;
     post_source_t_ = s.post_source_t;
@@ -85,11 +89,7 @@ BoundGeom::BoundGeom(const Settings& s, const Scene& scene, const RunOptions& op
         if (!sp.has_box) continue;  // only Box regions are sources (src/disp.cpp:597)
         const SourceInfo si = source_info(sp);
         sources_.push_back(si);
-        if (si.component >= 3) {
-            std::fprintf(stderr, "warning: magnetic-current source (component %d) is not supported; skipped\n",
-                         si.component);
-            continue;
-        }
+        // any component: Ex..Ez are electric currents, Hx..Hz magnetic currents (src/disp.cpp:614,619)
         const sjf_pulse pl = pulse_of(si, um_scale_);
         const double box[6] = {sp.box[0].x, sp.box[0].y, sp.box[0].z, sp.box[1].x, sp.box[1].y, sp.box[1].z};
         for (int p = 0; p < nparts_; ++p)
@@ -126,6 +126,8 @@ BoundGeom::~BoundGeom() {
 void BoundGeom::run(const std::string& out_dir) {
     const int n3[3] = {n_, n_, n_};
     write_grid_h5(out_dir + "/eps-000000.00.h5", "eps", eps_cell_, n3);
+    if (!(ttot_ > 0))
+        throw std::runtime_error("no source sets the run time (total time " + std::to_string(ttot_) + ")");
     n_t_pts_ = (unsigned)((ttot_ + dt_ / 2) / dt_);
     const size_t n_locs = monitor_locs_.size();
     field_times_.assign(n_locs, {});

@@ -127,6 +127,15 @@ def gaussian_pulse(wavelength_um, fwhm_fs, cep, um_scale=1.0, start_fs=5.0):
                 phase=float(cep), t_start=start_fs * ca, t_end=end_fs * ca)
 
 
+def cw_pulse(wavelength_um, width_fs, start_fs, end_fs, um_scale=1.0, slowness=3.0):
+    """CW_source semantics (reference src/cgs_data.cpp:55-83, src/disp.cpp:318-376,604-619):
+    meep::continuous_src_time(frequency, width, start_time, end_time) with meep's default
+    slowness 3; times converted to meep units with c_a = 0.299792458 um_scale."""
+    ca = LIGHT_SPEED * um_scale
+    return dict(kind="continuous", freq=1.0 / (wavelength_um * um_scale), width=width_fs * ca,
+                t_start=start_fs * ca, t_end=end_fs * ca, slowness=float(slowness))
+
+
 def plane_source(shape, dx, z_cell, pulse, comp="Ex", amp=1.0):
     nx, ny, _ = shape
     box = [0.0, 0.0, z_cell * dx, nx * dx, ny * dx, z_cell * dx]

@@ -94,6 +94,37 @@ def test_dft_region_at_and_away_from_the_source_plane_fp64(through_source):
     eng.close()
 
 
+@pytest.mark.parametrize("comp,width,amp,part", [
+    ("Ex", 0.6, 1.0, 0), ("Ey", 0.0, 0.7 - 0.4j, 1), ("Hz", 0.6, 1.0, 0), ("Hy", 0.0, 0.5 + 0.5j, 1),
+    ("Hx", 0.6, 0.3 + 0.9j, 0),
+])
+def test_continuous_and_magnetic_sources_fp64(comp, width, amp, part):
+    """meep::continuous_src_time sources (reference src/disp.cpp:616-619, tests/test.geom:8 and
+    tests/context_test.geom:3 declare CW sources, the latter on Hz) and magnetic-current sources
+    on any H component, as a plane sheet and as a point source, in random dispersive media with
+    PML: fields and Ex/Hz monitors equal the oracle's to FP64 rounding, for the real and the
+    imaginary part of a complex amplitude."""
+    shape = (14, 12, 20)
+    scn, _ = random_scenario(shape, seed=7 + part, pml=[2, 2, 2, 2, 3, 3], steps=160)
+    scn.part = part
+    dx = scn.dx
+    # width > 0: tanh turn-on and turn-off (end 5 fs = 1.5 time units) inside the 1.6-unit run
+    pulse = scenario.cw_pulse(0.8, 0.0, 0.0, 1e6) if width == 0.0 else scenario.cw_pulse(0.8, width, 0.0, 5.0)
+    scn.sources = [scenario.plane_source(shape, dx, 9, pulse, comp=comp, amp=amp),
+                   dict(comp=comp, box=[5.3 * dx, 6.1 * dx, 12.4 * dx] * 2, amp=0.8 * amp, **pulse)]
+    pts = np.array([[7.1 * dx, 6.3 * dx, 4.2 * dx], [3.3 * dx, 8.6 * dx, 15.5 * dx]])
+    scn.monitors = [("Ex", pts), ("Hz", pts)]
+    eng, orc = run_pair(scn)
+    assert abs(eng.last_source_time() - orc.last_source_time()) == 0
+    errs = field_errors(eng, orc)
+    assert max(errs) < FP64_TOL, errs
+    assert np.abs(orc.read_field(comp)).max() > 0
+    m_e, m_o = eng.read_monitors(), orc.read_monitors()
+    assert np.abs(m_o).max() > 0
+    assert rel(m_e, m_o) < FP64_TOL
+    eng.close()
+
+
 def test_plain_box_fp32_within_tolerance():
     scn = early_pulse(scenario.plain_box(n=32, steps=300, precision="fp32", pml=6))
     eng, orc = run_pair(scn)

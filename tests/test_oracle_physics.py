@@ -187,6 +187,116 @@ def test_current_sheet_radiation_matches_green_function():
         assert err < 1e-2, err
 
 
+def logistic(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def test_continuous_src_time_closed_form():
+    """meep continuous_src_time (reference src/disp.cpp:618): dipole(t) = env(t) e^{-i w t} / (-i w)
+    with env = (1 + tanh(ts)) (1 + tanh(te)) / 4, ts = (t - t0)/width - s, te = (t1 - t)/width - s.
+    Pinned here through identities a slip would break: the steady state times -i w is exactly
+    e^{-i w t}; (1 + tanh x)/2 is the logistic function of 2x (so env = sigma(2 ts) sigma(2 te));
+    env is 1/2 at t0 + s width far from the end; turn-on and turn-off mirror each other; the
+    float gate switches on where float(t) >= t0."""
+    f, st, et = 1.3, 0.1, 40.0
+    w = 2 * math.pi * f
+    for t in (0.37, 5.0, 17.123, 39.9):
+        v = orc.cw_dipole(f, 0.0, st, et, 3.0, t)
+        assert abs(v * (-1j * w) - np.exp(-1j * w * t)) < 1e-14
+        assert abs(abs(v) - 1 / w) < 1e-15
+    width, slow = 0.7, 3.0
+    ts_ = np.linspace(st, et, 997)
+    for t in ts_:
+        v = orc.cw_dipole(f, width, st, et, slow, t)
+        env = logistic(2 * ((t - st) / width - slow)) * logistic(2 * ((et - t) / width - slow))
+        assert abs(abs(v) * w - env) < 1e-13
+        assert abs(v * (-1j * w) - env * np.exp(-1j * w * t)) < 1e-13
+    half = abs(orc.cw_dipole(f, width, st, et, slow, st + slow * width)) * w
+    assert abs(half - 0.5) < 1e-12
+    for x in (0.05, 0.8, 2.1, 3.3):
+        a = abs(orc.cw_dipole(f, width, st, et, slow, st + x)) * w
+        b = abs(orc.cw_dipole(f, width, st, et, slow, et - x)) * w
+        assert abs(a - b) < 1e-12
+    # float gate: float(0.1) > 0.1, so a time just below t0 that rounds up to it is inside
+    assert orc.cw_dipole(f, 0.0, st, et, slow, st - 1e-10) != 0
+    assert orc.cw_dipole(f, 0.0, st, et, slow, st - 1e-4) == 0
+    assert orc.cw_dipole(f, 0.0, st, et, slow, et + 1e-4) == 0
+
+
+def test_continuous_source_current_is_a_sampled_derivative():
+    """The injected current (dipole(t + dt) - dipole(t)) / dt of a steady CW source is
+    e^{-i w (t + dt/2)} sin(w dt/2)/(w dt/2): unit amplitude up to the Yee sinc factor, centred
+    half a step after t."""
+    f, dt = 1.25, 0.005
+    w = 2 * math.pi * f
+    for t in (1.0, 3.7, 9.99):
+        J = (orc.cw_dipole(f, 0.0, 0.0, 100.0, 3.0, t + dt) - orc.cw_dipole(f, 0.0, 0.0, 100.0, 3.0, t)) / dt
+        want = np.exp(-1j * w * (t + dt / 2)) * math.sin(w * dt / 2) / (w * dt / 2)
+        assert abs(J - want) < 1e-10
+
+
+def cw_sheet_run(nz, src_z, steps, comp="Ex", mon_comp="Ex", off=0.0, pml=40, dx=0.01, mon_dz=(60, -60)):
+    o = Oracle((1, 1, nz), dx, courant=0.5, periodic=(1, 1, 0))
+    o.set_pml([0, 0, 0, 0, pml, pml])
+    pulse = scenario.cw_pulse(0.8, 2.0, 1.0, 1e6)
+    z = (src_z + off) * dx
+    o.add_source(comp, [0, 0, z, dx, dx, z], **pulse)
+    o.add_monitors(mon_comp, np.array([[0.5 * dx, 0.0, (src_z + d) * dx] for d in mon_dz]))
+    o.set_save_span(1)
+    o.step(steps)
+    return o, pulse
+
+
+def cw_K(pulse):
+    """d/dt Re dipole(t) of the continuous source, from the logistic form of its envelope."""
+    w = 2 * math.pi * pulse["freq"]
+    st, et, wd, s = pulse["t_start"], pulse["t_end"], pulse["width"], pulse["slowness"]
+
+    def redip(tt):
+        env = logistic(2 * ((tt - st) / wd - s)) * logistic(2 * ((et - tt) / wd - s))
+        return np.where(tt >= st, env * np.sin(w * tt) / w, 0.0)
+
+    h = 1e-6
+    return lambda tt: (redip(tt + h) - redip(tt - h)) / (2 * h)
+
+
+def test_cw_current_sheet_radiates_green_function():
+    """An electric CW sheet radiates E = -K(t - |z|)/2 on both sides (c = 1, Z0 = 1), through
+    the tanh turn-on into the steady sinusoid."""
+    dx, d = 0.01, 60
+    o, pulse = cw_sheet_run(900, 450, 3000, dx=dx)
+    s = o.read_monitors().real
+    t = np.arange(s.shape[0]) * o.dt
+    K = cw_K(pulse)
+    want = -0.5 * K(t - d * dx)
+    assert np.abs(want).max() > 0.4  # steady state reached (|K| -> 1)
+    for col in range(2):
+        err = np.linalg.norm(s[:, col] - want) / np.linalg.norm(want)
+        assert err < 1e-2, err
+
+
+def test_magnetic_current_sheet_radiates_dual_field():
+    """A magnetic current sheet K (component Hy, on the Hy lattice half a cell above z0) radiates
+    Hy = -K(t - |z|)/2 on both sides and Ex = -sign(z) K(t - |z|)/2 (dB/dt = -curl E - K, the
+    dual of the electric sheet): Ex flips sign across the sheet, Hy does not."""
+    dx, d = 0.01, 60
+    o, pulse = cw_sheet_run(900, 450, 3000, comp="Hy", mon_comp="Ex", off=0.5, dx=dx)
+    s = o.read_monitors().real
+    t = np.arange(s.shape[0]) * o.dt
+    K = cw_K(pulse)
+    up = -0.5 * K(t - (d - 0.5) * dx)
+    down = 0.5 * K(t - (d + 0.5) * dx)
+    assert np.linalg.norm(s[:, 0] - up) / np.linalg.norm(up) < 1e-2
+    assert np.linalg.norm(s[:, 1] - down) / np.linalg.norm(down) < 1e-2
+    h, _ = cw_sheet_run(900, 450, 3000, comp="Hy", mon_comp="Hy", off=0.5, dx=dx, mon_dz=(60.5, -59.5))
+    sh = h.read_monitors().real
+    # Hy samples are H^{n-1/2}: half a step earlier than the E samples
+    th = t - 0.5 * o.dt
+    for col, dist in ((0, 60.0), (1, 60.0)):
+        want = -0.5 * K(th - dist * dx)
+        assert np.linalg.norm(sh[:, col] - want) / np.linalg.norm(want) < 1e-2
+
+
 def test_pml_reflection_small():
     """Monitor series next to the PML equals that of a doubled domain to 1e-4 of the peak."""
     d = 100
