@@ -168,6 +168,13 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap);
  * in a fixed order; cap >= the region's frequencies.  New (the reference takes no flux: the
  * derived monitor of meep's add_flux / get_fluxes, for the DFT planes of this engine). */
 int sjf_dft_flux(sjf_engine* e, int id, int axis, double* flux, size_t cap);
+/* sjf_dft_flux without any device allocation: the reduction's partial sums and results go to a
+ * static device buffer of 64 x 65 doubles (33 KiB) that each device holds until the process ends,
+ * so the region may have at most SJF_MAX_DFT_FREQS frequencies (SJF_E_ARG beyond).  The buffer is
+ * shared by every call on a device, so calls that overlap on one device race.  Same results as
+ * sjf_dft_flux, bit for bit.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code inside a hipGraph capture, serving loops, harnesses that forbid hipMalloc). */
+int sjf_dft_flux_noalloc(sjf_engine* e, int id, int axis, double* flux, size_t cap);
 
 /* ---- time stepping ---------------------------------------------------------------------- */
 

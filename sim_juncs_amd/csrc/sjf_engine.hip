@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1613,6 +1614,87 @@ extern "C" int sjf_adjoint_kernel_info(const sjf_engine* e, double* ms_mean, int
 
 // ============================================================================ batches ====
 
+namespace {
+
+// ======================================================================== flux read-out ===
+// The flux of a DFT plane is reduced on the device in two launches (k_dft_flux_part,
+// k_dft_flux_sum) through a buffer of nf * (nb + 1) doubles: nb partial sums per frequency, then
+// the nf results that are copied out.  sjf_dft_flux allocates that buffer per call;
+// sjf_dft_flux_noalloc uses `scratch`, a static device array of at most 64 frequencies x 65
+// doubles (33 KiB) that every device the library runs on holds until the process ends.  Every
+// slot a call reads is written earlier in the same call, so nothing needs clearing; calls that
+// overlap on one device share the array and race.
+constexpr int kFluxMaxBlocks = 64;
+constexpr int kFluxScratchDoubles = SJF_MAX_DFT_FREQS * (kFluxMaxBlocks + 1);
+static __device__ __align__(256) double scratch[kFluxScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device: the address of `scratch` there
+
+// Resolves this device's copy of `scratch` (the device must be current).  The array's name in
+// host code is only a stand-in; kernels and copies take the address resolved here.
+int scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return SJF_OK;
+    void* p = nullptr;
+    const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (st != hipSuccess || !p) {
+        g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(st);
+        return SJF_E_HIP;
+    }
+    g_scratch_ptr[device] = (double*)p;
+    return SJF_OK;
+}
+
+double* scratch_on(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+struct FluxPlan {
+    const Dft* d;
+    int slot[4];  // E_a, E_b, H_a, H_b
+    int nf, nb;
+    long scratch_doubles() const { return (long)nf * (nb + 1); }
+};
+
+int flux_plan(sjf_engine* e, int id, int axis, const double* flux, size_t cap, FluxPlan* fp) {
+    if (!e || !flux || id < 0 || id >= (int)e->dft.size() || axis < 0 || axis > 2) return SJF_E_ARG;
+    const Dft& d = e->dft[id];
+    if (d.hi[axis] - d.lo[axis] != 1) return fail(e, SJF_E_ARG, "a flux plane is one cell thick along its normal");
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    int slot[6] = {-1, -1, -1, -1, -1, -1};
+    for (int q = 0; q < d.ncomp; ++q) slot[d.comps[q]] = q;
+    if (slot[a] < 0 || slot[b] < 0 || slot[3 + a] < 0 || slot[3 + b] < 0)
+        return fail(e, SJF_E_ARG, "a flux plane needs both tangential E and both tangential H components");
+    fp->d = &d;
+    fp->slot[0] = slot[a];
+    fp->slot[1] = slot[b];
+    fp->slot[2] = slot[3 + a];
+    fp->slot[3] = slot[3 + b];
+    fp->nf = (int)d.freqs.size();
+    fp->nb = (int)std::min<long>(kFluxMaxBlocks, std::max<long>(1, (d.ncell + 255) / 256));
+    if (cap < (size_t)fp->nf) return fail(e, SJF_E_ARG, "output buffer too small");
+    return SJF_OK;
+}
+
+// buf: nf * nb partial sums, then the nf results
+int flux_run(sjf_engine* e, const FluxPlan& fp, double* buf, double* flux) {
+    const Dft& d = *fp.d;
+    hipLaunchKernelGGL(sjf::k_dft_flux_part, dim3(fp.nb, fp.nf), dim3(256), 0, e->stream, (const double2*)d.acc,
+                       (const double2*)d.cmp, d.ncell, fp.nf, fp.slot[0], fp.slot[1], fp.slot[2], fp.slot[3], fp.nb,
+                       buf);
+    if (hipGetLastError() != hipSuccess) return fail(e, SJF_E_HIP, "k_dft_flux_part launch failed");
+    double* out = buf + (long)fp.nf * fp.nb;
+    hipLaunchKernelGGL(sjf::k_dft_flux_sum, dim3(1), dim3(64), 0, e->stream, (const double*)buf, fp.nf, fp.nb,
+                       e->g.dx * e->g.dx, out);
+    if (hipGetLastError() != hipSuccess) return fail(e, SJF_E_HIP, "k_dft_flux_sum launch failed");
+    return d2h(e, flux, out, sizeof(double) * fp.nf);
+}
+
+}  // namespace
+
 struct sjf_batch {
     LaunchSet* set = nullptr;
     std::string err;
@@ -1670,6 +1752,7 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
         g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(st);
         return SJF_E_HIP;
     }
+    if (int rc = scratch_setup(g->device)) return rc;
     sjf_engine* e = new sjf_engine();
     e->g = *g;
     e->prec = g->precision;
@@ -2079,26 +2162,27 @@ int sjf_read_dft(sjf_engine* e, int id, double* out, size_t cap) {
 }
 
 int sjf_dft_flux(sjf_engine* e, int id, int axis, double* flux, size_t cap) {
-    if (!e || !flux || id < 0 || id >= (int)e->dft.size() || axis < 0 || axis > 2) return SJF_E_ARG;
-    const Dft& d = e->dft[id];
-    if (d.hi[axis] - d.lo[axis] != 1) return fail(e, SJF_E_ARG, "a flux plane is one cell thick along its normal");
-    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
-    int slot[6] = {-1, -1, -1, -1, -1, -1};
-    for (int q = 0; q < d.ncomp; ++q) slot[d.comps[q]] = q;
-    if (slot[a] < 0 || slot[b] < 0 || slot[3 + a] < 0 || slot[3 + b] < 0)
-        return fail(e, SJF_E_ARG, "a flux plane needs both tangential E and both tangential H components");
-    const int nf = (int)d.freqs.size();
-    if (cap < (size_t)nf) return fail(e, SJF_E_ARG, "output buffer too small");
+    FluxPlan fp;
+    if (int rc = flux_plan(e, id, axis, flux, cap, &fp)) return rc;
     if (int rc = sync_engine(e)) return rc;
-    double* dout = nullptr;
-    if (int rc = dev_alloc(e, (void**)&dout, sizeof(double) * nf)) return rc;
-    hipLaunchKernelGGL(sjf::k_dft_flux, dim3(nf), dim3(256), 0, e->stream, (const double2*)d.acc,
-                       (const double2*)d.cmp, d.ncell, nf, slot[a], slot[b], slot[3 + a], slot[3 + b],
-                       e->g.dx * e->g.dx, dout);
-    int rc = hipGetLastError() == hipSuccess ? SJF_OK : fail(e, SJF_E_HIP, "k_dft_flux launch failed");
-    if (!rc) rc = d2h(e, flux, dout, sizeof(double) * nf);
-    dev_free(dout);
+    double* buf = nullptr;
+    if (int rc = dev_alloc(e, (void**)&buf, sizeof(double) * fp.scratch_doubles())) return rc;
+    int rc = flux_run(e, fp, buf, flux);
+    dev_free(buf);
     return rc;
+}
+
+int sjf_dft_flux_noalloc(sjf_engine* e, int id, int axis, double* flux, size_t cap) {
+    FluxPlan fp;
+    if (int rc = flux_plan(e, id, axis, flux, cap, &fp)) return rc;
+    if (fp.scratch_doubles() > (long)kFluxScratchDoubles)
+        return fail(e, SJF_E_ARG, "flux needs " + std::to_string(fp.scratch_doubles()) +
+                                      " doubles of scratch; the allocation-free buffer holds " +
+                                      std::to_string(kFluxScratchDoubles));
+    double* buf = scratch_on(e->g.device);
+    if (!buf) return fail(e, SJF_E_STATE, "no scratch address for this device (sjf_create sets it up)");
+    if (int rc = sync_engine(e)) return rc;
+    return flux_run(e, fp, buf, flux);
 }
 
 int sjf_step(sjf_engine* e, long n) {

@@ -1285,34 +1285,47 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, con
 }
 
 // ---- flux through a DFT plane ------------------------------------------------------------
-// out[f] = dA * Re sum_cells (E_a conj(H_b) - E_b conj(H_a)) for frequency f = blockIdx.x, with
-// each spectrum value acc + cmp of the region's [slot][f][cell] accumulators (sa, sb, ha, hb:
-// the slots of E_a, E_b, H_a, H_b).  Each thread sums a strided share of the cells in FP64; the
-// wavefronts reduce by shuffles and the first thread adds the wavefront sums in order, so the
-// result does not vary from run to run.
-__global__ void __launch_bounds__(256) k_dft_flux(const double2* __restrict__ acc, const double2* __restrict__ cmp,
-                                                  long ncell, int nf, int sa, int sb, int ha, int hb, double dA,
-                                                  double* __restrict__ out) {
-    const int f = blockIdx.x;
+// out[f] = dA * Re sum_cells (E_a conj(H_b) - E_b conj(H_a)) for each frequency f, with each
+// spectrum value acc + cmp of the region's [slot][f][cell] accumulators (sa, sb, ha, hb: the slots
+// of E_a, E_b, H_a, H_b).  Two launches, both in a fixed order, so the result does not vary from
+// run to run:
+//   k_dft_flux_part, grid (nb, nf): workgroup (b, f) sums cells b*256 + t + m*nb*256 in FP64 per
+//     thread, the wavefronts reduce by shuffles and the first thread adds the wavefront sums in
+//     order into part[f * nb + b];
+//   k_dft_flux_sum, one thread per frequency: out[f] = dA * (part[f*nb] + ... + part[f*nb + nb-1]).
+__global__ void __launch_bounds__(256) k_dft_flux_part(const double2* __restrict__ acc, const double2* __restrict__ cmp,
+                                                       long ncell, int nf, int sa, int sb, int ha, int hb, int nb,
+                                                       double* __restrict__ part) {
+    const int b = blockIdx.x, f = blockIdx.y;
     auto value = [&](int slot, long c) {
         const long o = ((long)slot * nf + f) * ncell + c;
-        const double2 a = acc[o], b = cmp[o];
-        return make_double2(a.x + b.x, a.y + b.y);
+        const double2 x = acc[o], y = cmp[o];
+        return make_double2(x.x + y.x, x.y + y.y);
     };
     double sum = 0;
-    for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const long stride = (long)nb * blockDim.x;
+    for (long c = (long)b * blockDim.x + threadIdx.x; c < ncell; c += stride) {
         const double2 ea = value(sa, c), eb = value(sb, c), Ha = value(ha, c), Hb = value(hb, c);
         sum += (ea.x * Hb.x + ea.y * Hb.y) - (eb.x * Ha.x + eb.y * Ha.y);
     }
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
-    __shared__ double part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+    __shared__ double wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0;
-        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += part[w];
-        out[f] = t * dA;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += wsum[w];
+        part[(long)f * nb + b] = t;
     }
+}
+
+__global__ void __launch_bounds__(64) k_dft_flux_sum(const double* __restrict__ part, int nf, int nb, double dA,
+                                                     double* __restrict__ out) {
+    const int f = threadIdx.x;
+    if (f >= nf) return;
+    double t = 0;
+    for (int b = 0; b < nb; ++b) t += part[(long)f * nb + b];
+    out[f] = t * dA;
 }
 
 // ===================================================== adjoint (reverse-mode) kernels ======

@@ -212,6 +212,14 @@ class Engine:
         self._chk(self.lib.sjf_dft_flux(self.h, int(i), int(axis), _dp(out), out.size), "sjf_dft_flux")
         return out
 
+    def dft_flux_noalloc(self, i: int, axis: int) -> np.ndarray:
+        """dft_flux through the allocation-free entry point (sjf_dft_flux_noalloc)."""
+        nf = self._dft[i][1]
+        out = np.zeros(nf)
+        self._chk(self.lib.sjf_dft_flux_noalloc(self.h, int(i), int(axis), _dp(out), out.size),
+                  "sjf_dft_flux_noalloc")
+        return out
+
     # ------------------------------------------------------------------ stepping ----------
     def step(self, n: int = 1):
         self._chk(self.lib.sjf_step(self.h, int(n)), "sjf_step")
