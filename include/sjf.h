@@ -121,6 +121,17 @@ int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const do
  * z_hi} PML thicknesses in cells; R = asymptotic reflection (meep default 1e-15); power =
  * profile exponent (meep default 2). */
 int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power);
+/* Media of the PML slabs.  A Lorentz medium inside the PML admits modes that grow without bound
+ * once the pulse has gone (measured on the junctions: a Lorentz SiO2 substrate reaching the PML
+ * grows x3-5 per 500 steps from the rounding floor; FP32 runs leave the 1e-4 tolerance within
+ * ~6000 steps and blow up past ~10000, FP64 ones later).  With f_ref > 0 every cell inside a PML
+ * slab runs its material without Lorentz poles: their real susceptibility at f_ref joins eps_inf
+ * (kept at eps_inf when the sum falls below 1, as for metals) and Drude poles stay, which keeps
+ * the slabs matched to the medium near f_ref.  f_ref = 0 (the default) runs the given materials
+ * in the PML as well.  Callable in any order with sjf_set_materials and sjf_set_pml before the
+ * first step; each re-derives the map and resets the polarisation state.  New (meep keeps the
+ * poles in its PML, whose formulation differs). */
+int sjf_set_pml_dispersion(sjf_engine* e, double f_ref);
 
 /* ---- sources, monitors, DFT ------------------------------------------------------------ */
 

@@ -235,6 +235,10 @@ struct RunOptions {
     int device = 0;
     bool complex_fields = true; // two engines (real / imaginary part), as meep's complex run
     int graph_chunk = 32;
+    // fold Lorentz poles out of the PML slabs at the first source's frequency
+    // (sjf_set_pml_dispersion): a Lorentz substrate reaching the PML otherwise grows without bound
+    // in long runs; false keeps the poles there, as meep does
+    bool pml_fold = true;
 };
 
 class BoundGeom {

@@ -88,6 +88,7 @@ typedef struct orc {
     double* psiE[3][3]; /* [axis][E comp] */
     double* psiH[3][3]; /* [axis][H comp] */
     uint32_t* matw;
+    double fold_fref; /* > 0: Lorentz poles folded out of the PML slabs at this frequency */
     int nmat;
     double eps_inf[ORC_MAXMAT];
     int npole[ORC_MAXMAT];
@@ -200,6 +201,56 @@ int orc_set_eps_box(orc* o, const int* lo, const int* hi, const double* eps) {
                 for (int c = 0; c < 3; c++) o->inv_eps[c][id] = 1.0 / eps[q];
                 if (o->matw) o->matw[id] = 0x00ffffffu; /* id 255: no poles */
             }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ PML media ----- */
+
+static int in_pml_cell(const orc* o, long q) {
+    int ijk[3];
+    ijk[2] = (int)(q % o->n[2]);
+    ijk[1] = (int)((q / o->n[2]) % o->n[1]);
+    ijk[0] = (int)(q / ((long)o->n[1] * o->n[2]));
+    for (int d = 0; d < 3; d++)
+        if (o->has_pml[d] && (ijk[d] < o->npml[2 * d] || ijk[d] >= o->n[d] - o->npml[2 * d + 1])) return 1;
+    return 0;
+}
+
+/* eps_inf of material m inside a PML slab when Lorentz poles are folded out at frequency f:
+ * eps_inf + sum over Lorentz poles of Re chi(f), chi = sigma w0^2 / (w0^2 - w^2 - i w gamma'),
+ * kept at eps_inf when the sum falls below 1 (metals). */
+static double folded_eps(const orc* o, int m, double f) {
+    double w = 2 * M_PI * f, e = o->eps_inf[m];
+    for (int p = 0; p < o->npole[m]; p++) {
+        if (o->pole[m][p][3] != 0.0) continue; /* Drude pole: kept as a pole */
+        double w0 = 2 * M_PI * o->pole[m][p][0], g = 2 * M_PI * o->pole[m][p][1], sg = o->pole[m][p][2];
+        double complex_re = sg * w0 * w0 * (w0 * w0 - w * w) /
+                            ((w0 * w0 - w * w) * (w0 * w0 - w * w) + w * w * g * g);
+        e += complex_re;
+    }
+    return e >= 1.0 ? e : o->eps_inf[m];
+}
+
+static int has_lorentz(const orc* o, int m) {
+    for (int p = 0; p < o->npole[m]; p++)
+        if (o->pole[m][p][3] == 0.0) return 1;
+    return 0;
+}
+
+/* Lorentz poles inside a PML admit growing modes; with f > 0 the cells of the PML slabs run
+ * their material without its Lorentz poles, whose real susceptibility at f joins eps_inf
+ * (update_PE skips those poles there).  Call after orc_set_materials, orc_set_eps_box and
+ * orc_set_pml. */
+int orc_set_pml_dispersion(orc* o, double f) {
+    o->fold_fref = f;
+    if (!o->matw || f <= 0) return 0;
+    for (long q = 0; q < o->N; q++) {
+        if (!in_pml_cell(o, q)) continue;
+        for (int c = 0; c < 3; c++) {
+            int m = (o->matw[q] >> (8 * c)) & 0xff;
+            if (m < o->nmat && has_lorentz(o, m)) o->inv_eps[c][q] = 1.0 / folded_eps(o, m, f);
+        }
+    }
     return 0;
 }
 
@@ -676,12 +727,14 @@ static void update_PE(orc* o) {
         }
 #pragma omp parallel for schedule(static)
     for (long q = 0; q < o->N; q++) {
+        int fold = o->fold_fref > 0 && o->maxp && in_pml_cell(o, q);
         for (int c = 0; c < 3; c++) {
             double E = o->F[c][q];
             double sumP = 0;
             if (o->maxp) {
                 int m = (o->matw[q] >> (8 * c)) & 0xff;
                 for (int p = 0; p < o->npole[m]; p++) {
+                    if (fold && o->pole[m][p][3] == 0.0) continue; /* Lorentz pole folded out */
                     double pn = o->P[p][c][q], pm = o->Pm[p][c][q];
                     double pp = cA[m][p] * pn + cB[m][p] * pm + cC[m][p] * E;
                     o->Pm[p][c][q] = pn;

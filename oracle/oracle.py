@@ -44,6 +44,7 @@ def load():
         "orc_set_materials": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_int, dp, ip, dp]),
         "orc_set_eps_box": (C.c_int, [vp, ip, ip, dp]),
         "orc_set_pml": (C.c_int, [vp, ip, C.c_double, C.c_double]),
+        "orc_set_pml_dispersion": (C.c_int, [vp, C.c_double]),
         "orc_add_source": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, C.c_double, C.c_double]),
         "orc_last_source_time": (C.c_double, [vp]),
         "orc_add_monitors": (C.c_int, [vp, C.c_int, dp, C.c_int]),
@@ -154,6 +155,10 @@ class Oracle:
 
     def set_pml(self, cells, R=1e-15, power=2.0):
         self.lib.orc_set_pml(self.h, _ip(cells), float(R), float(power))
+
+    def set_pml_dispersion(self, f_ref):
+        """Fold Lorentz poles out of the PML slabs at frequency f_ref (0: keep them)."""
+        self.lib.orc_set_pml_dispersion(self.h, float(f_ref))
 
     def add_source(self, comp, box, kind="gaussian", freq=1.0, width=1.0, phase=0.0,
                    t_start=0.0, t_end=0.0, slowness=3.0, amp=1.0):

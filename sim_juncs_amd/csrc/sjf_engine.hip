@@ -73,8 +73,12 @@ struct sjf_engine {
     long* d_step = nullptr;
     long step = 0;
 
-    // materials
+    // materials: given_* as the caller set them; matw_h / lut_h / nmat / maxp are what the kernels
+    // run (the given map, with Lorentz poles folded out of the PML slabs when pml_fref > 0)
     bool have_mat = false;
+    std::vector<uint32_t> given_w;
+    std::vector<sjf_material> given_lut;
+    double pml_fref = 0;
     std::vector<uint32_t> matw_h;
     std::vector<sjf_material> lut_h;
     uint32_t* matw = nullptr;
@@ -973,6 +977,97 @@ int upload_lut_t(sjf_engine* e) {
 
 int upload_lut(sjf_engine* e) { return e->prec == 4 ? upload_lut_t<float>(e) : upload_lut_t<double>(e); }
 
+// Real part of a Lorentz pole's susceptibility at meep frequency f:
+// sigma w0^2 (w0^2 - w^2) / ((w0^2 - w^2)^2 + w^2 gamma'^2), w = 2 pi f, w0 = 2 pi f0, gamma' = 2 pi gamma.
+double lorentz_re_chi(const sjf_material& M, int s, double f) {
+    const double w = 2 * M_PI * f, w0 = 2 * M_PI * M.f0[s], g = 2 * M_PI * M.gamma[s];
+    const double d = w0 * w0 - w * w;
+    return M.sigma[s] * w0 * w0 * d / (d * d + w * w * g * g);
+}
+
+bool cell_in_pml(const sjf_engine* e, long q) {
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    const long nyz = (long)n[1] * n[2];
+    const int ijk[3] = {(int)(q / nyz), (int)((q % nyz) / n[2]), (int)(q % n[2])};
+    for (int d = 0; d < 3; ++d)
+        if (e->pml_on[d] && (ijk[d] < e->pml[2 * d] || ijk[d] >= n[d] - e->pml[2 * d + 1])) return true;
+    return false;
+}
+
+// The material map the kernels run: the given map, except that with pml_fref > 0 every cell inside
+// a PML slab whose material has Lorentz poles takes a twin of that material with them folded out
+// (their real susceptibility at pml_fref added to eps_inf when the sum stays >= 1, else eps_inf kept;
+// Drude poles kept).  Uploads the words and the LUT and resets the polarisation state.
+int apply_materials(sjf_engine* e) {
+    const int ng = (int)e->given_lut.size();
+    e->lut_h = e->given_lut;
+    e->matw_h = e->given_w;
+    bool any_pml = e->pml_on[0] || e->pml_on[1] || e->pml_on[2];
+    if (e->pml_fref > 0 && any_pml) {
+        std::vector<int> twin(ng, -1);
+        for (int m = 0; m < ng; ++m) {
+            const sjf_material& M = e->given_lut[m];
+            sjf_material T = M;
+            T.npoles = 0;
+            double eps = M.eps_inf;
+            bool has_lor = false;
+            for (int s = 0; s < M.npoles; ++s) {
+                if (M.is_drude[s]) {
+                    const int t = T.npoles++;
+                    T.f0[t] = M.f0[s];
+                    T.gamma[t] = M.gamma[s];
+                    T.sigma[t] = M.sigma[s];
+                    T.is_drude[t] = 1;
+                } else {
+                    has_lor = true;
+                    eps += lorentz_re_chi(M, s, e->pml_fref);
+                }
+            }
+            if (!has_lor) continue;
+            T.eps_inf = eps >= 1.0 ? eps : M.eps_inf;
+            for (int s = T.npoles; s < SJF_MAX_POLES; ++s) T.f0[s] = T.gamma[s] = T.sigma[s] = 0, T.is_drude[s] = 0;
+            if ((int)e->lut_h.size() >= SJF_MAX_MATERIALS)
+                return fail(e, SJF_E_UNSUPPORTED, "materials and their PML twins exceed 255");
+            twin[m] = (int)e->lut_h.size();
+            e->lut_h.push_back(T);
+        }
+        for (long q = 0; q < e->N; ++q) {
+            if (!cell_in_pml(e, q)) continue;
+            uint32_t w = e->matw_h[q];
+            for (int c = 0; c < 3; ++c) {
+                const int m = (w >> (8 * c)) & 0xff;
+                if (m < ng && twin[m] >= 0) w = (w & ~(0xffu << (8 * c))) | ((uint32_t)twin[m] << (8 * c));
+            }
+            e->matw_h[q] = w;
+        }
+    }
+    int maxp = 0;
+    bool anylor = false;
+    for (const sjf_material& M : e->lut_h) {
+        maxp = std::max(maxp, M.npoles);
+        for (int s = 0; s < M.npoles; ++s)
+            if (!M.is_drude[s]) anylor = true;
+    }
+    e->nmat = (int)e->lut_h.size();
+    e->maxp = maxp;
+    e->uniform = (e->nmat == 1 && maxp == 0 && e->deps_h.empty());
+    int rc = dev_alloc(e, (void**)&e->matw, sizeof(uint32_t) * e->N);
+    if (!rc) rc = h2d(e, e->matw, e->matw_h.data(), sizeof(uint32_t) * e->N);
+    if (rc) return rc;
+    // polarisation state, zeroed
+    dev_free(e->Qbuf);
+    dev_free(e->Pbuf);
+    if (maxp > 0) {
+        rc = dev_alloc(e, &e->Qbuf, (size_t)e->prec * 3 * maxp * e->N);
+        if (!rc && anylor) rc = dev_alloc(e, &e->Pbuf, (size_t)e->prec * 3 * maxp * e->N);
+        if (rc) return rc;
+    }
+    rc = upload_lut(e);
+    if (rc) return rc;
+    e->version++;
+    return SJF_OK;
+}
+
 void trilinear(const sjf_engine* e, int comp, const double* xyz, long* idx, double* w) {
     int i0[3];
     double f[3];
@@ -1822,15 +1917,10 @@ int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_materia
     if (!e || !mat_word || !lut) return SJF_E_ARG;
     if (n_mat < 1 || n_mat > SJF_MAX_MATERIALS) return fail(e, SJF_E_ARG, "n_mat must be in [1, 255]");
     if (int rc = sync_engine(e)) return rc;
-    int maxp = 0;
-    bool anylor = false;
     for (int m = 0; m < n_mat; ++m) {
         if (lut[m].npoles < 0 || lut[m].npoles > SJF_MAX_POLES)
             return fail(e, SJF_E_ARG, "npoles must be in [0, 4]");
         if (!(lut[m].eps_inf > 0)) return fail(e, SJF_E_ARG, "eps_inf must be positive");
-        maxp = std::max(maxp, lut[m].npoles);
-        for (int s = 0; s < lut[m].npoles; ++s)
-            if (!lut[m].is_drude[s]) anylor = true;
     }
     for (long q = 0; q < e->N; ++q)
         for (int c = 0; c < 3; ++c) {
@@ -1840,27 +1930,19 @@ int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_materia
             if (m == sjf::kDesignMat && e->deps_h.empty())
                 return fail(e, SJF_E_STATE, "design id 255 used before sjf_set_design_eps");
         }
-    e->lut_h.assign(lut, lut + n_mat);
-    e->nmat = n_mat;
-    e->matw_h.assign(mat_word, mat_word + e->N);
+    e->given_lut.assign(lut, lut + n_mat);
+    e->given_w.assign(mat_word, mat_word + e->N);
     e->have_mat = true;
-    e->maxp = maxp;
-    e->uniform = (n_mat == 1 && maxp == 0 && e->deps_h.empty());
-    int rc = dev_alloc(e, (void**)&e->matw, sizeof(uint32_t) * e->N);
-    if (!rc) rc = h2d(e, e->matw, mat_word, sizeof(uint32_t) * e->N);
-    if (rc) return rc;
-    // polarisation state, zeroed
-    dev_free(e->Qbuf);
-    dev_free(e->Pbuf);
-    if (maxp > 0) {
-        rc = dev_alloc(e, &e->Qbuf, (size_t)e->prec * 3 * maxp * e->N);
-        if (!rc && anylor) rc = dev_alloc(e, &e->Pbuf, (size_t)e->prec * 3 * maxp * e->N);
-        if (rc) return rc;
-    }
-    rc = upload_lut(e);
-    if (rc) return rc;
-    e->version++;
-    return SJF_OK;
+    return apply_materials(e);
+}
+
+int sjf_set_pml_dispersion(sjf_engine* e, double f_ref) {
+    if (!e) return SJF_E_ARG;
+    if (!(f_ref >= 0) || !std::isfinite(f_ref)) return fail(e, SJF_E_ARG, "f_ref must be >= 0");
+    if (e->step != 0) return fail(e, SJF_E_STATE, "the PML media must be set before the first step");
+    if (int rc = sync_engine(e)) return rc;
+    e->pml_fref = f_ref;
+    return e->have_mat ? apply_materials(e) : SJF_OK;
 }
 
 int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const double* eps) {
@@ -1893,13 +1975,18 @@ int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const do
     // mark design voxels in the material word (all three E components)
     if (!e->have_mat) {
         e->matw_h.assign(e->N, 0u);
+        e->given_w.assign(e->N, 0u);
+        e->given_lut = e->lut_h;
         e->have_mat = true;
         rc = dev_alloc(e, (void**)&e->matw, sizeof(uint32_t) * e->N);
         if (rc) return rc;
     }
     for (int i = lo[0]; i < hi[0]; ++i)
         for (int j = lo[1]; j < hi[1]; ++j)
-            for (int k = lo[2]; k < hi[2]; ++k) e->matw_h[cell_index(e, i, j, k)] = 0x00ffffffu;
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                const long q = cell_index(e, i, j, k);
+                e->matw_h[q] = e->given_w[q] = 0x00ffffffu;
+            }
     rc = h2d(e, e->matw, e->matw_h.data(), sizeof(uint32_t) * e->N);
     if (rc) return rc;
     e->uniform = false;
@@ -1962,6 +2049,7 @@ int sjf_set_pml(sjf_engine* e, const int cells[6], double R, double power) {
     int rc = e->prec == 4 ? convert_to_device<float>(e, e->coefbuf, tab.data(), off)
                           : convert_to_device<double>(e, e->coefbuf, tab.data(), off);
     if (rc) return rc;
+    if (e->have_mat && e->pml_fref > 0) return apply_materials(e);  // the slabs moved
     e->version++;
     return SJF_OK;
 }

@@ -121,6 +121,10 @@ class Engine:
     def set_pml(self, cells, R: float = 1e-15, power: float = 2.0):
         self._chk(self.lib.sjf_set_pml(self.h, _ip(cells), float(R), float(power)), "sjf_set_pml")
 
+    def set_pml_dispersion(self, f_ref: float):
+        """Run the PML slabs without Lorentz poles, folded into eps_inf at f_ref (0: keep them)."""
+        self._chk(self.lib.sjf_set_pml_dispersion(self.h, float(f_ref)), "sjf_set_pml_dispersion")
+
     # ------------------------------------------------------------------ sources -----------
     def add_source(self, comp, box, kind: str = "gaussian", freq: float = 1.0,
                    width: float = 1.0, phase: float = 0.0, t_start: float = 0.0,

@@ -98,6 +98,11 @@ BoundGeom::BoundGeom(const Settings& s, const Scene& scene, const RunOptions& op
         check(sjf_last_source_time(eng_[0], &last), "sjf_last_source_time");
         ttot_ = last + post_source_t_ * 0.299792458 * um_scale_;
     }
+    // Lorentz poles out of the PML slabs, matched at the first source's frequency
+    if (opt.pml_fold && pc > 0 && !sources_.empty()) {
+        const double f = pulse_of(sources_.front(), um_scale_).freq;
+        for (int p = 0; p < nparts_; ++p) check(sjf_set_pml_dispersion(eng_[p], f), "sjf_set_pml_dispersion", p);
+    }
     for (const auto& cl : scene.monitor_clusters) {
         for (const Vec3& v : cl) monitor_locs_.push_back(v);
         monitor_clusters_.push_back(monitor_locs_.size());

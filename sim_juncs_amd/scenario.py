@@ -34,6 +34,7 @@ class Scenario:
     lut: list = field(default_factory=lambda: [M.VACUUM])
     design: tuple | None = None          # (lo, hi, eps array)
     pml: tuple | None = None             # (cells[6], R, power)
+    pml_fold: float = 0.0                # f_ref of sjf_set_pml_dispersion (0: Lorentz poles kept in the PML)
     sources: list = field(default_factory=list)
     monitors: list = field(default_factory=list)  # (comp, xyz[n,3])
     save_span: int = 1
@@ -59,6 +60,8 @@ class Scenario:
             sim.set_design_eps(*self.design)
         if self.pml is not None:
             sim.set_pml(*self.pml)
+            if self.pml_fold > 0:
+                sim.set_pml_dispersion(self.pml_fold)
         for s in self.sources:
             sim.add_source(**s)
         for comp, xyz in self.monitors:
@@ -214,7 +217,7 @@ def bowtie_junction(nx=512, ny=512, nz=256, steps=6000, precision="fp32", pml=12
     dfts = [((p[0], p[1], p[2]), (p[0] + 1, p[1] + 1, p[2] + 1), ("Ex", "Ey", "Ez"), freqs) for p in pts]
     return Scenario(
         name=f"bowtie_{nx}x{ny}x{nz}", shape=shape, dx=dx, precision=precision, matw=pack(mat),
-        lut=lut, pml=([pml] * 6, 1e-15, 2.0), sources=[plane_source(shape, dx, src_z, pulse)],
+        lut=lut, pml=([pml] * 6, 1e-15, 2.0), pml_fold=f0, sources=[plane_source(shape, dx, src_z, pulse)],
         dfts=dfts, monitors=[("Ex", np.array([[(cx + 0.5) * dx, cy * dx, gz * dx]]))],
         save_span=20, steps=steps, meta=dict(src_z=src_z, points=pts))
 
@@ -253,7 +256,7 @@ def gap_junction(shape, gap, kind="box", dx=0.005, pml=12, fwhm_fs=10.0, cep=0.0
     freqs = list(np.linspace(0.8, 1.2, nfreq) * f0)
     return Scenario(
         name=name or f"{kind}_{nx}x{ny}x{nz}_g{gap}", shape=tuple(shape), dx=dx,
-        precision=precision, matw=pack(mat), lut=lut, pml=([pml] * 6, 1e-15, 2.0),
+        precision=precision, matw=pack(mat), lut=lut, pml=([pml] * 6, 1e-15, 2.0), pml_fold=f0,
         sources=[plane_source(shape, dx, src_z, pulse)],
         dfts=[(p, (p[0] + 1, p[1] + 1, p[2] + 1), ("Ex", "Ey", "Ez"), freqs)],
         monitors=[("Ex", np.array([[(p[0] + 0.5) * dx, p[1] * dx, p[2] * dx]]))], save_span=20,
@@ -308,7 +311,7 @@ def adjoint_junction(shape=(256, 256, 128), design=(64, 64, 16), steps=4000, see
     return Scenario(
         name=name or f"adjoint_{nx}x{ny}x{nz}_d{design[0]}x{design[1]}x{design[2]}", shape=tuple(shape),
         dx=dx, precision=precision, matw=pack(mat), lut=lut, pml=([pml] * 6, 1e-15, 2.0),
-        sources=[plane_source(shape, dx, pml + 2, pulse)], design=(lo, hi, eps), steps=steps,
+        pml_fold=pulse["freq"], sources=[plane_source(shape, dx, pml + 2, pulse)], design=(lo, hi, eps), steps=steps,
         meta=dict(gap=gap))
 
 

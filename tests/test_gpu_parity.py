@@ -125,6 +125,29 @@ def test_continuous_and_magnetic_sources_fp64(comp, width, amp, part):
     eng.close()
 
 
+@pytest.mark.parametrize("tile", [None, ("thread_rows", 2, 3), ("register_rows", 1, 4)])
+def test_pml_without_lorentz_poles_fp64(tile):
+    """sjf_set_pml_dispersion: random Au (Drude + 2 Lorentz) / SiO2 (Lorentz) / dielectric media
+    filling the grid, PML on every face; inside the slabs the engine runs twin materials without
+    the Lorentz poles (folded into eps_inf at f_ref, Au keeps eps_inf and its Drude pole) while the
+    oracle skips those poles cell by cell.  Fields and monitors agree to FP64 rounding, and differ
+    from the run that keeps the poles in the PML."""
+    shape = (16, 14, 22)
+    scn, _ = random_scenario(shape, seed=3, pml=[3, 2, 2, 3, 3, 4], steps=150)
+    scn.pml_fold = 1.25
+    pts = np.array([[7.1 * scn.dx, 6.3 * scn.dx, 4.2 * scn.dx], [1.2 * scn.dx, 1.6 * scn.dx, 20.5 * scn.dx]])
+    scn.monitors = [("Ex", pts)]
+    eng, orc = run_pair(scn, tile=tile)
+    assert max(field_errors(eng, orc)) < FP64_TOL
+    assert rel(eng.read_monitors(), orc.read_monitors()) < FP64_TOL
+    scn.pml_fold = 0.0
+    keep = scn.make(Engine)
+    keep.step(scn.steps)
+    assert rel(keep.read_field("Ex"), orc.read_field("Ex")) > 1e-6
+    keep.close()
+    eng.close()
+
+
 def test_plain_box_fp32_within_tolerance():
     scn = early_pulse(scenario.plain_box(n=32, steps=300, precision="fp32", pml=6))
     eng, orc = run_pair(scn)

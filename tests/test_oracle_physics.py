@@ -297,6 +297,39 @@ def test_magnetic_current_sheet_radiates_dual_field():
         assert np.linalg.norm(sh[:, col] - want) / np.linalg.norm(want) < 1e-2
 
 
+def column_cw(nz, src, mon, steps, fold):
+    """1-D column (periodic 1x1 cross-section) of SiO2 (one Lorentz pole) filling every cell,
+    PML on both z faces, a CW Ex sheet at f = 1.25 (800 nm) with a smooth turn-on."""
+    dx = 0.01
+    mat = np.ones((1, 1, nz, 3), np.uint8)
+    o = Oracle((1, 1, nz), dx, courant=0.5, periodic=(1, 1, 0))
+    o.set_materials(scenario.pack(mat), [M.VACUUM, M.SIO2.scaled(1.0)])
+    o.set_pml([0, 0, 0, 0, 40, 40])
+    pulse = scenario.cw_pulse(0.8, 3.0, 0.0, 1e6)
+    if fold:
+        o.set_pml_dispersion(pulse["freq"])
+    o.add_source("Ex", [0, 0, src * dx, dx, dx, src * dx], **pulse)
+    o.add_monitors("Ex", np.array([[0.5 * dx, 0.0, mon * dx]]))
+    o.set_save_span(1)
+    o.step(steps)
+    return o.read_monitors().real[:, 0]
+
+
+def test_pml_with_lorentz_poles_folded_is_matched_at_f_ref():
+    """With the Lorentz poles folded out of the PML at f_ref, the slabs hold a non-dispersive
+    medium of eps = eps_inf + Re chi(f_ref), the interior's permittivity at f_ref: a CW wave at
+    f_ref meets no interface.  The series 150 cells in front of the PML equals that of a column
+    long enough that nothing comes back within the run, to 2e-3 of its amplitude; an unfolded
+    eps_inf (1 instead of 2.14) in the slabs would reflect ~19%."""
+    steps = 3200
+    near = column_cw(600, 300, 410, steps, fold=True)
+    far = column_cw(3000, 1500, 1610, steps, fold=True)
+    assert np.abs(far).max() > 0.1
+    assert np.abs(near - far).max() < 2e-3 * np.abs(far).max()
+    kept = column_cw(600, 300, 410, steps, fold=False)
+    assert np.abs(kept - far).max() < 2e-3 * np.abs(far).max()
+
+
 def test_pml_reflection_small():
     """Monitor series next to the PML equals that of a doubled domain to 1e-4 of the peak."""
     d = 100
