@@ -10,6 +10,8 @@ themselves, each bounded to about a minute of GPU time:
 * the 512x512x256 bowtie junction: the FP64 engine against the oracle at full size (200 steps
   of a pulse that starts at t = 0), then FP32 against FP64 engines over all 6000 steps (DFT
   spectra, monitor series and the final fields, to 1e-4);
+* the 64-instance sweep: all 64 instances batched, FP32 against FP64 over 3000 steps, and 8
+  seeded instances (FP64 batch) against the oracle;
 * the 1024x768x512 junction, FP32 against FP64 engines over 1000 steps;
 * the 16 design variants at 192x192x96 with 3000 steps: the batched gradients equal each
   engine's own run, and FP32 gradients stay within 1e-4 of FP64 ones;
@@ -144,6 +146,61 @@ def test_bowtie_fp32_over_all_6000_steps_within_tolerance():
         err = rel(a["f"][c], b["f"][c])
         print(f"  final {c}: {err:.2e}", flush=True)
         assert err < FP32_TOL, (c, err)
+
+
+# ------------------------------------------------------------------ 64-instance sweep -------
+
+@pytest.mark.timeout(600)
+def test_sweep_all_64_instances_fp32_within_tolerance_of_fp64():
+    """All 64 seeded instances (64^3 .. 192x192x128, box and side-tie junctions) as one batch over
+    the configured 3000 steps, FP32 against the FP64 batch: the per-instance DFT at the gap centre
+    and the monitor series to 1e-4."""
+    scns = scenario.batched_sweep()
+    assert len(scns) == 64 and scns[0].steps == 3000
+    print("[case] sweep 64 instances, 3000 steps, fp32 vs fp64 batches", flush=True)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        engs = [s.make(Engine, precision=prec) for s in scns]
+        b = Batch(engs)
+        b.step(scns[0].steps)
+        b.close()
+        res[prec] = [(e.read_dft(0), e.read_monitors()) for e in engs]
+        for e in engs:
+            e.close()
+        print(f"  {prec} batch done", flush=True)
+    worst = 0.0
+    for i, ((d32, m32), (d64, m64)) in enumerate(zip(res["fp32"], res["fp64"])):
+        assert np.abs(d64).max() > 0, i
+        err = max(rel(d32, d64), rel(m32, m64))
+        worst = max(worst, err)
+        assert err < FP32_TOL, (i, scns[i].name, err)
+    print(f"  worst instance: {worst:.2e}", flush=True)
+
+
+@pytest.mark.timeout(600)
+def test_sweep_8_seeded_instances_match_oracle():
+    """8 instances drawn with a fixed seed from the 64, full size, as one FP64 batch against the
+    oracle (300 steps of the instance's pulse started at t = 0): fields and DFT to 1e-10."""
+    scns = [early(s) for s in scenario.batched_sweep(precision="fp64")]
+    pick = np.random.default_rng(3).choice(len(scns), size=8, replace=False)
+    sub = [scns[i] for i in pick]
+    steps = 300
+    print(f"[case] sweep instances {sorted(pick.tolist())} vs oracle, {steps} steps", flush=True)
+    engs = [s.make(Engine) for s in sub]
+    b = Batch(engs)
+    b.step(steps)
+    b.close()
+    for s, e in zip(sub, engs):
+        o = s.make(Oracle)
+        o.step(steps)
+        for c in range(6):
+            ref = o.read_field(c)
+            assert np.abs(ref).max() > 0 or c == 2
+            assert rel(e.read_field(c), ref) < FP64_TOL, (s.name, c)
+        assert rel(e.read_dft(0), o.read_dft(0)) < FP64_TOL, s.name
+        o.close()
+        e.close()
+        print(f"  {s.name} {s.shape} ok", flush=True)
 
 
 # ------------------------------------------------------------------ 1024x768x512 junction ---
