@@ -114,6 +114,15 @@ const char* sjf_last_error(const sjf_engine* e);
  * `lut`, n_mat <= 255).  Resets all polarisation state. */
 int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_material* lut,
                       int n_mat);
+/* Per-cell eps_inf of the three E components, replacing the LUT's eps_inf everywhere outside the
+ * design region: eps holds 3 * nx*ny*nz values, [Ex Ey Ez][x][y][z] (z fastest), each > 0; poles
+ * stay those of each cell's material.  This is how the fractional permittivity of the reference's
+ * smoothed material function reaches the engine: chi1p1 = sum over roots of [ambient + (eps_root -
+ * ambient) * in_bound fraction] (src/disp.cpp:56-112, 264-283 with smooth_n > 0), a continuum that a
+ * 255-entry LUT cannot hold.  Inside the PML slabs under sjf_set_pml_dispersion the folded Lorentz
+ * susceptibility is added to the map's value.  eps = NULL removes the map.  Adjoint runs need the
+ * LUT's eps_inf (SJF_E_UNSUPPORTED with a map).  Resets the polarisation state. */
+int sjf_set_eps_maps(sjf_engine* e, const double* eps);
 /* Per-voxel permittivity (no poles) over the cell box [lo, hi).  New; used for the design
  * region of the adjoint workloads.  eps is (hi-lo) sized, z fastest. */
 int sjf_set_design_eps(sjf_engine* e, const int lo[3], const int hi[3], const double* eps);

@@ -46,10 +46,12 @@ int sjfh_source_info(const char* scene_json, int s, double um_scale, double* out
 int sjfh_fft(const double* in, long n, double* out, long cap, long* n_out);
 
 /* voxelize on an n^3 grid of cell size dx: words (n^3) and up to cap_mat materials (each 26
- * doubles: eps_inf, npoles, f0[4], gamma[4], sigma[4], is_drude[4], 4 unused). */
+ * doubles: eps_inf, npoles, f0[4], gamma[4], sigma[4], is_drude[4], 4 unused).  With smooth_n > 0
+ * *has_map = 1 and eps_map (3 n^3 doubles, [Ex Ey Ez][cell], may be NULL) receives the per-cell
+ * eps_inf for sjf_set_eps_maps; the materials then carry the poles only. */
 int sjfh_voxelize(const char* scene_json, double ambient_eps, unsigned smooth_n, double smooth_rad,
                   double um_scale, double resolution, int n, double dx, unsigned* words, double* mats,
-                  int cap_mat, int* n_mat);
+                  int cap_mat, int* n_mat, double* eps_map, int* has_map);
 
 /* Runs the .geom scene script geom_path (the reference's scene language: scene::read_file,
  * src/cgs.cpp:510-570) in the context context_from_settings builds (src/disp.cpp:28-50) from

@@ -208,6 +208,9 @@ struct VoxelMaps {
     int n[3] = {0, 0, 0};
     std::vector<uint32_t> words;
     std::vector<sjf_material> lut;
+    // with smoothing (smooth_n > 0) the permittivity is fractional: per-cell eps_inf of Ex, Ey, Ez
+    // ([c][cell], for sjf_set_eps_maps) and a LUT of the pole combinations only (eps_inf 1)
+    std::vector<double> eps_map;
 };
 VoxelMaps voxelize(const Scene& scene, const Settings& s, const int n[3], double dx);
 

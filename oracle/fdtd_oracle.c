@@ -89,6 +89,7 @@ typedef struct orc {
     double* psiH[3][3]; /* [axis][H comp] */
     uint32_t* matw;
     double fold_fref; /* > 0: Lorentz poles folded out of the PML slabs at this frequency */
+    double* emap[3];  /* per-cell eps_inf of Ex, Ey, Ez (orc_set_eps_maps), or NULL */
     int nmat;
     double eps_inf[ORC_MAXMAT];
     int npole[ORC_MAXMAT];
@@ -135,7 +136,7 @@ orc* orc_create(int nx, int ny, int nz, double dx, double courant, const int* bc
 void orc_destroy(orc* o) {
     if (!o) return;
     for (int c = 0; c < 6; c++) free(o->F[c]);
-    for (int c = 0; c < 3; c++) { free(o->D[c]); free(o->inv_eps[c]); }
+    for (int c = 0; c < 3; c++) { free(o->D[c]); free(o->inv_eps[c]); free(o->emap[c]); }
     for (int d = 0; d < 3; d++) {
         free(o->bE[d]); free(o->aE[d]); free(o->bH[d]); free(o->aH[d]);
         for (int c = 0; c < 3; c++) { free(o->psiE[d][c]); free(o->psiH[d][c]); }
@@ -190,6 +191,20 @@ int orc_set_materials(orc* o, const uint32_t* matw, int nmat, const double* eps_
     return 0;
 }
 
+/* Per-cell eps_inf of the three E components (eps: 3 * N values, [Ex Ey Ez][cell]), in place of
+ * the materials' eps_inf: the fractional permittivity of the reference's smoothed material
+ * function (chi1p1, src/disp.cpp:264-283 with smooth_n > 0).  Call after orc_set_materials and
+ * before orc_set_eps_box and orc_set_pml_dispersion. */
+int orc_set_eps_maps(orc* o, const double* eps) {
+    for (int c = 0; c < 3; c++) {
+        free(o->emap[c]);
+        o->emap[c] = (double*)malloc(sizeof(double) * o->N);
+        memcpy(o->emap[c], eps + (long)c * o->N, sizeof(double) * o->N);
+        for (long q = 0; q < o->N; q++) o->inv_eps[c][q] = 1.0 / eps[(long)c * o->N + q];
+    }
+    return 0;
+}
+
 /* Per-voxel permittivity over a box of cells (all three E components of a voxel take
  * the voxel's eps).  Used for design regions; overrides eps_inf there, no poles. */
 int orc_set_eps_box(orc* o, const int* lo, const int* hi, const double* eps) {
@@ -216,11 +231,12 @@ static int in_pml_cell(const orc* o, long q) {
     return 0;
 }
 
-/* eps_inf of material m inside a PML slab when Lorentz poles are folded out at frequency f:
- * eps_inf + sum over Lorentz poles of Re chi(f), chi = sigma w0^2 / (w0^2 - w^2 - i w gamma'),
- * kept at eps_inf when the sum falls below 1 (metals). */
-static double folded_eps(const orc* o, int m, double f) {
-    double w = 2 * M_PI * f, e = o->eps_inf[m];
+/* eps_inf of a cell of material m (base: the material's eps_inf, or the cell's map value) inside a
+ * PML slab when Lorentz poles are folded out at frequency f: base + sum over Lorentz poles of
+ * Re chi(f), chi = sigma w0^2 / (w0^2 - w^2 - i w gamma'), kept at base when the sum falls below 1
+ * (metals). */
+static double folded_eps(const orc* o, int m, double base, double f) {
+    double w = 2 * M_PI * f, e = base;
     for (int p = 0; p < o->npole[m]; p++) {
         if (o->pole[m][p][3] != 0.0) continue; /* Drude pole: kept as a pole */
         double w0 = 2 * M_PI * o->pole[m][p][0], g = 2 * M_PI * o->pole[m][p][1], sg = o->pole[m][p][2];
@@ -228,7 +244,7 @@ static double folded_eps(const orc* o, int m, double f) {
                             ((w0 * w0 - w * w) * (w0 * w0 - w * w) + w * w * g * g);
         e += complex_re;
     }
-    return e >= 1.0 ? e : o->eps_inf[m];
+    return e >= 1.0 ? e : base;
 }
 
 static int has_lorentz(const orc* o, int m) {
@@ -248,7 +264,10 @@ int orc_set_pml_dispersion(orc* o, double f) {
         if (!in_pml_cell(o, q)) continue;
         for (int c = 0; c < 3; c++) {
             int m = (o->matw[q] >> (8 * c)) & 0xff;
-            if (m < o->nmat && has_lorentz(o, m)) o->inv_eps[c][q] = 1.0 / folded_eps(o, m, f);
+            if (m < o->nmat && has_lorentz(o, m)) {
+                double base = o->emap[c] ? o->emap[c][q] : o->eps_inf[m];
+                o->inv_eps[c][q] = 1.0 / folded_eps(o, m, base, f);
+            }
         }
     }
     return 0;

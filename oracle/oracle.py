@@ -45,6 +45,7 @@ def load():
         "orc_set_eps_box": (C.c_int, [vp, ip, ip, dp]),
         "orc_set_pml": (C.c_int, [vp, ip, C.c_double, C.c_double]),
         "orc_set_pml_dispersion": (C.c_int, [vp, C.c_double]),
+        "orc_set_eps_maps": (C.c_int, [vp, dp]),
         "orc_add_source": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, C.c_double, C.c_double]),
         "orc_last_source_time": (C.c_double, [vp]),
         "orc_add_monitors": (C.c_int, [vp, C.c_int, dp, C.c_int]),
@@ -148,6 +149,11 @@ class Oracle:
                                         _dp(eps), npole, _dp(poles))
         if rc:
             raise ValueError(f"orc_set_materials failed ({rc})")
+
+    def set_eps_maps(self, eps):
+        """Per-cell eps_inf of Ex, Ey, Ez (array [3, nx, ny, nz])."""
+        v = np.ascontiguousarray(eps, dtype=np.float64).reshape((3,) + self.shape)
+        self.lib.orc_set_eps_maps(self.h, _dp(v))
 
     def set_design_eps(self, lo, hi, eps):
         eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1)

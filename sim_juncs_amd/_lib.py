@@ -36,7 +36,7 @@ EXPORTS = [
     "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
     "sjf_adjoint_kernel_info", "sjf_batch_launch_info", "sjf_batch_adjoint_run", "sjf_batch_adjoint_kernel_info",
     "sjf_dft_flux", "sjf_reserve_steps", "sjf_batch_reserve_steps", "sjf_dft_flux_noalloc",
-    "sjf_set_pml_dispersion",
+    "sjf_set_pml_dispersion", "sjf_set_eps_maps",
 ]
 
 
@@ -109,6 +109,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_synchronize": (C.c_int, [vp]),
         "sjf_reserve_steps": (C.c_int, [vp, C.c_long]),
         "sjf_set_pml_dispersion": (C.c_int, [vp, C.c_double]),
+        "sjf_set_eps_maps": (C.c_int, [vp, dp]),
         "sjf_batch_reserve_steps": (C.c_int, [vp, C.c_long]),
         "sjf_timed_steps": (C.c_int, [vp, C.c_long, dp, dp]),
         "sjf_read_field": (C.c_int, [vp, C.c_int, dp]),

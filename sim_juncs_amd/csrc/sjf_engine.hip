@@ -91,6 +91,9 @@ struct sjf_engine {
     int dlo[3] = {0, 0, 0}, dhi[3] = {0, 0, 0};
     std::vector<double> deps_h;
     void* dinv = nullptr;
+    std::vector<double> emap_h;  // per-cell eps_inf [Ex Ey Ez][x][y][z] as given (sjf_set_eps_maps)
+    std::vector<double> emap_eff;  // as run (Lorentz poles folded in inside the PML slabs)
+    void* emap = nullptr;          // 1/eps_inf on the device, [x][Ex Ey Ez][y][z]
 
     // PML
     int pml[6] = {0, 0, 0, 0, 0, 0};
@@ -238,6 +241,7 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
                         (k - e->dlo[2]);
         return 1.0 / e->deps_h[dq];
     }
+    if (!e->emap_eff.empty()) return 1.0 / e->emap_eff[(size_t)c * e->N + q];
     return 1.0 / e->lut_h[m].eps_inf;
 }
 
@@ -306,6 +310,7 @@ sjf::StepArgs<T> step_args(const sjf_engine* e, int ty, int xc) {
     }
     a.coef = (const T*)e->coefbuf;
     a.dinv = (const T*)e->dinv;
+    a.emap = (const T*)e->emap;
     return a;
 }
 
@@ -901,6 +906,7 @@ int free_all(sjf_engine* e) {
     dev_free(e->matw);
     dev_free(e->lut);
     dev_free(e->dinv);
+    dev_free(e->emap);
     dev_free(e->Qbuf);
     dev_free(e->Pbuf);
     dev_free(e->coefbuf);
@@ -977,6 +983,22 @@ int upload_lut_t(sjf_engine* e) {
 
 int upload_lut(sjf_engine* e) { return e->prec == 4 ? upload_lut_t<float>(e) : upload_lut_t<double>(e); }
 
+// 1/eps_inf of the effective map on the device in the plane layout [x][Ex Ey Ez][y][z]
+template <typename T>
+int upload_emap_t(sjf_engine* e) {
+    const long nyz = (long)e->g.ny * e->g.nz;
+    std::vector<T> h((size_t)3 * e->N);
+    for (int i = 0; i < e->g.nx; ++i)
+        for (int c = 0; c < 3; ++c)
+            for (long r = 0; r < nyz; ++r)
+                h[((size_t)i * 3 + c) * nyz + r] = (T)(1.0 / e->emap_eff[(size_t)c * e->N + (long)i * nyz + r]);
+    int rc = dev_alloc(e, &e->emap, sizeof(T) * 3 * e->N);
+    if (rc) return rc;
+    return h2d(e, e->emap, h.data(), sizeof(T) * 3 * e->N);
+}
+
+int upload_emap(sjf_engine* e) { return e->prec == 4 ? upload_emap_t<float>(e) : upload_emap_t<double>(e); }
+
 // Real part of a Lorentz pole's susceptibility at meep frequency f:
 // sigma w0^2 (w0^2 - w^2) / ((w0^2 - w^2)^2 + w^2 gamma'^2), w = 2 pi f, w0 = 2 pi f0, gamma' = 2 pi gamma.
 double lorentz_re_chi(const sjf_material& M, int s, double f) {
@@ -1003,8 +1025,10 @@ int apply_materials(sjf_engine* e) {
     e->lut_h = e->given_lut;
     e->matw_h = e->given_w;
     bool any_pml = e->pml_on[0] || e->pml_on[1] || e->pml_on[2];
-    if (e->pml_fref > 0 && any_pml) {
-        std::vector<int> twin(ng, -1);
+    std::vector<double> fold_add(ng, 0.0);  // sum of the folded Lorentz poles' Re chi per material
+    std::vector<int> twin(ng, -1);
+    const bool fold = e->pml_fref > 0 && any_pml;
+    if (fold) {
         for (int m = 0; m < ng; ++m) {
             const sjf_material& M = e->given_lut[m];
             sjf_material T = M;
@@ -1024,6 +1048,7 @@ int apply_materials(sjf_engine* e) {
                 }
             }
             if (!has_lor) continue;
+            fold_add[m] = eps - M.eps_inf;
             T.eps_inf = eps >= 1.0 ? eps : M.eps_inf;
             for (int s = T.npoles; s < SJF_MAX_POLES; ++s) T.f0[s] = T.gamma[s] = T.sigma[s] = 0, T.is_drude[s] = 0;
             if ((int)e->lut_h.size() >= SJF_MAX_MATERIALS)
@@ -1041,6 +1066,21 @@ int apply_materials(sjf_engine* e) {
             e->matw_h[q] = w;
         }
     }
+    // per-cell eps_inf map: the given values, with the same fold inside the PML slabs
+    if (!e->emap_h.empty()) {
+        e->emap_eff = e->emap_h;
+        if (fold)
+            for (long q = 0; q < e->N; ++q) {
+                if (!cell_in_pml(e, q)) continue;
+                for (int c = 0; c < 3; ++c) {
+                    const int m = (e->given_w[q] >> (8 * c)) & 0xff;
+                    if (m >= ng || twin[m] < 0) continue;
+                    double& v = e->emap_eff[(size_t)c * e->N + q];
+                    if (v + fold_add[m] >= 1.0) v += fold_add[m];
+                }
+            }
+        if (int rc = upload_emap(e)) return rc;
+    }
     int maxp = 0;
     bool anylor = false;
     for (const sjf_material& M : e->lut_h) {
@@ -1050,7 +1090,7 @@ int apply_materials(sjf_engine* e) {
     }
     e->nmat = (int)e->lut_h.size();
     e->maxp = maxp;
-    e->uniform = (e->nmat == 1 && maxp == 0 && e->deps_h.empty());
+    e->uniform = (e->nmat == 1 && maxp == 0 && e->deps_h.empty() && e->emap_h.empty());
     int rc = dev_alloc(e, (void**)&e->matw, sizeof(uint32_t) * e->N);
     if (!rc) rc = h2d(e, e->matw, e->matw_h.data(), sizeof(uint32_t) * e->N);
     if (rc) return rc;
@@ -1111,6 +1151,7 @@ double engine_step_bytes(const sjf_engine* e) {
         b += 3 * p * (double)pc;
     }
     if (!e->deps_h.empty()) b += p * (double)e->deps_h.size();
+    if (!e->emap_h.empty()) b += 3 * p * (double)e->N;  // per-cell 1/eps_inf of the three E components
     // PML: per cell and PML axis, 2 E + 2 H auxiliaries, each read + written
     const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
     for (int d = 0; d < 3; ++d)
@@ -1684,6 +1725,7 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
 extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                                double* grad, size_t cap) {
     if (!e || nsteps < 1 || checkpoint_every < 0 || !gap || !J || !grad) return SJF_E_ARG;
+    if (!e->emap_h.empty()) return fail(e, SJF_E_UNSUPPORTED, "adjoint runs take the LUT's eps_inf (no eps maps)");
     if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
     if (e->step != 0) return fail(e, SJF_E_STATE, "the adjoint run starts from step 0");
     if (e->deps_h.empty()) return fail(e, SJF_E_STATE, "no design region (sjf_set_design_eps)");
@@ -1933,6 +1975,26 @@ int sjf_set_materials(sjf_engine* e, const uint32_t* mat_word, const sjf_materia
     e->given_lut.assign(lut, lut + n_mat);
     e->given_w.assign(mat_word, mat_word + e->N);
     e->have_mat = true;
+    return apply_materials(e);
+}
+
+int sjf_set_eps_maps(sjf_engine* e, const double* eps) {
+    if (!e) return SJF_E_ARG;
+    if (int rc = sync_engine(e)) return rc;
+    if (!eps) {
+        e->emap_h.clear();
+        e->emap_eff.clear();
+        dev_free(e->emap);
+    } else {
+        for (long q = 0; q < 3 * e->N; ++q)
+            if (!(eps[q] > 0) || !std::isfinite(eps[q])) return fail(e, SJF_E_ARG, "map eps must be positive and finite");
+        e->emap_h.assign(eps, eps + 3 * e->N);
+        if (!e->have_mat) {
+            e->given_w.assign(e->N, 0u);
+            e->given_lut = e->lut_h;
+            e->have_mat = true;
+        }
+    }
     return apply_materials(e);
 }
 
@@ -2499,6 +2561,11 @@ int sjf_batch_launch_info(const sjf_batch* b, int* threads, int* groups) {
 int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, int checkpoint_every, const int* gaps, double* J,
                           double* const* grads, const size_t* caps) {
     if (!b || nsteps < 1 || checkpoint_every < 0 || !gaps || !J || !grads || !caps) return SJF_E_ARG;
+    for (const sjf_engine* e : b->set->eng)
+        if (!e->emap_h.empty()) {
+            b->err = "adjoint runs take the LUT's eps_inf (no eps maps)";
+            return SJF_E_UNSUPPORTED;
+        }
     LaunchSet* s = b->set;
     sjf_engine* e0 = s->eng[0];
     const int n = (int)s->eng.size();

@@ -74,6 +74,7 @@ struct StepArgs {
     int coff[3];
     int dlo[3], dhi[3];
     const T* dinv;
+    const T* emap;  // per-cell 1/eps_inf [x][Ex Ey Ez][y][z] (sjf_set_eps_maps), or null: the LUT's
 };
 
 #define GLOBAL __attribute__((address_space(1)))
@@ -490,6 +491,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                         mm[c] = des[c] ? 0 : m;
                         inv[c] = slut[mm[c]].inv_eps;
                     }
+                    if (a.emap) {  // per-cell eps_inf (smoothed maps)
+                        const Rsrc re = rsrc((const char*)a.emap + (long)pi * 3 * PB);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) inv[c] = bld<T>(re, o, c * PB);
+                    }
                     if (wave_any(des[0] || des[1] || des[2])) {
                         // design voxels: 1/eps from the design array [dx][dy][dz]
                         const Rsrc rd = rsrc(a.dinv);
@@ -626,6 +632,7 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     }
     const int syl = ypml ? (j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1]) : 0;
     const int szl = zpml ? (k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) : 0;
+    lds_barrier();  // LUT in LDS: the first plane's pole loads read it before the plane's barriers
 
     const int x0 = xci * a.xc;
     const int x1 = min(x0 + a.xc, nx);
@@ -860,6 +867,11 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                         mm[c] = des[c] ? 0 : m;
                     }
                     inv[c] = slut[mm[c]].inv_eps;
+                }
+                if (a.emap) {  // per-cell eps_inf (smoothed maps)
+                    const Rsrc re = rsrc((const char*)a.emap + (long)pi * 3 * PB);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) inv[c] = bld<T>(re, wo, c * PB);
                 }
                 if (wave_any(des[0] || des[1] || des[2])) {
                     const Rsrc rd = rsrc(a.dinv);

@@ -121,6 +121,14 @@ class Engine:
     def set_pml(self, cells, R: float = 1e-15, power: float = 2.0):
         self._chk(self.lib.sjf_set_pml(self.h, _ip(cells), float(R), float(power)), "sjf_set_pml")
 
+    def set_eps_maps(self, eps):
+        """Per-cell eps_inf of Ex, Ey, Ez (array [3, nx, ny, nz]); None removes the map."""
+        if eps is None:
+            self._chk(self.lib.sjf_set_eps_maps(self.h, None), "sjf_set_eps_maps")
+            return
+        v = np.ascontiguousarray(eps, dtype=np.float64).reshape((3,) + tuple(self.shape))
+        self._chk(self.lib.sjf_set_eps_maps(self.h, _dp(v)), "sjf_set_eps_maps")
+
     def set_pml_dispersion(self, f_ref: float):
         """Run the PML slabs without Lorentz poles, folded into eps_inf at f_ref (0: keep them)."""
         self._chk(self.lib.sjf_set_pml_dispersion(self.h, float(f_ref)), "sjf_set_pml_dispersion")

@@ -40,7 +40,7 @@ def lib():
             "sjfh_source_info": (C.c_int, [cp, C.c_int, C.c_double, dp, dp]),
             "sjfh_fft": (C.c_int, [dp, C.c_long, dp, C.c_long, lp]),
             "sjfh_voxelize": (C.c_int, [cp, C.c_double, C.c_uint, C.c_double, C.c_double, C.c_double, C.c_int,
-                                        C.c_double, up, dp, C.c_int, ip]),
+                                        C.c_double, up, dp, C.c_int, ip, dp, ip]),
             "sjfh_run": (C.c_int, [cp, C.c_int, cpp, cp, C.c_int, C.c_int, C.c_int, dp]),
             "sjfh_write_field_samples": (C.c_int, [cp, cp, C.c_double, C.c_double, C.c_uint, C.c_uint, dp, C.c_long]),
             "sjfh_h5_read": (C.c_int, [cp, cp, cpp, C.c_int, dp, C.c_size_t, szp]),
@@ -154,20 +154,27 @@ def fft(x) -> np.ndarray:
     return o[:, 0] + 1j * o[:, 1]
 
 
-def voxelize(scene_json, n, dx, ambient_eps=1.0, smooth_n=0, smooth_rad=0.05, um_scale=1.0, resolution=None):
-    """Material words (n^3, z fastest) and the material table as dicts."""
+def voxelize(scene_json, n, dx, ambient_eps=1.0, smooth_n=0, smooth_rad=0.05, um_scale=1.0, resolution=None,
+             with_map=False):
+    """Material words (n^3, z fastest) and the material table as dicts; with_map=True also returns
+    the per-cell eps_inf map [3, n, n, n] (None without smoothing)."""
     words = np.zeros(n ** 3, dtype=np.uint32)
     mats = np.zeros(26 * 256)
+    emap = np.zeros(3 * n ** 3)
     nm = C.c_int()
+    has = C.c_int()
     res = resolution if resolution is not None else 1.0 / dx
     _chk(lib().sjfh_voxelize(_b(scene_json), ambient_eps, smooth_n, smooth_rad, um_scale, res, n, dx,
-                             words.ctypes.data_as(C.POINTER(C.c_uint)), _dp(mats), 256, C.byref(nm)), "sjfh_voxelize")
+                             words.ctypes.data_as(C.POINTER(C.c_uint)), _dp(mats), 256, C.byref(nm), _dp(emap),
+                             C.byref(has)), "sjfh_voxelize")
     lut = []
     for m in range(nm.value):
         d = mats[26 * m: 26 * (m + 1)]
         np_ = int(d[1])
         lut.append(dict(eps_inf=d[0], npoles=np_, f0=list(d[2:2 + np_]), gamma=list(d[6:6 + np_]),
                         sigma=list(d[10:10 + np_]), is_drude=[int(v) for v in d[14:14 + np_]]))
+    if with_map:
+        return words.reshape(n, n, n), lut, (emap.reshape(3, n, n, n) if has.value else None)
     return words.reshape(n, n, n), lut
 
 

@@ -58,9 +58,11 @@ BoundGeom::BoundGeom(const Settings& s, const Scene& scene, const RunOptions& op
     const int n3[3] = {n_, n_, n_};
     VoxelMaps vm = voxelize(scene, s, n3, dx);
     eps_cell_.resize(vm.words.size());
-    for (size_t q = 0; q < vm.words.size(); ++q) {
+    const size_t N = vm.words.size();
+    for (size_t q = 0; q < N; ++q) {
         double e = 0;
-        for (int c = 0; c < 3; ++c) e += vm.lut[(vm.words[q] >> (8 * c)) & 0xff].eps_inf;
+        for (int c = 0; c < 3; ++c)
+            e += vm.eps_map.empty() ? vm.lut[(vm.words[q] >> (8 * c)) & 0xff].eps_inf : vm.eps_map[c * N + q];
         eps_cell_[q] = e / 3;
     }
 
@@ -79,6 +81,7 @@ BoundGeom::BoundGeom(const Settings& s, const Scene& scene, const RunOptions& op
         if (sjf_create(&g, &eng_[p]))
             throw std::runtime_error(std::string("sjf_create: ") + sjf_last_error(nullptr));
         check(sjf_set_materials(eng_[p], vm.words.data(), vm.lut.data(), (int)vm.lut.size()), "sjf_set_materials", p);
+        if (!vm.eps_map.empty()) check(sjf_set_eps_maps(eng_[p], vm.eps_map.data()), "sjf_set_eps_maps", p);
         if (pc > 0) check(sjf_set_pml(eng_[p], cells, 1e-15, 2.0), "sjf_set_pml", p);
         check(sjf_set_graph(eng_[p], opt.graph_chunk), "sjf_set_graph", p);
     }

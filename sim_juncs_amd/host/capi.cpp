@@ -204,7 +204,8 @@ int sjfh_fft(const double* in, long n, double* out, long cap, long* n_out) {
 }
 
 int sjfh_voxelize(const char* scene_json, double ambient_eps, unsigned smooth_n, double smooth_rad, double um_scale,
-                  double resolution, int n, double dx, unsigned* words, double* mats, int cap_mat, int* n_mat) {
+                  double resolution, int n, double dx, unsigned* words, double* mats, int cap_mat, int* n_mat,
+                  double* eps_map, int* has_map) {
     try {
         const Scene sc = Scene::from_json_file(scene_json);
         Settings s;
@@ -221,6 +222,8 @@ int sjfh_voxelize(const char* scene_json, double ambient_eps, unsigned smooth_n,
             return -3;
         }
         std::memcpy(words, vm.words.data(), sizeof(unsigned) * vm.words.size());
+        if (has_map) *has_map = vm.eps_map.empty() ? 0 : 1;
+        if (eps_map && !vm.eps_map.empty()) std::memcpy(eps_map, vm.eps_map.data(), sizeof(double) * vm.eps_map.size());
         for (size_t m = 0; m < vm.lut.size(); ++m) {
             double* d = mats + 26 * m;
             const sjf_material& M = vm.lut[m];

@@ -236,6 +236,10 @@ VoxelMaps voxelize(const Scene& scene, const Settings& s, const int n[3], double
 
     static const double off[3][3] = {{0.5, 0, 0}, {0, 0.5, 0}, {0, 0, 0.5}};
     const int nx = n[0], ny = n[1], nz = n[2];
+    // smoothed inclusion fractions make eps_inf a continuum: it goes to a per-cell map and the LUT
+    // keys only the poles
+    const bool map_eps = s.smooth_n > 0;
+    if (map_eps) out.eps_map.assign(3 * N, 1.0);
     // each x plane collects its materials in first-seen order; planes merge in order afterwards
     std::vector<std::vector<Key>> plane_keys(nx);
     std::string error;
@@ -251,6 +255,10 @@ VoxelMaps voxelize(const Scene& scene, const Settings& s, const int n[3], double
                     Key key;
                     std::memset(&key, 0, sizeof key);
                     key.eps = eps_fn.in_bound(p);
+                    if (map_eps) {
+                        out.eps_map[(size_t)c * N + ((size_t)i * ny + j) * nz + k] = key.eps;
+                        key.eps = 1.0;
+                    }
                     for (size_t q = 0; q < poles.size(); ++q) {
                         const double sg = pole_fn[q].in_bound(p);
                         if (sg == 0) continue;

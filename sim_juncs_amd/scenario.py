@@ -32,6 +32,7 @@ class Scenario:
     part: int = 0
     matw: np.ndarray | None = None
     lut: list = field(default_factory=lambda: [M.VACUUM])
+    eps_maps: np.ndarray | None = None  # per-cell eps_inf [3, nx, ny, nz] (sjf_set_eps_maps)
     design: tuple | None = None          # (lo, hi, eps array)
     pml: tuple | None = None             # (cells[6], R, power)
     pml_fold: float = 0.0                # f_ref of sjf_set_pml_dispersion (0: Lorentz poles kept in the PML)
@@ -56,6 +57,8 @@ class Scenario:
     def apply(self, sim):
         if self.matw is not None:
             sim.set_materials(self.matw, self.lut)
+        if self.eps_maps is not None:
+            sim.set_eps_maps(self.eps_maps)
         if self.design is not None:
             sim.set_design_eps(*self.design)
         if self.pml is not None:

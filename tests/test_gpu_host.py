@@ -53,6 +53,67 @@ def test_driver_small_case_layout(tmp_path):
     assert all(np.all(np.isfinite(t)) for t in series)
 
 
+def test_driver_smoothed_materials_match_oracle(tmp_path):
+    """smooth_n > 0 with two roots (a Drude box and a dielectric slab): the smoothed, fractional
+    permittivity reaches the engines as per-cell maps (a LUT could not hold it), and the driver's
+    monitor series match the FP64 oracle run on the same voxelised maps and materials."""
+    with open(os.path.join(G, "scene_run.json")) as f:
+        d = json.load(f)
+    d["sources"][0]["component"] = 0
+    ident = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    box = {"kind": "box", "invert": 0, "trans": ident, "center": [0.95, 2.0, 2.0], "offset": [0.95, 2.0, 2.0]}
+    slab = {"kind": "box", "invert": 0, "trans": ident, "center": [2.0, 2.0, 2.6], "offset": [2.0, 2.0, 0.45]}
+    d["roots"][0]["tree"] = {"kind": "composite", "invert": 0, "trans": ident, "combine": "union",
+                             "children": [box, None]}
+    d["roots"][0]["metadata"]["susceptibilities"] = [[1.0, 0.48, 68.5971845, "drude"]]
+    d["roots"].append({"metadata": {"eps": 2.5, "color": 7},
+                       "tree": {"kind": "composite", "invert": 0, "trans": ident, "combine": "union",
+                                "children": [slab, None]}})
+    scene = str(tmp_path / "scene.json")
+    with open(scene, "w") as f:
+        json.dump(d, f)
+    with open(os.path.join(G, "conf", "run.conf")) as f:
+        text = f.read()
+    conf = str(tmp_path / "smooth.conf")
+    with open(conf, "w") as f:
+        f.write(text.replace("ambient_eps = 1.0", "ambient_eps = 1.0\nsmooth_n = 2\nsmooth_rad = 0.15"))
+    out = tmp_path / "out"
+    out.mkdir()
+    argv = ["--conf-file", conf, "--length", "2", "--grid-num", "21", "--out-dir", str(out)]
+    print("[case] driver smoothed materials 21^3 fp64 complex", flush=True)
+    s = host.run(conf, scene, argv, precision=8, complex_fields=True)
+    h5 = str(out / "field_samples.h5")
+    n_t = s["n_t_pts"]
+    locs = host.h5_read(h5, "cluster_0/locations", ["x", "y", "z"])
+    series = [host.h5_read(h5, f"cluster_0/point_{p}/time", ["Re", "Im"]) for p in range(2)]
+    L = 4.0
+    dx = L / 21
+    words, lut, emap = host.voxelize(scene, 21, dx, ambient_eps=1.0, smooth_n=2, smooth_rad=0.15, um_scale=2.0,
+                                     resolution=21 / L, with_map=True)
+    assert emap is not None and len(np.unique(emap)) > 4
+    info, p = host.source_info(scene, 0, um_scale=2.0)
+    ref = []
+    for part in (0, 1):
+        o = Oracle((21, 21, 21), dx, precision="fp64", part=part)
+        o.set_materials(words, lut_specs(lut))
+        o.set_eps_maps(emap)
+        o.set_pml([5] * 6, 1e-15, 2.0)
+        o.add_source(info["component"], [0, 0, 1, 4, 4, 1], kind="gaussian", freq=p["freq"], width=p["width"],
+                     phase=p["phase"], t_start=p["t_start"], t_end=p["t_end"], amp=info["amplitude"])
+        o.add_monitors("Ex", locs)
+        o.set_save_span(1)
+        o.step(n_t)
+        ref.append(o.read_monitors().real[:n_t])
+        o.close()
+    scale = max(np.abs(ref[0]).max(), np.abs(ref[1]).max())
+    assert scale > 1e-3
+    for pt in range(2):
+        for part in (0, 1):
+            assert np.abs(series[pt][:, part] - ref[part][:, pt]).max() <= 1e-10 * scale, (pt, part)
+    eps_dump = host.h5_read(str(out / "eps-000000.00.h5"), "eps")
+    assert np.allclose(eps_dump.reshape(21, 21, 21), emap.mean(axis=0), rtol=0, atol=1e-12)
+
+
 def test_driver_matches_oracle_with_materials(tmp_path):
     """The small case with an Ex sheet and a Drude box filling x < 1.9: the driver's monitor
     series (real and imaginary parts from two real runs) match the FP64 oracle driven with the

@@ -10,6 +10,7 @@ import pytest
 from oracle.oracle import Oracle
 from sim_juncs_amd import materials as M
 from sim_juncs_amd import scenario
+from sim_juncs_amd._lib import EngineError
 from sim_juncs_amd.engine import Engine
 
 pytestmark = pytest.mark.gpu
@@ -146,6 +147,36 @@ def test_pml_without_lorentz_poles_fp64(tile):
     assert rel(keep.read_field("Ex"), orc.read_field("Ex")) > 1e-6
     keep.close()
     eng.close()
+
+
+@pytest.mark.parametrize("precision,tile,fold", [("fp64", None, 0.0), ("fp64", ("thread_rows", 2, 3), 1.25),
+                                                  ("fp64", ("register_rows", 2, 4), 1.25), ("fp32", None, 1.25)])
+def test_eps_maps_match_oracle(precision, tile, fold):
+    """sjf_set_eps_maps: a per-cell eps_inf for each E component (the smoothed, fractional
+    permittivity) over random dispersive materials, with a design box on top and PML (with and
+    without the Lorentz fold, which adds to the map's value in the slabs): fields, monitors and a
+    CW source's injection (its coefficient takes 1/eps from the map) against the oracle's own per-
+    cell eps arrays.  Adjoint runs refuse a map."""
+    shape = (14, 12, 20)
+    scn, rng = random_scenario(shape, seed=17, pml=[2, 2, 3, 2, 3, 3], steps=120)
+    scn.precision = precision
+    scn.pml_fold = fold
+    scn.eps_maps = rng.uniform(1.0, 6.0, size=(3,) + shape)
+    lo, hi = (5, 4, 8), (9, 7, 12)
+    scn.design = (lo, hi, rng.uniform(1.0, 4.0, size=(4, 3, 4)))
+    pulse = scenario.cw_pulse(0.8, 0.6, 0.0, 5.0)
+    scn.sources = [scenario.plane_source(shape, scn.dx, 10, pulse)]
+    scn.monitors = [("Ex", np.array([[6.2 * scn.dx, 5.1 * scn.dx, 14.4 * scn.dx]]))]
+    eng, orc = run_pair(scn, tile=tile)
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    assert max(field_errors(eng, orc)) < tol
+    assert rel(eng.read_monitors(), orc.read_monitors()) < tol
+    eng.close()
+    e = scn.make(Engine)
+    with pytest.raises(EngineError, match="eps maps"):
+        e.adjoint_run(10, 0, (6, 5, 10), 48)
+    e.set_eps_maps(None)
+    e.close()
 
 
 def test_plain_box_fp32_within_tolerance():

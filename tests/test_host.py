@@ -233,6 +233,31 @@ def test_fft_closed_form():
 
 # ---- voxelisation (src/disp.cpp:482-550) --------------------------------------------------------
 
+def test_voxelize_with_smoothing_gives_eps_maps():
+    """With smooth_n > 0 the permittivity is the fractional sum over roots (src/disp.cpp:56-112,
+    264-283): voxelize returns it per cell and E component for sjf_set_eps_maps, equal at every
+    Yee point to the smoothed material function, and a LUT of the pole combinations alone."""
+    d = load("test")
+    n, L = 17, 1.0
+    dx = L / n
+    words, lut, emap = host.voxelize(scene_path("test"), n, dx, ambient_eps=1.5, smooth_n=2, smooth_rad=0.03,
+                                     um_scale=1.0, resolution=n / L, with_map=True)
+    assert emap is not None and emap.shape == (3, n, n, n)
+    assert all(m["eps_inf"] == 1.0 for m in lut) and len(lut) <= 255
+    assert len(np.unique(emap)) > 3  # fractional values at the boundaries
+    off = [(0.5, 0, 0), (0, 0.5, 0), (0, 0, 0.5)]
+    rng = np.random.default_rng(5)
+    cells = rng.integers(0, n, size=(300, 3))
+    for c in range(3):
+        pts = (cells + np.array(off[c])) * dx
+        v = host.in_bound(scene_path("test"), pts, def_ret=1.5, smooth_n=2, smooth_rad=0.03)
+        assert np.array_equal(emap[c][cells[:, 0], cells[:, 1], cells[:, 2]], v), c
+    # without smoothing: no map, eps_inf in the LUT as before
+    w0, lut0, m0 = host.voxelize(scene_path("test"), n, dx, ambient_eps=1.5, um_scale=1.0, resolution=n / L,
+                                 with_map=True)
+    assert m0 is None and any(m["eps_inf"] != 1.0 for m in lut0)
+
+
 def test_voxelize_bowtie_materials():
     d = load("bowtie")
     s = d["settings"]
