@@ -60,31 +60,25 @@ std::vector<Suscept> parse_susceptibilities(const Value& v, int* er) {
 // and all eight sign reflections of each point.  Seed: the reference's DEF_SEED 0xd9a28bf3.
 MaterialFunction::MaterialFunction(double def_ret, unsigned smooth_n, double smooth_rad)
     : def_ret_(def_ret), smooth_n_(smooth_n) {
-    const uint64_t seed = (uint64_t)(double)0xd9a28bf3;
-    std::seed_seq seeder{(uint32_t)(seed & 0x0000ffff), (uint32_t)((seed & 0xffff0000) >> 32)};
-    std::mt19937 gen(seeder);
-    std::normal_distribution<double> gaussian(0, smooth_rad);
-    std::uniform_real_distribution<double> unif(0.0, 1.0);
     if (smooth_n_ == 0) return;
+    const uint64_t key = (uint64_t)(double)0xd9a28bf3;
+    std::seed_seq seq{(uint32_t)(key & 0x0000ffff), (uint32_t)((key & 0xffff0000) >> 32)};
+    std::mt19937 rng(seq);
+    std::normal_distribution<double> radius(0, smooth_rad);
+    std::uniform_real_distribution<double> uniform01(0.0, 1.0);
     pts_.resize(24 * (size_t)smooth_n_);
-    for (unsigned i = 0; i < smooth_n_; ++i) {
-        const double theta_inv = unif(gen);
-        const double cos_theta = 1 - 2 * theta_inv;
-        const double sin_theta = 2 * std::sqrt(theta_inv * (1 - theta_inv));
-        const double phi = 2 * M_PI * unif(gen);
-        const double r = gaussian(gen);
-        const double x = r * sin_theta * std::cos(phi);
-        const double y = r * sin_theta * std::cos(phi);
-        const double z = r * cos_theta;
-        int j = 0;
-        for (int xf = -1; xf < 2; xf += 2)
-            for (int yf = -1; yf < 2; yf += 2)
-                for (int zf = -1; zf < 2; zf += 2) {
-                    pts_[24 * i + 3 * j] = x * xf;
-                    pts_[24 * i + 3 * j + 1] = y * yf;
-                    pts_[24 * i + 3 * j + 2] = z * zf;
-                    ++j;
-                }
+    double* out = pts_.data();
+    for (unsigned p = 0; p < smooth_n_; ++p) {
+        // draw order: polar parameter, azimuth, radius
+        const double u = uniform01(rng);
+        const double mu = 1 - 2 * u;                       // cosine of the polar angle
+        const double rho = 2 * std::sqrt(u * (1 - u));     // its sine
+        const double az = 2 * M_PI * uniform01(rng);
+        const double rad = radius(rng);
+        const double base[3] = {rad * rho * std::cos(az), rad * rho * std::cos(az), rad * mu};
+        // reflections in the order x sign, then y, then z (bit set: +, clear: -)
+        for (int refl = 0; refl < 8; ++refl)
+            for (int d = 0; d < 3; ++d) *out++ = ((refl >> (2 - d)) & 1) ? base[d] : -base[d];
     }
     smooth_n_ *= 8;
 }
