@@ -219,6 +219,11 @@ int sjf_timed_steps(sjf_engine* e, long n, double* ms_total, double* ms_kernel_m
 /* Replaces output_hdf5 (src/disp.cpp:696,735) as the data source: copies a whole component
  * (nx*ny*nz values, converted to FP64) to host memory. */
 int sjf_read_field(sjf_engine* e, int comp, double* out);
+/* The cell box [lo, hi) of one component (output_hdf5 over a sub-volume, src/disp.cpp:696,735
+ * passes vol.surroundings(), the whole grid, which sjf_read_field covers), converted to FP64,
+ * row-major over (x, y, z) of the box.  An empty box copies nothing; a box outside the grid is
+ * SJF_E_ARG. */
+int sjf_read_field_box(sjf_engine* e, int comp, const int lo[3], const int hi[3], double* out);
 /* Sets a whole component from host FP64 values (initial conditions, tests). */
 int sjf_write_field(sjf_engine* e, int comp, const double* in);
 

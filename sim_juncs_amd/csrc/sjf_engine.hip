@@ -952,6 +952,24 @@ int read_component(sjf_engine* e, int comp, double* dst) {
     return SJF_OK;
 }
 
+// Copies the cell box [lo, hi) of one component: one strided 2-D copy (y rows of the box's z
+// span) per x plane of the box into a pinned-free host staging array, then converts to FP64.
+template <typename T>
+int read_component_box(sjf_engine* e, int comp, const int* lo, const int* hi, double* dst) {
+    const size_t plane = sizeof(T) * (size_t)e->g.ny * e->g.nz;
+    const size_t bx = (size_t)(hi[0] - lo[0]), by = (size_t)(hi[1] - lo[1]), bz = (size_t)(hi[2] - lo[2]);
+    std::vector<T> tmp(bx * by * bz);
+    const char* base = (const char*)field_set(e) + plane * comp;
+    for (size_t i = 0; i < bx; ++i) {
+        const char* src = base + 6 * plane * (size_t)(lo[0] + i) + sizeof(T) * ((size_t)lo[1] * e->g.nz + lo[2]);
+        HIPCHK(e, hipMemcpy2DAsync(tmp.data() + i * by * bz, sizeof(T) * bz, src, sizeof(T) * (size_t)e->g.nz,
+                                   sizeof(T) * bz, by, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (size_t q = 0; q < tmp.size(); ++q) dst[q] = (double)tmp[q];
+    return SJF_OK;
+}
+
 template <typename T>
 int upload_lut_t(sjf_engine* e) {
     std::vector<sjf::MatLut<T>> l(e->nmat);
@@ -2372,6 +2390,18 @@ int sjf_read_field(sjf_engine* e, int comp, double* out) {
     if (!e || !out || comp < 0 || comp > 5) return SJF_E_ARG;
     if (int rc = sync_engine(e)) return rc;
     return e->prec == 4 ? read_component<float>(e, comp, out) : read_component<double>(e, comp, out);
+}
+
+int sjf_read_field_box(sjf_engine* e, int comp, const int lo[3], const int hi[3], double* out) {
+    if (!e || !lo || !hi || comp < 0 || comp > 5) return SJF_E_ARG;
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] < 0 || hi[d] > n[d] || lo[d] > hi[d]) return fail(e, SJF_E_ARG, "field box outside the grid");
+    if (lo[0] == hi[0] || lo[1] == hi[1] || lo[2] == hi[2]) return SJF_OK;  // empty box: nothing to copy
+    if (!out) return SJF_E_ARG;
+    if (int rc = sync_engine(e)) return rc;
+    return e->prec == 4 ? read_component_box<float>(e, comp, lo, hi, out)
+                        : read_component_box<double>(e, comp, lo, hi, out);
 }
 
 int sjf_write_field(sjf_engine* e, int comp, const double* in) {

@@ -252,9 +252,17 @@ class Engine:
         return a.value, b.value
 
     # ------------------------------------------------------------------ fields ------------
-    def read_field(self, comp) -> np.ndarray:
-        out = np.zeros(self.shape)
-        self._chk(self.lib.sjf_read_field(self.h, comp_index(comp), _dp(out)), "sjf_read_field")
+    def read_field(self, comp, lo=None, hi=None) -> np.ndarray:
+        """A whole component, or with lo/hi the cell box [lo, hi) of it (FP64, shape of the box)."""
+        if lo is None and hi is None:
+            out = np.zeros(self.shape)
+            self._chk(self.lib.sjf_read_field(self.h, comp_index(comp), _dp(out)), "sjf_read_field")
+            return out
+        lo = np.ascontiguousarray(lo if lo is not None else (0, 0, 0), dtype=np.int32)
+        hi = np.ascontiguousarray(hi if hi is not None else self.shape, dtype=np.int32)
+        out = np.zeros(tuple(max(int(h) - int(l), 0) for l, h in zip(lo, hi)))
+        self._chk(self.lib.sjf_read_field_box(self.h, comp_index(comp), _ip(lo), _ip(hi), _dp(out)),
+                  "sjf_read_field_box")
         return out
 
     def write_field(self, comp, values: np.ndarray):

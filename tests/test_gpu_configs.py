@@ -17,7 +17,8 @@ themselves, each bounded to about a minute of GPU time:
   engine's own run, and FP32 gradients stay within 1e-4 of FP64 ones;
 * the 256x256x128 adjoint junction (64x64x16 design, 4000 steps, checkpoint every 100): the
   FP64 and FP32 gradients against central differences of the FP64 forward run at 32 seeded
-  design voxels, h = 1e-3 eps.
+  design voxels, h = 1e-3 eps, and along a random direction over all design voxels (Taylor
+  remainder of second order, Richardson-extrapolated central difference).
 """
 import numpy as np
 import pytest
@@ -343,3 +344,35 @@ def test_adjoint_junction_gradient_matches_central_differences(c4_gradients, par
         print(f"  voxel {v}: fd {fd:.6e} fp64 {g64[v]:.6e} fp32 {g32[v]:.6e}", flush=True)
         assert abs(g64[v] - fd) <= 1e-6 * gmax + 1e-6 * abs(fd), (v, g64[v], fd)
         assert abs(g32[v] - fd) <= 1e-4 * gmax, (v, g32[v], fd)
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_junction_gradient_along_a_random_direction(c4_gradients):
+    """Dot-product check of the whole reverse pass: for a seeded random direction v over all
+    65536 design voxels, the adjoint's g.v equals the derivative of the FP64 forward J along v.
+    The Taylor remainder r(h) = J(eps + h v) - J(eps) - h g.v must fall as h^2 (ratio ~4 per
+    halving of h at the smallest steps; a gradient wrong in any direction component leaves an
+    O(h) part, ratio ~2), and the Richardson-extrapolated central difference quotient must agree
+    with g.v to 1e-6 (the FP32 adjoint's g.v to 1e-4)."""
+    scn, out, _ = c4_gradients
+    eps = scn.design[2]
+    _, g64 = out["fp64"]
+    _, g32 = out["fp32"]
+    rng = np.random.default_rng(11)
+    v = (rng.standard_normal(eps.size) * eps.reshape(-1)).reshape(eps.shape)
+    gv = float(g64 @ v.reshape(-1))
+    gv32 = float(g32 @ v.reshape(-1))
+    J0 = forward_J(scn, eps)
+    hs = [1e-2, 5e-3, 2.5e-3, 1.25e-3]
+    Jp = [forward_J(scn, eps + h * v) for h in hs]
+    Jm = [forward_J(scn, eps - h * v) for h in hs]
+    r = [abs(jp - J0 - h * gv) for jp, h in zip(Jp, hs)]
+    cd = [(jp - jm) / (2 * h) for jp, jm, h in zip(Jp, Jm, hs)]
+    rich = (4 * cd[-1] - cd[-2]) / 3
+    print(f"  J {J0:.9e}  g.v {gv:.9e}  fp32 g.v {gv32:.9e}", flush=True)
+    for h, ri, ci in zip(hs, r, cd):
+        print(f"  h {h:.2e}: remainder {ri:.3e}  central {ci:.9e} ({abs(ci - gv) / abs(gv):.2e})", flush=True)
+    print(f"  Richardson {rich:.12e} ({abs(rich - gv) / abs(gv):.2e})", flush=True)
+    assert 3.5 < r[-2] / r[-1] < 4.5, r
+    assert abs(rich - gv) <= 1e-6 * abs(gv)
+    assert abs(gv32 - gv) <= 1e-4 * abs(gv)

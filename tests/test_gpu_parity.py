@@ -357,3 +357,26 @@ def test_field_set_beyond_4gib_matches_small_grid(tile):
     assert np.array_equal(m0, m1)
     for a, b in zip(f0, f1):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_read_field_box_is_the_slice_of_the_whole_component(precision):
+    """sjf_read_field_box (output_hdf5 over a sub-volume) returns exactly the box of the
+    whole-component read-out, for interior, edge and whole-grid boxes; an empty box copies
+    nothing and a box outside the grid is an argument error."""
+    scn = early_pulse(scenario.plain_box(n=21, steps=30, precision=precision, pml=3))
+    eng = scn.make(Engine)
+    eng.step(30)
+    for c in range(6):
+        whole = eng.read_field(c)
+        assert np.abs(whole).max() > 0 or c >= 3
+        for lo, hi in [((0, 0, 0), (21, 21, 21)), ((3, 5, 7), (10, 6, 20)), ((20, 0, 0), (21, 21, 1)),
+                       ((0, 17, 4), (21, 21, 5))]:
+            box = eng.read_field(c, lo, hi)
+            assert box.shape == tuple(h - l for l, h in zip(lo, hi))
+            assert np.array_equal(box, whole[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]])
+    assert eng.read_field("Ex", (4, 4, 4), (4, 9, 9)).size == 0
+    for lo, hi in [((0, 0, 0), (22, 21, 21)), ((-1, 0, 0), (3, 3, 3)), ((5, 5, 5), (4, 6, 6))]:
+        with pytest.raises(EngineError):
+            eng.read_field("Ey", lo, hi)
+    eng.close()
