@@ -7,7 +7,10 @@
 // This is the E-direct form of meep's D/E split (src/disp.cpp:740 fields.step(): B, H, D,
 // polarisation, E): with D = eps_inf E + sum P, D^{n+1} - D^n = dt curl H, and
 // Q = P^{n+1} - P^n the textbook Lorentz/Drude recurrence becomes the (P, Q) pair above,
-// which needs no D array and, for Drude poles, no P array.
+// which needs no D array and, for Drude poles, no P array.  A Lorentz pole keeps P^n and
+// P^{n-1} (Q^n = P^n - P^{n-1}) in two slots whose roles alternate with the step parity, so
+// each step reads both and writes only P^{n+1} over P^{n-1}: 3 values moved per pole and
+// component instead of 4 for a stored (P, Q).  A Drude pole keeps Q^n alone (2 values).
 //
 // Layout in HBM: each component is a dense nx*ny*nz array, z fastest.  E and H and the H
 // auxiliaries are double-buffered (read buffer 0, write buffer 1, swapped every step) because
@@ -183,6 +186,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
+    // Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in the P
+    // array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
+    const bool podd = (step & 1) != 0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tyi = t % a.nty, xci = t / a.nty;
     const int j0 = tyi * TY;
@@ -519,11 +525,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                             const MatLut<T>& L = slut[mm[c]];
                             const bool vq = s < npl[c];
                             const bool vp = vq && ((L.lorentz_mask >> s) & 1);
-                            // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
-                            T qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
+                            const T cur = podd ? Pv[r][c][s] : Qv[r][c][s];  // Lorentz: P^n
+                            const T old = podd ? Qv[r][c][s] : Pv[r][c][s];  // and P^{n-1}
+                            T qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
+                                      : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
                             qn = vq ? qn : T(0);
-                            bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
-                            bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
+                            bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
+                            bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
                             sq[c] += qn;
                         }
                     }
@@ -579,6 +587,9 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
+    // Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in the P
+    // array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
+    const bool podd = (step & 1) != 0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int RT = a.rt, R = RT - 1;
     const int tid = threadIdx.x;
@@ -892,11 +903,13 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                         const MatLut<T>& L = slut[mm[c]];
                         const bool vq = s < npl[c];
                         const bool vp = vq && ((L.lorentz_mask >> s) & 1);
-                        // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
-                        T qn = L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c] + L.beta[s] * Pv[c][s];
+                        const T cur = podd ? Pv[c][s] : Qv[c][s];  // Lorentz: P^n
+                        const T old = podd ? Qv[c][s] : Pv[c][s];  // and P^{n-1}
+                        T qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[c]
+                                  : L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c];
                         qn = vq ? qn : T(0);
-                        bst(rP, vp ? wo : kOOB, (s * 3 + c) * PB, Pv[c][s] + qn);
-                        bst(rQ, vq ? wo : kOOB, (s * 3 + c) * PB, qn);
+                        bst(rP, (vp && !podd) ? wo : kOOB, (s * 3 + c) * PB, cur + qn);
+                        bst(rQ, (vq && !(vp && !podd)) ? wo : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
                         sq[c] += qn;
                     }
                 }
