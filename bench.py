@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--checkpoint-every", type=int, default=0,
                    help="adjoint workload: checkpoint every K steps and recompute (0: record the design "
                         "fields of every step, no recomputation; both give the same bits)")
+    p.add_argument("--adjoint-tile", default=None,
+                   help="adjoint workloads: tile of the fused reverse step as form,rows,xchunk "
+                        "(default: the engine's)")
     p.add_argument("--batch-tile", default=None,
                    help="tile of a batch's fused forward launch as form,xchunk (sweep, variants)")
     return p.parse_args()
@@ -240,6 +243,9 @@ def adjoint(a, variants=False):
     for s in scns:
         eng = s.make(Engine)
         set_tile(a, eng)  # the forward steps and the reverse steps run the tuned x-chunk
+        if a.adjoint_tile:
+            f, r, x = a.adjoint_tile.split(",")
+            eng.set_adjoint_tile(f, int(r), int(x))
         nd = s.design[2].size
         t = time.perf_counter()
         J, g = eng.adjoint_run(steps, a.checkpoint_every, s.meta["gap"], nd)
@@ -253,7 +259,8 @@ def adjoint(a, variants=False):
         eng.close()
     s0 = scns[0]
     workload = ("variants_" if variants else "") + s0.name
-    rl = roofline(rev[0], rev[1], "k_adj_step", rev[2], rev[3], workload)
+    rkernel = "k_adj_rows" if a.adjoint_tile and a.adjoint_tile.startswith("thread_rows") else "k_adj_step"
+    rl = roofline(rev[0], rev[1], rkernel, rev[2], rev[3], workload)
     cpu = None if a.no_cpu else cpu_baseline([s0], a.cpu_steps or cpu_budget_steps(s0.cells), s0.steps)
     if cpu:
         # the adjoint costs a forward and a reverse sweep per step (the design fields of every

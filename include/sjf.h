@@ -266,6 +266,13 @@ int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
  * ends in its final forward state. */
 int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                     double* grad, size_t cap);
+/* Tile of the fused reverse step of sjf_adjoint_run: SJF_TILE_REGISTER_ROWS (rows = 1: one y row
+ * per workgroup, the default) or SJF_TILE_THREAD_ROWS (`rows` owned y rows of nz threads each
+ * plus one halo row per workgroup, the y and z neighbours exchanged through LDS; needs
+ * (rows + 1) * nz <= 1024 and the exchange planes within 64 KiB of LDS), marching over `xchunk`
+ * x planes (0: enough chunks for about 16384 workgroups).  The gradient is the same either way
+ * (the same arithmetic per cell).  New (tuning). */
+int sjf_set_adjoint_tile(sjf_engine* e, int form, int rows, int xchunk);
 /* The fused reverse-step kernel of the last sjf_adjoint_run: mean duration in ms (HIP events on
  * the engine's stream around the first min(nsteps, 64) reverse launches), threads per workgroup
  * and workgroups.  SJF_E_STATE before any adjoint run.  It moves the same algorithmic bytes per

@@ -34,7 +34,7 @@ EXPORTS = [
     "sjf_launch_info", "sjf_set_tile", "sjf_copy_probe", "sjf_set_graph",
     "sjf_batch_create", "sjf_batch_set_tile", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
     "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
-    "sjf_adjoint_kernel_info", "sjf_batch_launch_info", "sjf_batch_adjoint_run", "sjf_batch_adjoint_kernel_info",
+    "sjf_adjoint_kernel_info", "sjf_set_adjoint_tile", "sjf_batch_launch_info", "sjf_batch_adjoint_run", "sjf_batch_adjoint_kernel_info",
     "sjf_dft_flux", "sjf_reserve_steps", "sjf_batch_reserve_steps", "sjf_dft_flux_noalloc",
     "sjf_set_pml_dispersion", "sjf_set_eps_maps",
 ]
@@ -130,6 +130,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_batch_destroy": (C.c_int, [vp]),
         "sjf_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, dp, C.c_size_t]),
         "sjf_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),
+        "sjf_set_adjoint_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "sjf_batch_launch_info": (C.c_int, [vp, ip, ip]),
         "sjf_batch_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, C.POINTER(dp), C.POINTER(C.c_size_t)]),
         "sjf_batch_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),

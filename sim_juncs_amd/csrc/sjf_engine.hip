@@ -118,6 +118,9 @@ struct sjf_engine {
     // fused reverse-step kernel of the last adjoint run: mean duration and launch shape
     double adj_ms = 0;
     int adj_threads = 0, adj_groups = 0;
+    // its tile (sjf_set_adjoint_tile): form, owned rows of a row-of-threads tile, x-chunk (0: default)
+    int adj_form = SJF_TILE_REGISTER_ROWS;
+    int adj_rows = 0, adj_xc = 0;
 
     LaunchSet* own = nullptr;
     sjf_batch* batch = nullptr;
@@ -1257,16 +1260,31 @@ void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned 
     else hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 1024>), dim3(tiles), dim3(block), lds, e->stream, g);
 }
 
+template <typename T, int NP>
+void launch_adj_rows_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
+    if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_rows<T, NP, 256>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else if (block <= 512) hipLaunchKernelGGL((sjf::k_adj_rows<T, NP, 512>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else hipLaunchKernelGGL((sjf::k_adj_rows<T, NP, 1024>), dim3(tiles), dim3(block), lds, e->stream, g);
+}
+
 template <typename T>
-int launch_adj_step(sjf_engine* e, const sjf::AdjArgs<T>& g, int npv, int block, unsigned tiles, size_t lds) {
+int launch_adj_step(sjf_engine* e, const sjf::AdjArgs<T>& g, int npv, int block, unsigned tiles, size_t lds,
+                    bool rows) {
     switch (npv) {
-        case 0: launch_adj_np<T, 0>(e, g, block, tiles, lds); break;
-        case 1: launch_adj_np<T, 1>(e, g, block, tiles, lds); break;
-        case 3: launch_adj_np<T, 3>(e, g, block, tiles, lds); break;
-        default: launch_adj_np<T, 4>(e, g, block, tiles, lds); break;
+        case 0: rows ? launch_adj_rows_np<T, 0>(e, g, block, tiles, lds) : launch_adj_np<T, 0>(e, g, block, tiles, lds); break;
+        case 1: rows ? launch_adj_rows_np<T, 1>(e, g, block, tiles, lds) : launch_adj_np<T, 1>(e, g, block, tiles, lds); break;
+        case 3: rows ? launch_adj_rows_np<T, 3>(e, g, block, tiles, lds) : launch_adj_np<T, 3>(e, g, block, tiles, lds); break;
+        default: rows ? launch_adj_rows_np<T, 4>(e, g, block, tiles, lds) : launch_adj_np<T, 4>(e, g, block, tiles, lds); break;
     }
     HIPCHK(e, hipGetLastError());
     return SJF_OK;
+}
+
+// LDS of the reverse step: E-term and H-term exchange planes and the material table
+template <typename T>
+size_t adj_lds_bytes(const sjf_engine* e, bool rows, int block) {
+    const size_t lut = sizeof(sjf::MatLut<T>) * e->nmat;
+    return rows ? sizeof(T) * (8 * (size_t)block + 2 * (size_t)e->g.nz) + lut : sizeof(T) * 12 * (size_t)block + lut;
 }
 
 template <typename T>
@@ -1359,17 +1377,25 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     // ---- backward pass
     sjf::AdjArgs<T> g;
     std::memset(&g, 0, sizeof(g));
-    g.f = step_args<T>(e, 1, adj_xchunk(e));  // one y row per workgroup
-    g.f.nty = e->g.ny;
+    // register form: one y row per workgroup; row-of-threads form: adj_rows owned rows
+    const bool rows = e->adj_form == SJF_TILE_THREAD_ROWS;
+    const int arows = rows ? e->adj_rows : 1;
+    int axc = e->adj_xc > 0 ? std::min(e->adj_xc, e->g.nx) : adj_xchunk(e);
+    if (rows && e->adj_xc == 0) {  // about 16384 workgroups over the tile rows
+        const long trows = (e->g.ny + arows - 1) / arows;
+        const long nxc = std::min<long>(std::max(1L, (16384 + trows - 1) / trows), e->g.nx);
+        axc = (int)((e->g.nx + nxc - 1) / nxc);
+    }
+    g.f = step_args<T>(e, arows, axc);
     g.A[0] = A;
     g.A[1] = A + (size_t)6 * e->N;
     g.rd = 0;
     g.Aq = Aq;
     g.Ap = Ap;
     for (int d = 0; d < 3; ++d) g.Apsi[d] = Apsi[d];
-    const int block = ((e->g.nz + 63) / 64) * 64;
-    const unsigned tiles = (unsigned)((long)e->g.ny * ((e->g.nx + g.f.xc - 1) / g.f.xc));
-    const size_t lds = sizeof(T) * 12 * block + sizeof(sjf::MatLut<T>) * e->nmat;
+    const int block = (((rows ? arows + 1 : 1) * e->g.nz + 63) / 64) * 64;
+    const unsigned tiles = (unsigned)((long)g.f.nty * ((e->g.nx + g.f.xc - 1) / g.f.xc));
+    const size_t lds = adj_lds_bytes<T>(e, rows, block);
     if (lds > 65536) return bail(fail(e, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
     const int npv = np_variant_of(e->maxp, e->uniform);
     auto ebar = [&]() { return g.A[g.rd]; };
@@ -1409,7 +1435,7 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
             g.ew = (n & 1) ? 6 : 0;
             const bool timed = ntimed < kTimed;
             if (timed) HIPCHK(e, hipEventRecord(evs[2 * ntimed], e->stream));
-            int r = launch_adj_step<T>(e, g, npv, block, tiles, lds);
+            int r = launch_adj_step<T>(e, g, npv, block, tiles, lds, rows);
             if (r) return r;
             if (timed) HIPCHK(e, hipEventRecord(evs[2 * ntimed++ + 1], e->stream));
             g.rd = 1 - g.rd;
@@ -1756,6 +1782,28 @@ extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every,
     const long q = cell_index(e, gap[0], gap[1], gap[2]);
     return e->prec == 4 ? adjoint_run_t<float>(e, nsteps, checkpoint_every, q, J, grad, nd)
                         : adjoint_run_t<double>(e, nsteps, checkpoint_every, q, J, grad, nd);
+}
+
+extern "C" int sjf_set_adjoint_tile(sjf_engine* e, int form, int rows, int xchunk) {
+    if (!e || xchunk < 0) return SJF_E_ARG;
+    if (form == SJF_TILE_REGISTER_ROWS) {
+        if (rows != 1) return fail(e, SJF_E_ARG, "the register form of the reverse step owns one row per workgroup");
+    } else if (form == SJF_TILE_THREAD_ROWS) {
+        if (!thread_rows_fit(e, rows))
+            return fail(e, SJF_E_ARG, "row-of-threads tile needs (rows + 1) * nz <= 1024");
+        const int block = (((rows + 1) * e->g.nz + 63) / 64) * 64;
+        // FP64 tiles stop at 512 threads: at 1024 the register budget of 128 per thread spills
+        if (e->prec == 8 && block > 512)
+            return fail(e, SJF_E_ARG, "FP64 row-of-threads reverse tile needs (rows + 1) * nz <= 512");
+        const size_t lds = e->prec == 4 ? adj_lds_bytes<float>(e, true, block) : adj_lds_bytes<double>(e, true, block);
+        if (lds > 65536) return fail(e, SJF_E_ARG, "adjoint row-of-threads tile needs more than 64 KiB of LDS");
+    } else {
+        return fail(e, SJF_E_ARG, "unknown tile form");
+    }
+    e->adj_form = form;
+    e->adj_rows = rows;
+    e->adj_xc = xchunk;
+    return SJF_OK;
 }
 
 extern "C" int sjf_adjoint_kernel_info(const sjf_engine* e, double* ms_mean, int* threads, int* groups) {

@@ -1792,6 +1792,364 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     }
 }
 
+// One tile of the fused reverse step, row-of-threads form.  A workgroup holds a.rt rows of nz
+// threads (one cell of a y row per thread): thread row 0 is the row below the tile, rows
+// 1..rt-1 own rows j0..j0+rt-2, and the top thread row also forms the y-terms of the row above
+// the tile.  The y and z neighbours' E-terms (for Hbar) and H-terms (for Ebar) pass through LDS,
+// so each adjoint value is loaded once per tile and the y halo costs one row per rt-1 owned
+// rows.  The arithmetic of every cell is that of adj_tile.
+template <typename T, int NP>
+__device__ __forceinline__ void adj_rows(const AdjArgs<T>& g, const int t) {
+    constexpr uint32_t TS = sizeof(T);
+    const StepArgs<T>& a = g.f;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int B = blockDim.x;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int RT = a.rt, R = RT - 1;
+    T* sE = reinterpret_cast<T*>(smem_raw);  // [4][B]: E-terms z (Ex->Hy), z (Ey->Hx), y (Ex->Hz), y (Ez->Hx)
+    T* sEt = sE + 4 * B;                     // [2][nz]: y-terms of the row above the tile
+    T* sD = sEt + 2 * nz;                    // [4][B]: H-terms z (Hy->Ex), z (Hx->Ey), y (Hx->Ez), y (Hz->Ex)
+    MatLut<T>* slut = reinterpret_cast<MatLut<T>*>(sD + 4 * B);
+    {
+        GLOBAL const uint32_t* src = (GLOBAL const uint32_t*)a.lut;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(slut);
+        const int words = (int)(sizeof(MatLut<T>) / 4) * a.nmat;
+        for (int q = threadIdx.x; q < words; q += B) dst[q] = src[q];
+    }
+    const char* AR = (const char*)g.A[g.rd];
+    const char* AW = (const char*)g.A[1 - g.rd];
+    const char* matw = (const char*)a.matw;
+    GLOBAL const T* coef = (GLOBAL const T*)a.coef;
+    const int tid = threadIdx.x;
+    const bool act = tid < RT * nz;
+    const int r = act ? tid / nz : 0;
+    const int k = act ? tid - r * nz : 0;
+    const int tyi = t % a.nty, xci = t / a.nty;
+    const int x0 = xci * a.xc, x1 = min(x0 + a.xc, nx);
+    const int jr = tyi * R + r - 1;  // unwrapped global row of this thread row
+    const int j = act ? wrap_or_ghost(jr, ny, a.py) : -1;
+    const bool rv = j >= 0;
+    const bool owned = act && r >= 1 && jr < ny;  // implies rv
+    const bool top = act && r == RT - 1;
+    const int jt = top ? wrap_or_ghost(jr + 1, ny, a.py) : -1;
+    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz), km1 = wrap_or_ghost(k - 1, nz, a.pz);
+    const T cdt = a.cdt;
+    const bool uniform = a.uniform != 0;
+    const uint32_t er = g.er, ew = g.ew, hr = g.hr, hw = g.hw;
+    const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
+    const long FPB = 6L * PB, QPB = (long)a.nqp * PB, PW = (long)ny * nz * 4;
+    const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
+    const uint32_t SB0 = (uint32_t)(a.S[0] * TS), SB1 = (uint32_t)(a.S[1] * TS), SB2 = (uint32_t)(a.S[2] * TS);
+    const Rsrc rpx = rsrc(g.Apsi[0]), rpy = rsrc(g.Apsi[1]), rpz = rsrc(g.Apsi[2]);
+    const bool ypml = rv && in_pml(a, 1, j, ny);
+    const int sy = ypml ? (j < a.plo[1] ? j : j - (ny - a.phi[1]) + a.plo[1]) : 0;
+    const bool ypmlt = jt >= 0 && in_pml(a, 1, jt, ny);
+    const int syt = ypmlt ? (jt < a.plo[1] ? jt : jt - (ny - a.phi[1]) + a.plo[1]) : 0;
+    const uint32_t oR = rv ? (uint32_t)(j * nz + k) * TS : kOOB;
+    const uint32_t oW = owned ? (uint32_t)(j * nz + k) * 4u : kOOB;
+    const uint32_t oT = jt >= 0 ? (uint32_t)(jt * nz + k) * TS : kOOB;
+    const bool zpml = act && in_pml(a, 2, k, nz);
+    const bool zany = wave_any(zpml);
+    const uint32_t oK = act ? (uint32_t)k * TS : kOOB;
+    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : kOOB;
+    GLOBAL const T* cy = coef + a.coff[1];
+    GLOBAL const T* cz = coef + a.coff[2];
+    GLOBAL const T* cx = coef + a.coff[0];
+    const T bEz = zpml ? cz[k] : T(1), bHz = zpml ? cz[2 * nz + k] : T(1);
+    const T bEy = ypml ? cy[j] : T(1), bHy = ypml ? cy[2 * ny + j] : T(1);
+    const T bEyt = ypmlt ? cy[jt] : T(1);
+    const Rsrc rd = rsrc(a.dinv);
+    const int dey = a.dhi[1] - a.dlo[1], dez = a.dhi[2] - a.dlo[2];
+
+    const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
+    // u = Ubar' of this thread's cell on planes i and i + 1, the top row's (Ubar'x, Ubar'z) of
+    // the row above, and the owned cell's material word
+    T uc[3], un[3], tc[2], tn[2];
+    uint32_t wc = 0u, wn = 0u;
+    auto load_u = [&](int p, T (&u)[3], T (&ut)[2], uint32_t& w) {
+        const int pp = wrap_or_ghost(p, nx, a.px);
+        const long pc = pp >= 0 ? pp : 0;
+        const Rsrc rf = rsrc(AR + pc * FPB);
+        const uint32_t o = pp >= 0 ? oR : kOOB;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c] = bld<T>(rf, o, c * PB);
+        ut[0] = ut[1] = T(0);
+        if (top) {
+            const uint32_t ot = pp >= 0 ? oT : kOOB;
+            ut[0] = bld<T>(rf, ot, 0);
+            ut[1] = bld<T>(rf, ot, 2 * PB);
+        }
+        w = uniform ? 0u : bld<uint32_t>(rsrc(matw + pc * PW), pp >= 0 ? oW : kOOB, 0);
+    };
+    lds_barrier();  // LUT in LDS
+    load_u(istart, uc, tc, wc);
+    T dbx_prev[2] = {T(0), T(0)};  // x H-terms of this row on plane i - 1: (Hz -> Ey), (Hy -> Ez)
+
+    for (int i = istart; i < x1; ++i) {
+        const bool own_plane = i >= x0;
+        const int pi = wrap_or_ghost(i, nx, a.px);
+        const int pn = wrap_or_ghost(i + 1, nx, a.px);
+        load_u(i + 1, un, tn, wn);
+        const bool xpml = in_pml(a, 0, pi, nx);
+        const bool xpmln = pn >= 0 && in_pml(a, 0, pn, nx);
+        const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
+        const uint32_t sxn = xpmln ? (uint32_t)(pn < a.plo[0] ? pn : pn - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
+        const T bEx = xpml ? cx[pi] : T(1), bHx = xpml ? cx[2 * nx + pi] : T(1);
+        const T bExn = xpmln ? cx[pn] : T(1);
+        const Rsrc rA = rsrc(AR + (long)pi * FPB);
+        const Rsrc rB = rsrc(AW + (long)pi * FPB);
+
+        // ---- the global loads of plane i: Hbar', the E and H auxiliary bars of this row, the
+        // owned cell's design 1/eps and pole state
+        T hb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hb[c] = bld<T>(rA, oR, (3 + c) * PB);
+        T pzE[2] = {T(0), T(0)}, phz[2] = {T(0), T(0)};
+        if (zany && rv) {  // lanes outside the layers: b = 1 and a zero auxiliary
+            const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+            pzE[0] = bld<T>(rpz, oZ, er * SB2 + so);
+            pzE[1] = bld<T>(rpz, oZ, (er + 1) * SB2 + so);
+            if (owned) {
+                phz[0] = bld<T>(rpz, oZ, hr * SB2 + so);
+                phz[1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
+            }
+        }
+        T pey[2] = {T(0), T(0)}, phy[2] = {T(0), T(0)};
+        if (ypml) {
+            const uint32_t so = (uint32_t)((pi * Ly + sy) * nz) * TS;
+            pey[0] = bld<T>(rpy, oK, er * SB1 + so);
+            pey[1] = bld<T>(rpy, oK, (er + 1) * SB1 + so);
+            phy[0] = bld<T>(rpy, oK, hr * SB1 + so);
+            phy[1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
+        }
+        T pex[2] = {T(0), T(0)}, phx[2] = {T(0), T(0)};
+        if (rv && xpml) {
+            pex[0] = bld<T>(rpx, oR, er * SB0 + sxo);
+            pex[1] = bld<T>(rpx, oR, (er + 1) * SB0 + sxo);
+            if (owned) {
+                phx[0] = bld<T>(rpx, oR, hr * SB0 + sxo);
+                phx[1] = bld<T>(rpx, oR, (hr + 1) * SB0 + sxo);
+            }
+        }
+        const bool wr = own_plane && owned;
+        int mm[3] = {0, 0, 0}, npl[3] = {0, 0, 0};
+        bool des[3] = {false, false, false};
+        T dv[3];
+        bool desany = false;
+        uint32_t sact = 0u;  // pole slots some lane of the wavefront uses (uniform)
+        T qv[NP > 0 ? NP : 1][3], pv[NP > 0 ? NP : 1][3];
+        if (wave_any(wr) && !uniform) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {  // lanes of other rows: no poles, no design voxel
+                const int m = (wc >> (8 * c)) & 0xff;
+                des[c] = wr && m == kDesignMat;
+                mm[c] = (wr && !des[c]) ? m : 0;
+                npl[c] = (wr && !des[c]) ? slut[mm[c]].np : 0;
+            }
+            desany = wave_any(des[0] || des[1] || des[2]);
+            if (desany) {
+                const uint32_t dq = (uint32_t)(((pi - a.dlo[0]) * dey + (j - a.dlo[1])) * dez + (k - a.dlo[2])) * TS;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dv[c] = bld<T>(rd, des[c] ? dq : kOOB, 0);
+            }
+            if (NP > 0) {
+                const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
+                const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
+#pragma unroll
+                for (int s = 0; s < NP; ++s) {
+                    if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+                    sact |= 1u << s;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const bool vq = s < npl[c];
+                        const bool vp = vq && ((slut[mm[c]].lorentz_mask >> s) & 1);
+                        const uint32_t so = (s * 3 + c) * PB;
+                        pv[s][c] = bld<T>(rP, vp ? oR : kOOB, so);
+                        qv[s][c] = bld<T>(rQ, vq ? oR : kOOB, so);
+                    }
+                }
+            }
+        }
+
+        // ---- phase 1: E-terms of this row, Hbar_total ---------------------------------------
+        T t_ex_y = cdt * uc[0], t_ex_z = -cdt * uc[0], t_ey_z = cdt * uc[1], t_ey_x = -cdt * uc[1],
+          t_ez_x = cdt * uc[2], t_ez_y = -cdt * uc[2];
+        if (ypml) {
+            t_ex_y = bEy * t_ex_y + (bEy - T(1)) * pey[0];
+            t_ez_y = bEy * t_ez_y + (bEy - T(1)) * pey[1];
+        }
+        if (rv && zany) {
+            t_ex_z = bEz * t_ex_z + (bEz - T(1)) * pzE[0];
+            t_ey_z = bEz * t_ey_z + (bEz - T(1)) * pzE[1];
+        }
+        if (rv && xpml) {
+            t_ey_x = bEx * t_ey_x + (bEx - T(1)) * pex[0];
+            t_ez_x = bEx * t_ez_x + (bEx - T(1)) * pex[1];
+        }
+        // x-terms of plane i + 1: (Ey -> Hz), (Ez -> Hy)
+        T xn0 = -cdt * un[1], xn1 = cdt * un[2];
+        if (rv && xpmln) {
+            xn0 = bExn * xn0 + (bExn - T(1)) * bld<T>(rpx, oR, er * SB0 + sxn);
+            xn1 = bExn * xn1 + (bExn - T(1)) * bld<T>(rpx, oR, (er + 1) * SB0 + sxn);
+        }
+        xn0 = (rv && pn >= 0) ? xn0 : T(0);
+        xn1 = (rv && pn >= 0) ? xn1 : T(0);
+        if (act) {
+            sE[0 * B + tid] = t_ex_z;
+            sE[1 * B + tid] = t_ey_z;
+            sE[2 * B + tid] = t_ex_y;
+            sE[3 * B + tid] = t_ez_y;
+        }
+        if (top) {  // y-terms of the row above the tile
+            T ay = cdt * tc[0], az = -cdt * tc[1];
+            if (ypmlt) {
+                const uint32_t so = (uint32_t)((pi * Ly + syt) * nz) * TS;
+                ay = bEyt * ay + (bEyt - T(1)) * bld<T>(rpy, oK, er * SB1 + so);
+                az = bEyt * az + (bEyt - T(1)) * bld<T>(rpy, oK, (er + 1) * SB1 + so);
+            }
+            sEt[k] = ay;
+            sEt[nz + k] = az;
+        }
+        lds_barrier();
+        T tot[3];
+        {
+            const T exz_k = kp1 >= 0 ? sE[0 * B + r * nz + kp1] : T(0);
+            const T eyz_k = kp1 >= 0 ? sE[1 * B + r * nz + kp1] : T(0);
+            const T exy_up = !act ? T(0) : r + 1 < RT ? sE[2 * B + tid + nz] : sEt[k];
+            const T ezy_up = !act ? T(0) : r + 1 < RT ? sE[3 * B + tid + nz] : sEt[nz + k];
+            // Hx: (Ey, z) and (Ez, y); Hy: (Ex, z) and (Ez, x); Hz: (Ex, y) and (Ey, x)
+            tot[0] = rv ? hb[0] + (t_ey_z - eyz_k) + (t_ez_y - ezy_up) : T(0);
+            tot[1] = rv ? hb[1] + (t_ex_z - exz_k) + (t_ez_x - xn1) : T(0);
+            tot[2] = rv ? hb[2] + (t_ex_y - exy_up) + (t_ey_x - xn0) : T(0);
+        }
+        // owned row: Hbar at the old time and the new H auxiliary bars
+        if (wr) {
+            bst(rB, oR, 3 * PB, tot[0]);
+            bst(rB, oR, 4 * PB, tot[1]);
+            bst(rB, oR, 5 * PB, tot[2]);
+            if (ypml) {  // (Hx, y, +, slot 0), (Hz, y, -, slot 1)
+                const uint32_t so = (uint32_t)((pi * Ly + sy) * nz) * TS;
+                bst(rpy, oK, hw * SB1 + so, bHy * (-cdt * tot[0] + phy[0]));
+                bst(rpy, oK, (hw + 1) * SB1 + so, bHy * (cdt * tot[2] + phy[1]));
+            }
+            if (zany) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
+                const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[0] + phz[0]));
+                bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1] + phz[1]));
+            }
+            if (xpml) {  // (Hy, x, -, slot 0), (Hz, x, +, slot 1)
+                bst(rpx, oR, hw * SB0 + sxo, bHx * (cdt * tot[1] + phx[0]));
+                bst(rpx, oR, (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[2] + phx[1]));
+            }
+        }
+
+        // ---- phase 2: H-terms of this row, Ebar of the owned rows --------------------------
+        T d_hx_y = -cdt * tot[0], d_hx_z = cdt * tot[0], d_hy_z = -cdt * tot[1], d_hy_x = cdt * tot[1],
+          d_hz_x = -cdt * tot[2], d_hz_y = cdt * tot[2];
+        if (ypml) {
+            d_hx_y = bHy * d_hx_y + (bHy - T(1)) * phy[0];
+            d_hz_y = bHy * d_hz_y + (bHy - T(1)) * phy[1];
+        }
+        if (owned && zany) {
+            d_hx_z = bHz * d_hx_z + (bHz - T(1)) * phz[0];
+            d_hy_z = bHz * d_hy_z + (bHz - T(1)) * phz[1];
+        }
+        if (owned && xpml) {
+            d_hy_x = bHx * d_hy_x + (bHx - T(1)) * phx[0];
+            d_hz_x = bHx * d_hz_x + (bHx - T(1)) * phx[1];
+        }
+        d_hx_y = rv ? d_hx_y : T(0);
+        d_hz_y = rv ? d_hz_y : T(0);
+        if (act) {
+            sD[0 * B + tid] = d_hy_z;
+            sD[1 * B + tid] = d_hx_z;
+            sD[2 * B + tid] = d_hx_y;
+            sD[3 * B + tid] = d_hz_y;
+        }
+        lds_barrier();
+        if (wr) {
+            const uint32_t o = oR;
+            const T hyz_k = km1 >= 0 ? sD[0 * B + r * nz + km1] : T(0);
+            const T hxz_k = km1 >= 0 ? sD[1 * B + r * nz + km1] : T(0);
+            const T hxy_dn = sD[2 * B + tid - nz];  // y-terms of the row below (r >= 1)
+            const T hzy_dn = sD[3 * B + tid - nz];
+            T inv[3], epsc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                inv[c] = slut[mm[c]].inv_eps;
+                epsc[c] = slut[mm[c]].eps_inf;
+            }
+            if (desany) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    inv[c] = des[c] ? dv[c] : inv[c];
+                    epsc[c] = des[c] ? T(1) / dv[c] : epsc[c];
+                }
+            }
+            // Ebar' = eps Ubar' plus the differences
+            // Ex: (Hy, z) and (Hz, y); Ey: (Hx, z) and (Hz, x); Ez: (Hx, y) and (Hy, x)
+            T out[3];
+            out[0] = epsc[0] * uc[0] + (hyz_k - d_hy_z) + (hzy_dn - d_hz_y);
+            out[1] = epsc[1] * uc[1] + (hxz_k - d_hx_z) + (dbx_prev[0] - d_hz_x);
+            out[2] = epsc[2] * uc[2] + (hxy_dn - d_hx_y) + (dbx_prev[1] - d_hy_x);
+            if (NP > 0 && !uniform) {
+                const Rsrc rQ = rsrc((const char*)g.Aq + (long)pi * QPB);
+                const Rsrc rP = rsrc((const char*)g.Ap + (long)pi * QPB);
+#pragma unroll
+                for (int s = 0; s < NP; ++s) {
+                    if (!((sact >> s) & 1u)) continue;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const MatLut<T>& L = slut[mm[c]];
+                        const bool vq = s < npl[c];
+                        const bool vp = vq && ((L.lorentz_mask >> s) & 1);
+                        const uint32_t so = (s * 3 + c) * PB;
+                        const T pb = pv[s][c];
+                        const T qb = vq ? qv[s][c] + pb - uc[c] : T(0);
+                        bst(rQ, vq ? o : kOOB, so, L.alpha[s] * qb);
+                        bst(rP, vp ? o : kOOB, so, pb + L.beta[s] * qb);
+                        out[c] += L.c3[s] * qb;
+                    }
+                }
+            }
+            // new E auxiliary bars: E-terms of this row with their signs
+            if (ypml) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
+                const uint32_t so = (uint32_t)((pi * Ly + sy) * nz) * TS;
+                bst(rpy, oK, ew * SB1 + so, bEy * (cdt * uc[0] + pey[0]));
+                bst(rpy, oK, (ew + 1) * SB1 + so, bEy * (-cdt * uc[2] + pey[1]));
+            }
+            if (zany) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
+                const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * uc[0] + pzE[0]));
+                bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * uc[1] + pzE[1]));
+            }
+            if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
+                bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * uc[1] + pex[0]));
+                bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * uc[2] + pex[1]));
+            }
+            // the state is Ubar = Ebar / eps
+            bst(rB, o, 0, inv[0] * out[0]);
+            bst(rB, o, PB, inv[1] * out[1]);
+            bst(rB, o, 2 * PB, inv[2] * out[2]);
+        }
+        dbx_prev[0] = owned ? d_hz_x : T(0);
+        dbx_prev[1] = owned ? d_hy_x : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) uc[c] = un[c];
+        tc[0] = tn[0];
+        tc[1] = tn[1];
+        wc = wn;
+    }
+}
+
+// Fused reverse step over one engine, row-of-threads form (tile t after the XCD-aware remap).
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_rows(const AdjArgs<T> g) {
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
+    adj_rows<T, NP>(g, t);
+}
+
 // Fused reverse step over one engine: workgroup w handles tile w after the XCD-aware remap.
 // With the loads of a plane issued together, the FP32 reverse step needs 95 registers with three
 // pole slots (5 wavefronts per SIMD) and 102 with four (4 wavefronts), without spilling.

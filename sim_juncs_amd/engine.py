@@ -300,6 +300,12 @@ class Engine:
                                            C.byref(J), _dp(grad), grad.size), "sjf_adjoint_run")
         return J.value, grad
 
+    def set_adjoint_tile(self, form: str, rows: int = 1, xchunk: int = 0):
+        """Tile of the fused reverse step: "register_rows" (rows = 1) or "thread_rows" (owned
+        rows per workgroup, (rows + 1) * nz <= 1024); xchunk 0 picks the default."""
+        code = {v: k for k, v in TILE_FORMS.items()}[form]
+        self._chk(self.lib.sjf_set_adjoint_tile(self.h, code, int(rows), int(xchunk)), "sjf_set_adjoint_tile")
+
     def adjoint_kernel_info(self):
         """(mean ms, threads, workgroups) of the fused reverse-step kernel of the last adjoint run."""
         ms, th, gr = C.c_double(), C.c_int(), C.c_int()

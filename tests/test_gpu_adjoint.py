@@ -95,6 +95,72 @@ def test_adjoint_gradient_matches_oracle_finite_differences(kind):
         assert abs(g[v] - fd) <= 1e-6 * np.abs(g).max() + 1e-9 * abs(fd), (kind, v, g[v], fd)
 
 
+# kind, owned rows of the row-of-threads reverse tile, x-chunk (0: default): rows that split
+# the y extent unevenly (12 = 5 + 5 + 2, 11 + 1), wrap periodic y rows into the last tile,
+# put two or three thread rows into one wavefront (nz = 20) or a z line across two (nz = 70)
+ROW_TILES = [("vacuum", 3, 0), ("pml", 5, 4), ("poles", 3, 0), ("poles", 11, 5), ("periodic", 5, 3),
+             ("periodic", 11, 0), ("poles_xchunk4", 2, 4), ("poles_long_z", 6, 3)]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("kind,rows,xc", ROW_TILES)
+def test_adjoint_row_of_threads_tile_matches_register_tile(kind, rows, xc):
+    """The row-of-threads reverse step (y and z neighbours through LDS) performs the arithmetic
+    of the one-row-per-workgroup tile: the same J and the same gradient to rounding, in FP64
+    and FP32, and the launch shape it reports."""
+    for precision, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        scn, gap, eps = design_case(kind, precision=precision)
+        print(f"[case] adjoint rows {kind} rows={rows} xc={xc} {precision}", flush=True)
+        out = []
+        for form, r in (("register_rows", 1), ("thread_rows", rows)):
+            eng = scn.make(Engine)
+            eng.set_adjoint_tile(form, r, xc)
+            out.append(eng.adjoint_run(scn.steps, 37, gap, eps.size))
+            _, threads, groups = eng.adjoint_kernel_info()
+            eng.close()
+        nx, ny, nz = scn.shape
+        nxc = -(-nx // xc) if xc else None
+        assert threads == -(-((rows + 1) * nz) // 64) * 64
+        if nxc:
+            assert groups == -(-ny // rows) * nxc
+        (J0, g0), (J1, g1) = out
+        assert J1 == J0  # the forward pass is the same
+        assert rel(g1, g0) < tol, (kind, rows, precision, rel(g1, g0))
+
+
+def test_adjoint_row_of_threads_tile_matches_oracle_finite_differences():
+    scn, gap, eps = design_case("poles")
+    eng = scn.make(Engine)
+    eng.set_adjoint_tile("thread_rows", 3, 4)
+    J, g = eng.adjoint_run(scn.steps, 37, gap, eps.size)
+    eng.close()
+    rng = np.random.default_rng(2)
+    flat = eps.reshape(-1)
+    for v in rng.choice(flat.size, size=6, replace=False):
+        h = 1e-4 * flat[v]
+        ep, em = flat.copy(), flat.copy()
+        ep[v] += h
+        em[v] -= h
+        fd = (oracle_J(scn, gap, ep.reshape(eps.shape)) - oracle_J(scn, gap, em.reshape(eps.shape))) / (2 * h)
+        assert abs(g[v] - fd) <= 1e-6 * np.abs(g).max() + 1e-9 * abs(fd), (v, g[v], fd)
+
+
+def test_adjoint_tile_arguments_are_checked():
+    scn, gap, eps = design_case("poles_long_z", precision="fp64", steps=10)
+    eng = scn.make(Engine)
+    with pytest.raises(EngineError):
+        eng.set_adjoint_tile("register_rows", 2, 0)
+    with pytest.raises(EngineError):
+        eng.set_adjoint_tile("thread_rows", 14, 0)  # 15 x 70 threads
+    with pytest.raises(EngineError):
+        eng.set_adjoint_tile("thread_rows", 7, 0)  # FP64 over 512 threads
+    eng.set_adjoint_tile("thread_rows", 6, 0)
+    eng.close()
+
+
 def test_adjoint_independent_of_checkpoint_interval_and_leaves_final_state():
     """Checkpoint intervals 1, 13, 90 and the record mode (0) give the same bits."""
     scn, gap, eps = design_case("poles", steps=90)
