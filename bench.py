@@ -79,12 +79,17 @@ def cpu_baseline(scns, steps, of_steps):
         secs += time.perf_counter() - t
         cells += s.cells
         o.close()
+    if steps < 200:
+        span = (f"{steps} of its {of_steps} steps each after 1 warm-up step ({secs:.1f} s; the sample "
+                f"is bounded to ~20 s of CPU work, fewer than the 200 steps a longer run would time, and "
+                f"the rate is extrapolated by steps)")
+    else:
+        span = (f"{steps} steps each after 1 warm-up step ({secs:.1f} s of CPU work; the workload runs "
+                f"{of_steps}, and the rate is per step)")
     return {"value": cells * steps / secs, "unit": "cell-updates/s", "cores": orc.num_threads(),
             "kind": "port",
             "sample": f"FP64 oracle (build's CPU stepper, not meep) on {len(scns)} full-size instance(s) "
-                      f"of the workload, {steps} of its {of_steps} steps each after 1 warm-up step "
-                      f"({secs:.1f} s, rate extrapolated by steps; the sample is bounded to ~20 s of CPU "
-                      f"work, not the 200 steps a longer run would time)"}
+                      f"of the workload, {span}"}
 
 
 def roofline(bytes_step, ms_kernel, kernel, threads, groups, workload):

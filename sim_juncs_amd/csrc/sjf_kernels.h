@@ -7,10 +7,12 @@
 // This is the E-direct form of meep's D/E split (src/disp.cpp:740 fields.step(): B, H, D,
 // polarisation, E): with D = eps_inf E + sum P, D^{n+1} - D^n = dt curl H, and
 // Q = P^{n+1} - P^n the textbook Lorentz/Drude recurrence becomes the (P, Q) pair above,
-// which needs no D array and, for Drude poles, no P array.  A Lorentz pole keeps P^n and
-// P^{n-1} (Q^n = P^n - P^{n-1}) in two slots whose roles alternate with the step parity, so
-// each step reads both and writes only P^{n+1} over P^{n-1}: 3 values moved per pole and
-// component instead of 4 for a stored (P, Q).  A Drude pole keeps Q^n alone (2 values).
+// which needs no D array and, for Drude poles, no P array.  In FP32 a Lorentz pole keeps P^n
+// and P^{n-1} (Q^n = P^n - P^{n-1}) in two slots whose roles alternate with the step parity,
+// so each step reads both and writes only P^{n+1} over P^{n-1}: 3 values moved per pole and
+// component instead of 4 for a stored (P, Q).  FP64 keeps (P, Q): its tile is held to 3
+// wavefronts per SIMD by its registers and runs faster without the role selects.  A Drude pole
+// keeps Q^n alone (2 values).
 //
 // Layout in HBM: each component is a dense nx*ny*nz array, z fastest.  E and H and the H
 // auxiliaries are double-buffered (read buffer 0, write buffer 1, swapped every step) because
@@ -34,6 +36,9 @@
 namespace sjf {
 
 constexpr int kMaxPoles = 4;
+// Lorentz poles stored as (P^n, P^{n-1}) with alternating roles (FP32) or as (P, Q) (FP64)
+template <typename T>
+constexpr bool kPolePair = sizeof(T) == 4;
 constexpr int kDesignMat = 255;
 constexpr int kMaxDftFreqs = 64;
 
@@ -186,8 +191,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
-    // Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in the P
-    // array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
+    // FP32 Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in
+    // the P array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
     const bool podd = (step & 1) != 0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tyi = t % a.nty, xci = t / a.nty;
@@ -525,13 +530,22 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                             const MatLut<T>& L = slut[mm[c]];
                             const bool vq = s < npl[c];
                             const bool vp = vq && ((L.lorentz_mask >> s) & 1);
-                            const T cur = podd ? Pv[r][c][s] : Qv[r][c][s];  // Lorentz: P^n
-                            const T old = podd ? Qv[r][c][s] : Pv[r][c][s];  // and P^{n-1}
-                            T qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
-                                      : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
-                            qn = vq ? qn : T(0);
-                            bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
-                            bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                            T qn;
+                            if constexpr (kPolePair<T>) {
+                                const T cur = podd ? Pv[r][c][s] : Qv[r][c][s];  // Lorentz: P^n
+                                const T old = podd ? Qv[r][c][s] : Pv[r][c][s];  // and P^{n-1}
+                                qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
+                                        : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
+                                qn = vq ? qn : T(0);
+                                bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
+                                bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                            } else {
+                                // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
+                                qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
+                                qn = vq ? qn : T(0);
+                                bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
+                                bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
+                            }
                             sq[c] += qn;
                         }
                     }
@@ -587,8 +601,8 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
     GLOBAL const T* coef = (GLOBAL const T*)a.coef;
     const uint32_t hr = (step & 1) ? 4 : 2;  // H auxiliary pair read this step
     const uint32_t hw = (step & 1) ? 2 : 4;  // and written
-    // Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in the P
-    // array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
+    // FP32 Lorentz pole slots hold P^n and P^{n-1}: P^n in the Q array on even steps and in
+    // the P array on odd ones, P^{n+1} overwrites P^{n-1}.  Drude slots hold Q^n in the Q array.
     const bool podd = (step & 1) != 0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int RT = a.rt, R = RT - 1;
@@ -903,13 +917,22 @@ __device__ __forceinline__ void step_rows(const StepArgs<T>& a, const int t) {
                         const MatLut<T>& L = slut[mm[c]];
                         const bool vq = s < npl[c];
                         const bool vp = vq && ((L.lorentz_mask >> s) & 1);
-                        const T cur = podd ? Pv[c][s] : Qv[c][s];  // Lorentz: P^n
-                        const T old = podd ? Qv[c][s] : Pv[c][s];  // and P^{n-1}
-                        T qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[c]
-                                  : L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c];
-                        qn = vq ? qn : T(0);
-                        bst(rP, (vp && !podd) ? wo : kOOB, (s * 3 + c) * PB, cur + qn);
-                        bst(rQ, (vq && !(vp && !podd)) ? wo : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                        T qn;
+                        if constexpr (kPolePair<T>) {
+                            const T cur = podd ? Pv[c][s] : Qv[c][s];  // Lorentz: P^n
+                            const T old = podd ? Qv[c][s] : Pv[c][s];  // and P^{n-1}
+                            qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[c]
+                                    : L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c];
+                            qn = vq ? qn : T(0);
+                            bst(rP, (vp && !podd) ? wo : kOOB, (s * 3 + c) * PB, cur + qn);
+                            bst(rQ, (vq && !(vp && !podd)) ? wo : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                        } else {
+                            // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
+                            qn = L.alpha[s] * Qv[c][s] + L.c3[s] * ec[c] + L.beta[s] * Pv[c][s];
+                            qn = vq ? qn : T(0);
+                            bst(rP, vp ? wo : kOOB, (s * 3 + c) * PB, Pv[c][s] + qn);
+                            bst(rQ, vq ? wo : kOOB, (s * 3 + c) * PB, qn);
+                        }
                         sq[c] += qn;
                     }
                 }
