@@ -30,6 +30,23 @@
  *   (i dx, (j+1/2)dx, k dx), Ez at (i dx, j dx, (k+1/2)dx), Hx at (i, j+1/2, k+1/2),
  *   Hy at (i+1/2, j, k+1/2), Hz at (i+1/2, j+1/2, k).  Fields outside the grid are zero
  *   (metallic outer edge) unless the axis is periodic.
+ *
+ * What pins each part (tests/test_oracle_physics.py unless named otherwise):
+ *   update_H / update_D curls   Yee plane-wave dispersion, discrete energy invariant, Courant
+ *                               blow-up
+ *   poles (update_PE)           Drude plasma and Lorentz longitudinal oscillations, static limit
+ *   CPML                        reflection against a doubled domain
+ *   orc_set_pml_dispersion      CW wave at f_ref meets no interface at the folded slabs
+ *   orc_set_eps_maps            plane-wave dispersion with a different eps per component
+ *   orc_set_eps_box             equals the same values given as maps
+ *   gauss_dipole, add_sources   window and phase known answers (reference src/main_test.cpp),
+ *                               current-sheet Green's function
+ *   cw_dipole                   closed form (logistic envelope, steady state, float gate), sampled
+ *                               current, CW sheet Green's function
+ *   add_h_sources               magnetic sheet: dual of the electric sheet's field
+ *   sample_monitors             trilinear interpolation exact on linear fields
+ *   accumulate_dft              equals the sum over the sampled series
+ *   flux (engine only)          plane-wave invariants (tests/test_gpu_flux.py)
  */
 #include <math.h>
 #include <stdint.h>

@@ -4,7 +4,9 @@ Each test checks a property the mathematics fixes, chosen so that a plausible mi
 oracle (wrong curl sign or stencil offset, wrong dt/dx scaling, missing pole term, wrong source
 normalisation or time alignment, wrong interpolation offsets) fails it:
 
-* Yee plane-wave dispersion in a periodic box: sin^2(w dt/2)/dt^2 = sum sin^2(k dx/2)/dx^2.
+* Yee plane-wave dispersion in a periodic box: sin^2(w dt/2)/dt^2 = sum sin^2(k dx/2)/dx^2,
+  and with per-component eps maps the same over eps_c; a design box equals the same values
+  given as maps.
 * Discrete energy W = sum eps E^n.E^n + H^{n+1/2}.H^{n-1/2} is invariant in a closed cavity.
 * Drude plasma oscillation and Lorentz longitudinal oscillation of a uniform field (curl-free):
   w^2 + i w gamma' = w_p^2 and w^2 + i w gamma' = w0^2 (1 + sigma/eps_inf), and the static limit
@@ -61,6 +63,66 @@ def test_yee_plane_wave_dispersion(comp, axis_k, m):
     # E^n = e0 (cos n theta - tan(theta/2) sin n theta) for E^0 = e0, H^{-1/2} = 0
     want = e0 * (np.cos(steps * theta) - math.tan(theta / 2) * np.sin(steps * theta))
     assert np.max(np.abs(np.array(vals) - want)) < 1e-11
+
+
+@pytest.mark.parametrize("comp,axis_k,m,eps", [
+    ("Ex", (0, 0, 1), 3, (2.25, 1.0, 1.0)), ("Ey", (1, 0, 0), 2, (1.0, 3.0, 1.0)),
+    ("Ez", (0, 1, 0), 5, (1.0, 1.0, 1.7)), ("Ex", (0, 1, 1), 1, (4.0, 2.0, 1.5)),
+])
+def test_eps_maps_set_each_components_permittivity(comp, axis_k, m, eps):
+    """Per-cell eps_inf maps (one array per E component): a plane wave polarised along c in a
+    medium whose map gives c the value eps_c oscillates at the Yee frequency of lambda / eps_c,
+    sin^2(w dt/2)/dt^2 = sum sin^2(k dx/2)/dx^2 / eps_c.  The other components' values differ,
+    so a map applied to the wrong component changes the frequency."""
+    n = (12, 10, 14)
+    dx, S = 0.1, 0.45
+    dt = S * dx
+    o = Oracle(n, dx, courant=S, periodic=(1, 1, 1))
+    o.set_eps_maps(np.stack([np.full(n, e) for e in eps]))
+    kvec = [2 * math.pi * m * axis_k[d] / (n[d] * dx) for d in range(3)]
+    c = ["Ex", "Ey", "Ez"].index(comp)
+    assert axis_k[c] == 0
+    off = scenario.COMP_OFF[c]
+    ii, jj, kk = np.meshgrid(*[np.arange(n[d]) + off[d] for d in range(3)], indexing="ij")
+    phase = kvec[0] * ii * dx + kvec[1] * jj * dx + kvec[2] * kk * dx
+    o.write_field(comp, np.cos(phase))
+    lam = sum(4 / dx ** 2 * math.sin(kvec[d] * dx / 2) ** 2 for d in range(3)) / eps[c]
+    theta = 2 * math.asin(math.sqrt(lam) * dt / 2)
+    probe = (3, 4, 5)
+    e0 = math.cos(phase[probe])
+    vals = []
+    for _ in range(60):
+        o.step(1)
+        vals.append(o.read_field(comp)[probe])
+    steps = np.arange(1, 61)
+    want = e0 * (np.cos(steps * theta) - math.tan(theta / 2) * np.sin(steps * theta))
+    assert np.max(np.abs(np.array(vals) - want)) < 1e-11
+
+
+def test_design_box_equals_the_same_permittivity_given_as_maps():
+    """A design box (one eps per voxel, z fastest, for all three E components, no poles) and eps
+    maps holding the same values in the box and 1 elsewhere give the same fields: the box's
+    voxel order and extent are those of the maps' cells."""
+    n = (11, 9, 13)
+    lo, hi = (2, 3, 4), (9, 7, 10)
+    rng = np.random.default_rng(7)
+    box = rng.uniform(1.2, 4.0, size=tuple(h - l for l, h in zip(lo, hi)))
+    runs = []
+    for mode in ("box", "maps"):
+        o = Oracle(n, 0.02, courant=0.5)
+        o.set_pml([2] * 6)
+        if mode == "box":
+            o.set_design_eps(lo, hi, box)
+        else:
+            e = np.ones(n)
+            e[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = box
+            o.set_eps_maps(np.stack([e, e, e]))
+        o.add_source("Ey", [0, 0, 0.1, 0.22, 0.18, 0.1], **scenario.gaussian_pulse(0.8, 1.5, 0.4, start_fs=0.0))
+        o.step(80)
+        runs.append([o.read_field(c) for c in range(6)])
+    assert np.abs(runs[0][1]).max() > 0
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
 
 
 def test_energy_invariant_closed_cavity():
