@@ -1235,8 +1235,7 @@ struct DftDev {
 
 // accumulation of one (component, cell, frequency) of region ri
 template <typename T>
-__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int ri, int f, long g, const double* ph,
-                                         int blk_reg);
+__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int ri, int f, long g);
 
 // One thread per (component, cell, frequency): blockIdx.y is the frequency index, and the
 // threads of a region run over its components' cells one component after another.  The phase
@@ -1246,10 +1245,7 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int
 template <typename T>
 __device__ __forceinline__ void dft_block(const DftDev<T>* __restrict__ regs, const int nreg, const long total,
                                           const int bx, const int f) {
-    __shared__ double ph[4];  // cos, sin at the E and H time levels for this block's region
-    __shared__ int blk_reg;
-    const long g0 = (long)bx * blockDim.x;
-    const long g = g0 + threadIdx.x;
+    const long g = (long)bx * blockDim.x + threadIdx.x;
     auto region_of = [&](long x) {
         int lo = 0, hi = nreg - 1;
         while (lo < hi) {  // region with first <= x
@@ -1259,19 +1255,7 @@ __device__ __forceinline__ void dft_block(const DftDev<T>* __restrict__ regs, co
         }
         return lo;
     };
-    if (threadIdx.x == 0) {
-        const int r0 = region_of(g0);
-        blk_reg = r0;
-        const DftDev<T>& a = regs[r0];
-        if (f < a.nf) {
-            const long step = *a.d_step;
-            const double w = 2 * M_PI * a.freq[f];
-            sincos(w * ((step + 1) * a.dt), &ph[1], &ph[0]);  // E after the step: t = (n+1) dt
-            sincos(w * ((step + 0.5) * a.dt), &ph[3], &ph[2]);  // H: t = (n + 1/2) dt
-        }
-    }
-    __syncthreads();
-    if (g < total) dft_cell(regs, region_of(g), f, g, ph, blk_reg);
+    if (g < total) dft_cell(regs, region_of(g), f, g);
 }
 
 template <typename T>
@@ -1297,8 +1281,7 @@ __global__ void __launch_bounds__(256) k_source_dft(const SrcTables<T> t, int ns
 }
 
 template <typename T>
-__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, const int ri, const int f, const long g,
-                                         const double* ph, const int blk_reg) {
+__device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, const int ri, const int f, const long g) {
     const DftDev<T>& a = regs[ri];
     if (f >= a.nf) return;
     const long local = g - a.first;
@@ -1311,17 +1294,12 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, con
     const int jj = (int)(rest % a.ext[1]);
     const int ii = (int)(rest / a.ext[1]);
     const long q = (long)(a.lo[0] + ii) * a.nyz + (long)(a.lo[1] + jj) * a.nz + (a.lo[2] + kk);
+    // phases of this thread's frequency at the E and H time levels, t = (n+1) dt and (n+1/2) dt
+    // (each thread forms its own: no serial set-up ahead of the loads)
     double cE, sE, cH, sH;
-    if (ri == blk_reg) {
-        cE = ph[0];
-        sE = ph[1];
-        cH = ph[2];
-        sH = ph[3];
-    } else {
-        const double w = 2 * M_PI * a.freq[f];
-        sincos(w * ((step + 1) * a.dt), &sE, &cE);
-        sincos(w * ((step + 0.5) * a.dt), &sH, &cH);
-    }
+    const double w = 2 * M_PI * a.freq[f];
+    sincos(w * ((step + 1) * a.dt), &sE, &cE);
+    sincos(w * ((step + 0.5) * a.dt), &sH, &cH);
     const int c = a.comps[s];
     const double v = (double)F[field_index(q, c, a.nyz)] * a.dt;
     const long o = ((long)s * a.nf + f) * a.ncell + cell;
