@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--checkpoint-every", type=int, default=0,
                    help="adjoint workload: checkpoint every K steps and recompute (0: record the design "
                         "fields of every step, no recomputation; both give the same bits)")
+    p.add_argument("--zseg", type=int, default=None,
+                   help="register-rows tile: z cells per workgroup pass (default: tuned cache or one pass)")
     p.add_argument("--adjoint-tile", default=None,
                    help="adjoint workloads: tile of the fused reverse step as form,rows,xchunk "
                         "(default: the engine's)")
@@ -117,8 +119,12 @@ def set_batch_tile(a, b):
         b.set_tile(form, int(xc))
 
 
-def step_kernel(info):
-    return "k_step" if info["form"] == "register_rows" else "k_step_rows"
+def step_kernel(info, nz=None):
+    """Name of the fused step kernel the launch shape runs (register rows with fewer threads
+    than the z line has cells step it in passes: k_step_z)."""
+    if info["form"] != "register_rows":
+        return "k_step_rows"
+    return "k_step_z" if nz is not None and info["threads"] < nz else "k_step"
 
 
 def set_tile(a, eng):
@@ -130,6 +136,8 @@ def set_tile(a, eng):
         eng.set_tile(form, int(rows), int(xc))
     else:
         autotune.apply_cached(eng)
+    if a.zseg is not None:
+        eng.set_zsegment(a.zseg)
 
 
 def single(a, scn, tag):
@@ -149,7 +157,8 @@ def single(a, scn, tag):
     log(f"timed {steps} steps in {wall:.3f}s")
     _, ms_kernel = eng.timed_steps(a.kernel_steps)
     info = eng.launch_info()
-    rl = roofline(eng.step_bytes(), ms_kernel, step_kernel(info), info["threads"], info["groups"], scn.name)
+    rl = roofline(eng.step_bytes(), ms_kernel, step_kernel(info, scn.shape[2]), info["threads"], info["groups"],
+                  scn.name)
     eng.close()
     cpu = None
     if not a.no_cpu:

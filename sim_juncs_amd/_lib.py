@@ -30,7 +30,7 @@ EXPORTS = [
     "sjf_set_pml", "sjf_add_source", "sjf_last_source_time", "sjf_dt", "sjf_time",
     "sjf_step_count", "sjf_add_monitors", "sjf_set_save_span", "sjf_read_monitors",
     "sjf_get_field", "sjf_add_dft", "sjf_read_dft", "sjf_step", "sjf_synchronize",
-    "sjf_timed_steps", "sjf_read_field", "sjf_read_field_box", "sjf_write_field", "sjf_step_bytes",
+    "sjf_timed_steps", "sjf_set_zsegment", "sjf_read_field", "sjf_read_field_box", "sjf_write_field", "sjf_step_bytes",
     "sjf_launch_info", "sjf_set_tile", "sjf_copy_probe", "sjf_set_graph",
     "sjf_batch_create", "sjf_batch_set_tile", "sjf_batch_step", "sjf_batch_synchronize", "sjf_batch_timed_steps",
     "sjf_batch_info", "sjf_batch_last_error", "sjf_batch_destroy", "sjf_adjoint_run",
@@ -131,6 +131,7 @@ def load(path: str | None = None) -> C.CDLL:
         "sjf_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, dp, C.c_size_t]),
         "sjf_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),
         "sjf_set_adjoint_tile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "sjf_set_zsegment": (C.c_int, [vp, C.c_int]),
         "sjf_batch_launch_info": (C.c_int, [vp, ip, ip]),
         "sjf_batch_adjoint_run": (C.c_int, [vp, C.c_long, C.c_int, ip, dp, C.POINTER(dp), C.POINTER(C.c_size_t)]),
         "sjf_batch_adjoint_kernel_info": (C.c_int, [vp, dp, ip, ip]),

@@ -57,6 +57,8 @@ def apply_cached(eng, tag="") -> bool:
     t = lookup(eng.shape, eng.precision, tag)
     if t and "form" in t:
         eng.set_tile(t["form"], t["rows"], t["xchunk"])
+        if "zseg" in t:
+            eng.set_zsegment(t["zseg"])
         return True
     return False
 

@@ -112,6 +112,7 @@ struct sjf_engine {
 
     int form = SJF_TILE_REGISTER_ROWS;  // fused-kernel form
     int ty = 1, xc = 16;                // y rows per thread (register form) or per workgroup
+    int zseg = 0;                       // register form: z cells per workgroup pass (0: default)
     bool tile_set = false;
     int graph_chunk = 32;  // steps per captured graph (0: no graphs)
 
@@ -251,11 +252,24 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
 // Row-of-threads form: (rows + 1) thread rows of nz threads per workgroup, 32-bit offsets.
 bool thread_rows_fit(const sjf_engine* e, int rows) { return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024; }
 
+// z cells per pass of a register-rows workgroup with `rows` rows per thread: the set value, else
+// 256-cell passes for one row per thread on a non-periodic z line that is a multiple of 256
+// cells above 256 or longer than 1024, else the whole line.  Measured on MI355X, 1024x768x512
+// FP32: one pass of 512 threads leaves two 8-wave workgroups per CU (4 waves per SIMD of the 5
+// the registers allow) and runs 8.07 ms per step; two passes of 256 threads (5 workgroups per
+// CU) run 7.31 ms.  On 512x512x256, 128-cell passes lose (1.370 against 1.275 ms).
+int zseg_of(const sjf_engine* e, int rows) {
+    const int nz = e->g.nz;
+    if (e->zseg > 0) return std::min(e->zseg, nz);
+    if (rows == 1 && !e->g.periodic[2] && nz > 256 && (nz % 256 == 0 || nz > 1024)) return 256;
+    return nz;
+}
+
 // FP64 tiles stop at 512 threads: at 1024 the register budget of 128 per thread spills
 int default_thread_rows(const sjf_engine* e) { return std::max(1, (e->prec == 8 ? 512 : 1024) / e->g.nz - 1); }
 
 int block_threads_for(const sjf_engine* e, int form, int ty) {
-    const int t = form == SJF_TILE_THREAD_ROWS ? (ty + 1) * e->g.nz : e->g.nz;
+    const int t = form == SJF_TILE_THREAD_ROWS ? (ty + 1) * e->g.nz : zseg_of(e, ty);
     return ((t + 63) / 64) * 64;
 }
 
@@ -281,6 +295,7 @@ sjf::StepArgs<T> step_args(const sjf_engine* e, int ty, int xc) {
     sjf::StepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.rt = ty + 1;
+    a.zseg = zseg_of(e, ty);
     a.nx = e->g.nx;
     a.ny = e->g.ny;
     a.nz = e->g.nz;
@@ -326,6 +341,7 @@ struct LaunchSet {
     int prec = 4, ty = 1, npv = 0, block = 64;
     int form = SJF_TILE_REGISTER_ROWS;
     int xc = 0;  // x-chunk of every instance of a batch (0: each engine's own)
+    bool zpass = false;  // register form with some z line stepped in passes (k_step_z)
     size_t lds = 0;
     std::vector<long> versions;
     bool built = false;
@@ -418,6 +434,7 @@ int build_set_t(LaunchSet* s) {
     int maxp = 0, block = 64;
     bool all_uniform = true;
     int t = 0;
+    size_t zh = 0;  // register form with z passes: values of the pass halo in LDS
     for (sjf_engine* e : s->eng) {
         // the engine's own set runs its tuned rows; a batch picks rows per engine for its form
         const int rows = (s == e->own) ? e->ty
@@ -429,7 +446,14 @@ int build_set_t(LaunchSet* s) {
         maxp = std::max(maxp, e->maxp);
         all_uniform = all_uniform && e->uniform;
         block = std::max(block, block_threads_for(e, s->form, rows));
+        if (s->form == SJF_TILE_REGISTER_ROWS && zseg_of(e, rows) < e->g.nz) {
+            if (rows != 1) return fail(e0, SJF_E_UNSUPPORTED, "z passes run one row per thread");
+            zh = std::max<size_t>(zh, 4 * (size_t)xc);
+        }
+        if (block_threads_for(e, s->form, rows) > 1024)
+            return fail(e0, SJF_E_UNSUPPORTED, "a z line longer than 1024 cells needs one row per thread (z passes)");
     }
+    s->zpass = zh > 0;
     first.push_back(t);
     s->ntiles = t;
     s->block = block;
@@ -437,7 +461,7 @@ int build_set_t(LaunchSet* s) {
     int nmat = 1;
     for (sjf_engine* e : s->eng) nmat = std::max(nmat, e->nmat);
     s->lds = (s->form == SJF_TILE_THREAD_ROWS ? sizeof(T) * 6 * block : sizeof(T) * 4 * s->ty * block) +
-             sizeof(sjf::MatLut<T>) * nmat;
+             sizeof(sjf::MatLut<T>) * nmat + sizeof(T) * zh;
     if (s->lds > 65536)
         return fail(e0, SJF_E_UNSUPPORTED, "tile needs more than 64 KiB of LDS; use fewer rows");
     int rc = upload_vec(e0, s->stream, &s->d_args, args);
@@ -611,6 +635,11 @@ int launch_step_np(LaunchSet* s) {
     const int n = (int)s->eng.size();
     if (s->form == SJF_TILE_THREAD_ROWS) {
         hipLaunchKernelGGL((sjf::k_step_rows<T, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+        SETCHK(s, hipGetLastError());
+        return SJF_OK;
+    }
+    if (s->zpass) {
+        hipLaunchKernelGGL((sjf::k_step_z<T, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
         SETCHK(s, hipGetLastError());
         return SJF_OK;
     }
@@ -1770,6 +1799,7 @@ extern "C" int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every,
                                double* grad, size_t cap) {
     if (!e || nsteps < 1 || checkpoint_every < 0 || !gap || !J || !grad) return SJF_E_ARG;
     if (!e->emap_h.empty()) return fail(e, SJF_E_UNSUPPORTED, "adjoint runs take the LUT's eps_inf (no eps maps)");
+    if (e->g.nz > 1024) return fail(e, SJF_E_UNSUPPORTED, "the reverse step spans the z line: nz <= 1024");
     if (e->batch) return fail(e, SJF_E_STATE, "engine is stepped by its batch");
     if (e->step != 0) return fail(e, SJF_E_STATE, "the adjoint run starts from step 0");
     if (e->deps_h.empty()) return fail(e, SJF_E_STATE, "no design region (sjf_set_design_eps)");
@@ -1928,8 +1958,8 @@ int sjf_create(const sjf_grid* g, sjf_engine** out) {
         g_create_error = "grid dimensions, dx and courant must be positive";
         return SJF_E_ARG;
     }
-    if (g->nz > 1024) {
-        g_create_error = "nz > 1024 is not supported (one workgroup spans the z line)";
+    if (g->nz > 1024 && g->periodic[2]) {
+        g_create_error = "a z line longer than 1024 cells is stepped in passes, which needs a non-periodic z axis";
         return SJF_E_UNSUPPORTED;
     }
     if (g->precision != SJF_FP32 && g->precision != SJF_FP64) {
@@ -2474,6 +2504,17 @@ int sjf_launch_info(const sjf_engine* e, int* form, int* ty, int* xchunk, int* t
     return SJF_OK;
 }
 
+int sjf_set_zsegment(sjf_engine* e, int cells) {
+    if (!e) return SJF_E_ARG;
+    if (cells != 0 && (cells < 64 || cells > 1024 || cells % 64 != 0))
+        return fail(e, SJF_E_ARG, "z segment must be 0 or a multiple of 64 from 64 to 1024");
+    if (cells > 0 && cells < e->g.nz && e->g.periodic[2])
+        return fail(e, SJF_E_ARG, "z passes need a non-periodic z axis");
+    e->zseg = cells;
+    e->version++;
+    return SJF_OK;
+}
+
 int sjf_set_tile(sjf_engine* e, int form, int ty, int xchunk) {
     if (!e) return SJF_E_ARG;
     if (form == SJF_TILE_REGISTER_ROWS) {
@@ -2655,6 +2696,7 @@ int sjf_batch_adjoint_run(sjf_batch* b, long nsteps, int checkpoint_every, const
         if (e->step != 0) why = "the adjoint run starts from step 0";
         else if (e->deps_h.empty()) why = "an engine has no design region (sjf_set_design_eps)";
         else if (e->g.nz != e0->g.nz) why = "engines of a batched adjoint run must share nz";
+        else if (e->g.nz > 1024) why = "the reverse step spans the z line: nz <= 1024";
         else if (np_variant_of(e->maxp, e->uniform) != np_variant_of(e0->maxp, e0->uniform))
             why = "engines of a batched adjoint run must share the pole-slot variant";
         else if (!grads[i]) why = "null gradient buffer";

@@ -63,6 +63,7 @@ struct StepArgs {
     int px, py, pz;
     int nty, xc;
     int rt;       // row-of-threads form: thread rows per workgroup (owned rows + 1 halo row)
+    int zseg;     // register form: z cells per pass of a workgroup (nz: one pass)
     long N;       // cells
     T cdt;        // dt / dx
     T* F[2];              // field sets [x][Ex Ey Ez Hx Hy Hz][y][z]; step n reads F[n&1], writes F[~n&1]
@@ -170,7 +171,7 @@ __host__ __device__ __forceinline__ long field_index(long q, int c, long nyz) { 
 // z PML layer, a pole slot or the design region carry the out-of-range offset instead of
 // branching; whole-wavefront conditions (row, plane, x/y PML, any lane in a z layer / using a
 // slot / in the design region) are uniform branches.
-template <typename T, int TY, int NP>
+template <typename T, int TY, int NP, bool ZP>
 __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     constexpr uint32_t TS = sizeof(T);
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -199,10 +200,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     const int j0 = tyi * TY;
     const int x0 = xci * a.xc;
     const int x1 = min(x0 + a.xc, nx);
-    const int k = threadIdx.x;
-    const bool kin = k < nz;
+    // The z line in passes of zseg cells (one pass when the workgroup spans the whole line).  The
+    // last lane of a pass keeps the H it forms in zh, where the first lane of the next pass reads
+    // its z neighbour; passes alternate between two halves of zh.
+    const int Z = ZP ? a.zseg : nz;  // ZP: compiled for z passes
+    const int npass = ZP ? (nz + Z - 1) / Z : 1;
+    T* zh = reinterpret_cast<T*>(slut + a.nmat);  // [2 pass parity][xc][TY][Hx, Hy]
+    const int kl = threadIdx.x;
+    for (int pass = 0; pass < npass; ++pass) {
+    const int k = pass * Z + kl;
+    const bool kin = kl < Z && k < nz;
     const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
     const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
+    // LDS lane of the z neighbour below: -1 none (edge ghost), -2 the previous pass's zh
+    const int kml = kl > 0 ? kl - 1 : (pass == 0 ? km1 : -2);
     const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
     const long FPB = 6L * PB;                             // bytes of one x plane of a field set
     const long QPB = (long)a.nqp * PB;                    // and of the polarisation
@@ -445,8 +456,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             T* buf = sh + (i & 1) * 2 * TY * B;
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
-                buf[r * B + k] = h[r + 1][0];
-                buf[(TY + r) * B + k] = h[r + 1][1];
+                buf[r * B + kl] = h[r + 1][0];
+                buf[(TY + r) * B + kl] = h[r + 1][1];
+            }
+            if (ZP && pass + 1 < npass && kl == Z - 1) {
+                T* zo = zh + ((pass & 1) * a.xc + (i - x0)) * 2 * TY;
+#pragma unroll
+                for (int r = 0; r < TY; ++r) {
+                    zo[2 * r] = h[r + 1][0];
+                    zo[2 * r + 1] = h[r + 1][1];
+                }
             }
             lds_barrier();
 
@@ -456,8 +475,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 if (!own_row[r]) continue;
                 const int j = gj[r + 1];
                 const uint32_t o = oR[r + 1];
-                const T hxkm = km1 >= 0 ? buf[r * B + km1] : T(0);
-                const T hykm = km1 >= 0 ? buf[(TY + r) * B + km1] : T(0);
+                T hxkm = T(0), hykm = T(0);
+                if (kml >= 0) {
+                    hxkm = buf[r * B + kml];
+                    hykm = buf[(TY + r) * B + kml];
+                } else if (ZP && kml == -2) {
+                    const T* zi = zh + (((pass - 1) & 1) * a.xc + (i - x0)) * 2 * TY;
+                    hxkm = zi[2 * r];
+                    hykm = zi[2 * r + 1];
+                }
                 T dHz_y = h[r + 1][2] - h[r][2];
                 T dHx_y = h[r + 1][0] - h[r][0];
                 T dHy_z = h[r + 1][1] - hykm;
@@ -570,6 +596,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
             }
         }
     }
+    }  // pass
 }
 
 // Fused step, row-of-threads form.  A workgroup holds a.rt rows of nz threads (one cell of a
@@ -987,7 +1014,16 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_
 k_step(const StepArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
     int t;
     const int b = tile_instance(tile_first, ninst, t);
-    step_tile<T, TY, NP>(args[b], t);
+    step_tile<T, TY, NP, false>(args[b], t);
+}
+
+// The same with the z line in passes of StepArgs::zseg cells (one row per thread)
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_waves<T, 1, NP>())))
+k_step_z(const StepArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
+    int t;
+    const int b = tile_instance(tile_first, ninst, t);
+    step_tile<T, 1, NP, true>(args[b], t);
 }
 
 // Fused step over all instances of a launch set (row-of-threads form).

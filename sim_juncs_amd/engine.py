@@ -300,6 +300,10 @@ class Engine:
                                            C.byref(J), _dp(grad), grad.size), "sjf_adjoint_run")
         return J.value, grad
 
+    def set_zsegment(self, cells: int):
+        """Register-rows tile: z cells per workgroup pass (a multiple of 64; 0: one pass)."""
+        self._chk(self.lib.sjf_set_zsegment(self.h, int(cells)), "sjf_set_zsegment")
+
     def set_adjoint_tile(self, form: str, rows: int = 1, xchunk: int = 0):
         """Tile of the fused reverse step: "register_rows" (rows = 1) or "thread_rows" (owned
         rows per workgroup, (rows + 1) * nz <= 1024); xchunk 0 picks the default."""

@@ -75,8 +75,10 @@ def test_create_rejects_bad_arguments_before_touching_a_device():
     g = _lib.Grid()
     g.nx, g.ny, g.nz = 8, 8, 2000
     g.dx, g.courant, g.precision = 0.1, 0.5, 4
+    g.periodic[2] = 1
     h = C.c_void_p()
-    assert lib.sjf_create(C.byref(g), C.byref(h)) == -4  # nz > 1024 unsupported
+    assert lib.sjf_create(C.byref(g), C.byref(h)) == -4  # nz > 1024 needs z passes: not periodic
+    g.periodic[2] = 0
     g.nz, g.precision = 8, 3
     assert lib.sjf_create(C.byref(g), C.byref(h)) == -1
 

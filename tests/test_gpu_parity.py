@@ -380,3 +380,66 @@ def test_read_field_box_is_the_slice_of_the_whole_component(precision):
         with pytest.raises(EngineError):
             eng.read_field("Ey", lo, hi)
     eng.close()
+
+
+@pytest.mark.parametrize("precision,zseg,shape", [
+    ("fp64", 64, (12, 10, 150)),   # three passes, the last 22 cells
+    ("fp64", 128, (10, 9, 200)),
+    ("fp32", 256, (8, 8, 512)),
+    ("fp64", 0, (8, 7, 1100)),     # longer than a workgroup: default 256-cell passes
+])
+def test_z_passes_match_one_pass_and_oracle(precision, zseg, shape):
+    """A workgroup that covers its z line in passes (sjf_set_zsegment; the last lane of a pass
+    hands its H to the first lane of the next) steps exactly as one pass over the line, and
+    z lines longer than 1024 cells match the oracle."""
+    scn, _ = random_scenario(shape, seed=21, pml=[3] * 6, steps=40)
+    print(f"[case] z passes {shape} zseg={zseg} {precision}", flush=True)
+    eng = scn.make(Engine, precision=precision)
+    eng.set_tile("register_rows", 1, 4)
+    eng.set_zsegment(zseg)
+    eng.step(40)
+    info = eng.launch_info()
+    assert info["threads"] == (zseg or 256)
+    if shape[2] <= 1024:
+        one = scn.make(Engine, precision=precision)
+        one.set_tile("register_rows", 1, 4)
+        one.set_zsegment(-(-shape[2] // 64) * 64)  # one pass over the line
+        assert one.launch_info()["threads"] == -(-shape[2] // 64) * 64
+        one.step(40)
+        for c in range(6):
+            assert np.array_equal(eng.read_field(c), one.read_field(c)), c
+        one.close()
+    if precision == "fp64":
+        orc = scn.make(Oracle)
+        orc.step(40)
+        errs = field_errors(eng, orc)
+        print("  ", ["%.2e" % e for e in errs], flush=True)
+        assert max(errs) < FP64_TOL
+    eng.close()
+
+
+def test_z_pass_arguments_are_checked():
+    scn, _ = random_scenario((6, 5, 130), seed=3, pml=[2] * 6, steps=4)
+    eng = scn.make(Engine, precision="fp64")
+    for bad in (32, 100, 2048, -64):
+        with pytest.raises(EngineError):
+            eng.set_zsegment(bad)
+    eng.set_tile("register_rows", 2, 4)
+    eng.set_zsegment(64)
+    with pytest.raises(EngineError):  # passes run one row per thread
+        eng.step(1)
+    eng.close()
+    # the default: 256-cell passes for a 512-cell line with one row per thread, one pass with two
+    scn, _ = random_scenario((6, 5, 512), seed=3, pml=[2] * 6, steps=4)
+    eng = scn.make(Engine, precision="fp32")
+    eng.set_tile("register_rows", 1, 4)
+    assert eng.launch_info()["threads"] == 256
+    eng.set_tile("register_rows", 2, 4)
+    assert eng.launch_info()["threads"] == 512
+    eng.step(2)
+    eng.close()
+    scn, _ = random_scenario((6, 5, 130), seed=3, pml=None, periodic=(0, 0, 1), steps=4)
+    eng = scn.make(Engine, precision="fp64")
+    with pytest.raises(EngineError):  # a periodic z axis needs one pass
+        eng.set_zsegment(64)
+    eng.close()
