@@ -208,394 +208,397 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     T* zh = reinterpret_cast<T*>(slut + a.nmat);  // [2 pass parity][xc][TY][Hx, Hy]
     const int kl = threadIdx.x;
     for (int pass = 0; pass < npass; ++pass) {
-    const int k = pass * Z + kl;
-    const bool kin = kl < Z && k < nz;
-    const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
-    const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
-    // LDS lane of the z neighbour below: -1 none (edge ghost), -2 the previous pass's zh
-    const int kml = kl > 0 ? kl - 1 : (pass == 0 ? km1 : -2);
-    const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
-    const long FPB = 6L * PB;                             // bytes of one x plane of a field set
-    const long QPB = (long)a.nqp * PB;                    // and of the polarisation
-    const long PW = (long)ny * nz * 4;                    // and of the material words
-    const T cdt = a.cdt;
-    const bool uniform = a.uniform != 0;
-    const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
-    const uint32_t SB0 = (uint32_t)(a.S[0] * TS), SB1 = (uint32_t)(a.S[1] * TS), SB2 = (uint32_t)(a.S[2] * TS);
-    const Rsrc rpx = rsrc(a.psi[0]), rpy = rsrc(a.psi[1]), rpz = rsrc(a.psi[2]);
+        const int k = pass * Z + kl;
+        const bool kin = kl < Z && k < nz;
+        const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
+        const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
+        const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
+        const long FPB = 6L * PB;                             // bytes of one x plane of a field set
+        const long QPB = (long)a.nqp * PB;                    // and of the polarisation
+        const long PW = (long)ny * nz * 4;                    // and of the material words
+        const T cdt = a.cdt;
+        const bool uniform = a.uniform != 0;
+        const int Ly = a.plo[1] + a.phi[1], Lz = a.plo[2] + a.phi[2];
+        const uint32_t SB0 = (uint32_t)(a.S[0] * TS), SB1 = (uint32_t)(a.S[1] * TS), SB2 = (uint32_t)(a.S[2] * TS);
+        const Rsrc rpx = rsrc(a.psi[0]), rpy = rsrc(a.psi[1]), rpz = rsrc(a.psi[2]);
 
-    int gj[TY + 2];        // global row of local row r-1 (r = 0 is the halo row below the tile)
-    bool rv[TY + 2];       // row inside the grid (uniform)
-    uint32_t oR[TY + 2];   // byte offset of (row, k) within a component plane, or kOOB
-    uint32_t oW[TY + 2];   // the same for the 4-byte material words
-#pragma unroll
-    for (int r = 0; r < TY + 2; ++r) {
-        gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
-        rv[r] = gj[r] >= 0;
-        const bool ok = rv[r] && kin;
-        oR[r] = ok ? (uint32_t)(gj[r] * nz + k) * TS : kOOB;
-        oW[r] = ok ? (uint32_t)(gj[r] * nz + k) * 4u : kOOB;
-    }
-    uint32_t oK1[TY + 1];  // byte offset of (row, k + 1), or kOOB
-#pragma unroll
-    for (int r = 0; r < TY + 1; ++r)
-        oK1[r] = (rv[r] && kin && kp1 >= 0) ? (uint32_t)(gj[r] * nz + kp1) * TS : kOOB;
-    bool own_row[TY];
-#pragma unroll
-    for (int r = 0; r < TY; ++r) own_row[r] = (j0 + r) < ny;
-    bool ypml[TY + 1];
-    int sy[TY + 1];
-#pragma unroll
-    for (int r = 0; r < TY + 1; ++r) {
-        ypml[r] = rv[r] && in_pml(a, 1, gj[r], ny);
-        sy[r] = ypml[r] ? (gj[r] < a.plo[1] ? gj[r] : gj[r] - (ny - a.phi[1]) + a.plo[1]) : 0;
-    }
-
-    const bool zpml = kin && in_pml(a, 2, k, nz);
-    const bool zany = wave_any(zpml);  // some lane of this wavefront lies in a z PML layer
-    const uint32_t oK = kin ? (uint32_t)k * TS : kOOB;
-    const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : kOOB;
-    GLOBAL const T* cz = coef + a.coff[2];
-    const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[nz + k] : T(0);
-    const T bHz = zpml ? cz[2 * nz + k] : T(0), aHz = zpml ? cz[3 * nz + k] : T(0);
-
-    lds_barrier();
-
-    T ec[TY + 2][3];  // E^n on plane i, rows -1..TY
-    T en[TY + 2][3];  // E^n on plane i+1
-    T hp[TY][2];      // H^{n+1/2} (Hy, Hz) on plane i-1, rows 0..TY-1
-    uint32_t mw[TY];  // material words of plane i, owned rows
-
-    const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
-    {
-        const int p0 = wrap_or_ghost(istart, nx, a.px);
-        const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
-        const Rsrc r0 = rsrc(F0 + (long)(p0 >= 0 ? p0 : 0) * FPB);
-        const Rsrc r1 = rsrc(F0 + (long)(p1 >= 0 ? p1 : 0) * FPB);
+        int gj[TY + 2];        // global row of local row r-1 (r = 0 is the halo row below the tile)
+        bool rv[TY + 2];       // row inside the grid (uniform)
+        uint32_t oR[TY + 2];   // byte offset of (row, k) within a component plane, or kOOB
+        uint32_t oW[TY + 2];   // the same for the 4-byte material words
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
-            const uint32_t o0 = p0 >= 0 ? oR[r] : kOOB, o1 = p1 >= 0 ? oR[r] : kOOB;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ec[r][c] = bld<T>(r0, o0, c * PB);
-                en[r][c] = bld<T>(r1, o1, c * PB);
-            }
+            gj[r] = wrap_or_ghost(j0 + r - 1, ny, a.py);
+            rv[r] = gj[r] >= 0;
+            const bool ok = rv[r] && kin;
+            oR[r] = ok ? (uint32_t)(gj[r] * nz + k) * TS : kOOB;
+            oW[r] = ok ? (uint32_t)(gj[r] * nz + k) * 4u : kOOB;
         }
-        const bool mw0 = !uniform && istart >= x0;
-        const Rsrc rm = rsrc(matw + (long)(p0 >= 0 ? p0 : 0) * PW);
+        uint32_t oK1[TY + 1];  // byte offset of (row, k + 1), or kOOB
 #pragma unroll
-        for (int r = 0; r < TY; ++r) {
-            hp[r][0] = hp[r][1] = T(0);
-            mw[r] = 0u;
-            if (mw0 && own_row[r]) mw[r] = bld<uint32_t>(rm, oW[r + 1], 0);
+        for (int r = 0; r < TY + 1; ++r)
+            oK1[r] = (rv[r] && kin && kp1 >= 0) ? (uint32_t)(gj[r] * nz + kp1) * TS : kOOB;
+        bool own_row[TY];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) own_row[r] = (j0 + r) < ny;
+        bool ypml[TY + 1];
+        int sy[TY + 1];
+#pragma unroll
+        for (int r = 0; r < TY + 1; ++r) {
+            ypml[r] = rv[r] && in_pml(a, 1, gj[r], ny);
+            sy[r] = ypml[r] ? (gj[r] < a.plo[1] ? gj[r] : gj[r] - (ny - a.phi[1]) + a.plo[1]) : 0;
         }
-    }
 
-    for (int i = istart; i < x1; ++i) {
-        const bool own_plane = i >= x0;
-        const bool has_next = i + 1 < x1;
-        const int pi = wrap_or_ghost(i, nx, a.px);
-        const int pn = wrap_or_ghost(i + 1, nx, a.px);
-        const int pnn = wrap_or_ghost(i + 2, nx, a.px);
-        const bool xpml = in_pml(a, 0, pi, nx);
-        const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
-        GLOBAL const T* cx = coef + a.coff[0];
-        const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[nx + pi] : T(0);
-        const T bHx = xpml ? cx[2 * nx + pi] : T(0), aHx = xpml ? cx[3 * nx + pi] : T(0);
-        const Rsrc rF = rsrc(F0 + (long)pi * FPB);
-        const Rsrc rW = rsrc(F1 + (long)pi * FPB);
+        const bool zpml = kin && in_pml(a, 2, k, nz);
+        const bool zany = wave_any(zpml);  // some lane of this wavefront lies in a z PML layer
+        const uint32_t oK = kin ? (uint32_t)k * TS : kOOB;
+        const uint32_t oZ = zpml ? (uint32_t)(k < a.plo[2] ? k : k - (nz - a.phi[2]) + a.plo[2]) * TS : kOOB;
+        GLOBAL const T* cz = coef + a.coff[2];
+        const T bEz = zpml ? cz[k] : T(0), aEz = zpml ? cz[nz + k] : T(0);
+        const T bHz = zpml ? cz[2 * nz + k] : T(0), aHz = zpml ? cz[3 * nz + k] : T(0);
 
-        // ---------------------------------------------------------------- loads ----------
-        T enn[TY + 2][3];
-        uint32_t mwn[TY];
+        lds_barrier();
+
+        T ec[TY + 2][3];  // E^n on plane i, rows -1..TY
+        T en[TY + 2][3];  // E^n on plane i+1
+        T hp[TY][2];      // H^{n+1/2} (Hy, Hz) on plane i-1, rows 0..TY-1
+        uint32_t mw[TY];  // material words of plane i, owned rows
+
+        const int istart = (x0 > 0 || a.px) ? x0 - 1 : x0;
         {
-            // plane i + 2 (a ghost plane reads as zero; after the last plane nothing is read)
-            const bool nn_ok = has_next && pnn >= 0;
-            const Rsrc rn = rsrc(F0 + (long)(nn_ok ? pnn : pi) * FPB);
+            const int p0 = wrap_or_ghost(istart, nx, a.px);
+            const int p1 = wrap_or_ghost(istart + 1, nx, a.px);
+            const Rsrc r0 = rsrc(F0 + (long)(p0 >= 0 ? p0 : 0) * FPB);
+            const Rsrc r1 = rsrc(F0 + (long)(p1 >= 0 ? p1 : 0) * FPB);
 #pragma unroll
             for (int r = 0; r < TY + 2; ++r) {
-                const uint32_t o = nn_ok ? oR[r] : kOOB;
-                enn[r][0] = bld<T>(rn, o, 0);
-                enn[r][1] = (r < TY + 1) ? bld<T>(rn, o, PB) : T(0);
-                enn[r][2] = bld<T>(rn, o, 2 * PB);
-            }
-            const bool mwn_ok = !uniform && has_next && pn >= 0;
-            const Rsrc rm = rsrc(matw + (long)(mwn_ok ? pn : pi) * PW);
-#pragma unroll
-            for (int r = 0; r < TY; ++r) {
-                mwn[r] = 0u;
-                if (mwn_ok && own_row[r]) mwn[r] = bld<uint32_t>(rm, oW[r + 1], 0);
-            }
-        }
-        T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
-#pragma unroll
-        for (int r = 0; r < TY + 1; ++r) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) h0[r][c] = bld<T>(rF, oR[r], (3 + c) * PB);
-#pragma unroll
-            for (int c = 0; c < 2; ++c) ek1[r][c] = bld<T>(rF, oK1[r], c * PB);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
-            if (rv[r] && xpml) {
-                pH[r][0][0] = bld<T>(rpx, oR[r], hr * SB0 + sxo);
-                pH[r][0][1] = bld<T>(rpx, oR[r], (hr + 1) * SB0 + sxo);
-            }
-            if (ypml[r]) {
-                const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
-                pH[r][1][0] = bld<T>(rpy, oK, hr * SB1 + so);
-                pH[r][1][1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
-            }
-            if (rv[r] && zany) {
-                const uint32_t so = (uint32_t)((pi * ny + gj[r]) * Lz) * TS;
-                pH[r][2][0] = bld<T>(rpz, oZ, hr * SB2 + so);
-                pH[r][2][1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
-            }
-        }
-        T pE[TY][3][2];
-        T Qv[TY][3][NP > 0 ? NP : 1], Pv[TY][3][NP > 0 ? NP : 1];
-        const Rsrc rQ = rsrc((const char*)a.Q + (long)pi * QPB);
-        const Rsrc rP = rsrc((const char*)a.P + (long)pi * QPB);
-#pragma unroll
-        for (int r = 0; r < TY; ++r) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) pE[r][d][0] = pE[r][d][1] = T(0);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[r][c][s] = Pv[r][c][s] = T(0);
-            if (!(own_plane && own_row[r])) continue;
-            const uint32_t o = oR[r + 1];
-            if (xpml) {
-                pE[r][0][0] = bld<T>(rpx, o, sxo);
-                pE[r][0][1] = bld<T>(rpx, o, SB0 + sxo);
-            }
-            if (ypml[r + 1]) {
-                const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
-                pE[r][1][0] = bld<T>(rpy, oK, so);
-                pE[r][1][1] = bld<T>(rpy, oK, SB1 + so);
-            }
-            if (zany) {
-                const uint32_t so = (uint32_t)((pi * ny + gj[r + 1]) * Lz) * TS;
-                pE[r][2][0] = bld<T>(rpz, oZ, so);
-                pE[r][2][1] = bld<T>(rpz, oZ, SB2 + so);
-            }
-            if (NP > 0 && !uniform) {
-                int np[3], lm[3];
+                const uint32_t o0 = p0 >= 0 ? oR[r] : kOOB, o1 = p1 >= 0 ? oR[r] : kOOB;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const int m = (mw[r] >> (8 * c)) & 0xff;
-                    const int mc = m == kDesignMat ? 0 : m;  // LUT index (design voxels have no poles)
-                    np[c] = m == kDesignMat ? 0 : slut[mc].np;
-                    lm[c] = slut[mc].lorentz_mask;
-                }
-#pragma unroll
-                for (int s = 0; s < NP; ++s) {
-                    if (!wave_any(s < np[0] || s < np[1] || s < np[2])) continue;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        Qv[r][c][s] = bld<T>(rQ, s < np[c] ? o : kOOB, (s * 3 + c) * PB);
-                        Pv[r][c][s] = bld<T>(rP, (s < np[c] && ((lm[c] >> s) & 1)) ? o : kOOB, (s * 3 + c) * PB);
-                    }
+                    ec[r][c] = bld<T>(r0, o0, c * PB);
+                    en[r][c] = bld<T>(r1, o1, c * PB);
                 }
             }
-        }
-
-        // ---------------------------------------------------------------- H^{n+1/2} -------
-        T h[TY + 1][3];
-#pragma unroll
-        for (int r = 0; r < TY + 1; ++r) {
-            const bool ok = rv[r];
-            const int j = gj[r];
-            const bool wr = own_plane && r >= 1 && own_row[r > 0 ? r - 1 : 0] && ok;
-            T dEz_y = ec[r + 1][2] - ec[r][2];
-            T dEx_y = ec[r + 1][0] - ec[r][0];
-            T dEy_z = ek1[r][1] - ec[r][1];
-            T dEx_z = ek1[r][0] - ec[r][0];
-            T dEz_x = en[r][2] - ec[r][2];
-            T dEy_x = en[r][1] - ec[r][1];
-            if (ypml[r]) {
-                GLOBAL const T* cy = coef + a.coff[1];
-                const T b = cy[2 * ny + j], c = cy[3 * ny + j];
-                dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
-                dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
-                if (wr) {
-                    const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
-                    bst(rpy, oK, hw * SB1 + so, pH[r][1][0]);
-                    bst(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
-                }
-            }
-            if (ok && zany) {  // lanes outside the layers: b = a = 0 leaves the term unchanged
-                dEy_z = cpml(dEy_z, bHz, aHz, pH[r][2][0]);
-                dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
-                if (wr) {
-                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    bst(rpz, oZ, hw * SB2 + so, pH[r][2][0]);
-                    bst(rpz, oZ, (hw + 1) * SB2 + so, pH[r][2][1]);
-                }
-            }
-            if (ok && xpml) {
-                dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
-                dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
-                if (wr) {
-                    bst(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
-                    bst(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
-                }
-            }
-            // outside a metallic edge H is identically zero (ghost row of the halo)
-            h[r][0] = ok ? h0[r][0] - cdt * (dEz_y - dEy_z) : T(0);
-            h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
-            h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
-            if (wr) {
-                bst(rW, oR[r], 3 * PB, h[r][0]);
-                bst(rW, oR[r], 4 * PB, h[r][1]);
-                bst(rW, oR[r], 5 * PB, h[r][2]);
-            }
-        }
-
-        if (own_plane) {
-            T* buf = sh + (i & 1) * 2 * TY * B;
+            const bool mw0 = !uniform && istart >= x0;
+            const Rsrc rm = rsrc(matw + (long)(p0 >= 0 ? p0 : 0) * PW);
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
-                buf[r * B + kl] = h[r + 1][0];
-                buf[(TY + r) * B + kl] = h[r + 1][1];
+                hp[r][0] = hp[r][1] = T(0);
+                mw[r] = 0u;
+                if (mw0 && own_row[r]) mw[r] = bld<uint32_t>(rm, oW[r + 1], 0);
             }
-            if (ZP && pass + 1 < npass && kl == Z - 1) {
-                T* zo = zh + ((pass & 1) * a.xc + (i - x0)) * 2 * TY;
+        }
+
+        for (int i = istart; i < x1; ++i) {
+            const bool own_plane = i >= x0;
+            const bool has_next = i + 1 < x1;
+            const int pi = wrap_or_ghost(i, nx, a.px);
+            const int pn = wrap_or_ghost(i + 1, nx, a.px);
+            const int pnn = wrap_or_ghost(i + 2, nx, a.px);
+            const bool xpml = in_pml(a, 0, pi, nx);
+            const uint32_t sxo = xpml ? (uint32_t)(pi < a.plo[0] ? pi : pi - (nx - a.phi[0]) + a.plo[0]) * PB : 0u;
+            GLOBAL const T* cx = coef + a.coff[0];
+            const T bEx = xpml ? cx[pi] : T(0), aEx = xpml ? cx[nx + pi] : T(0);
+            const T bHx = xpml ? cx[2 * nx + pi] : T(0), aHx = xpml ? cx[3 * nx + pi] : T(0);
+            const Rsrc rF = rsrc(F0 + (long)pi * FPB);
+            const Rsrc rW = rsrc(F1 + (long)pi * FPB);
+
+            // ---------------------------------------------------------------- loads ----------
+            T enn[TY + 2][3];
+            uint32_t mwn[TY];
+            {
+                // plane i + 2 (a ghost plane reads as zero; after the last plane nothing is read)
+                const bool nn_ok = has_next && pnn >= 0;
+                const Rsrc rn = rsrc(F0 + (long)(nn_ok ? pnn : pi) * FPB);
+#pragma unroll
+                for (int r = 0; r < TY + 2; ++r) {
+                    const uint32_t o = nn_ok ? oR[r] : kOOB;
+                    enn[r][0] = bld<T>(rn, o, 0);
+                    enn[r][1] = (r < TY + 1) ? bld<T>(rn, o, PB) : T(0);
+                    enn[r][2] = bld<T>(rn, o, 2 * PB);
+                }
+                const bool mwn_ok = !uniform && has_next && pn >= 0;
+                const Rsrc rm = rsrc(matw + (long)(mwn_ok ? pn : pi) * PW);
 #pragma unroll
                 for (int r = 0; r < TY; ++r) {
-                    zo[2 * r] = h[r + 1][0];
-                    zo[2 * r + 1] = h[r + 1][1];
+                    mwn[r] = 0u;
+                    if (mwn_ok && own_row[r]) mwn[r] = bld<uint32_t>(rm, oW[r + 1], 0);
                 }
             }
-            lds_barrier();
-
-            // ------------------------------------------------------------ E^{n+1} ---------
+            T h0[TY + 1][3], ek1[TY + 1][2], pH[TY + 1][3][2];
+#pragma unroll
+            for (int r = 0; r < TY + 1; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) h0[r][c] = bld<T>(rF, oR[r], (3 + c) * PB);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) ek1[r][c] = bld<T>(rF, oK1[r], c * PB);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) pH[r][d][0] = pH[r][d][1] = T(0);
+                if (rv[r] && xpml) {
+                    pH[r][0][0] = bld<T>(rpx, oR[r], hr * SB0 + sxo);
+                    pH[r][0][1] = bld<T>(rpx, oR[r], (hr + 1) * SB0 + sxo);
+                }
+                if (ypml[r]) {
+                    const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
+                    pH[r][1][0] = bld<T>(rpy, oK, hr * SB1 + so);
+                    pH[r][1][1] = bld<T>(rpy, oK, (hr + 1) * SB1 + so);
+                }
+                if (rv[r] && zany) {
+                    const uint32_t so = (uint32_t)((pi * ny + gj[r]) * Lz) * TS;
+                    pH[r][2][0] = bld<T>(rpz, oZ, hr * SB2 + so);
+                    pH[r][2][1] = bld<T>(rpz, oZ, (hr + 1) * SB2 + so);
+                }
+            }
+            T pE[TY][3][2];
+            T Qv[TY][3][NP > 0 ? NP : 1], Pv[TY][3][NP > 0 ? NP : 1];
+            const Rsrc rQ = rsrc((const char*)a.Q + (long)pi * QPB);
+            const Rsrc rP = rsrc((const char*)a.P + (long)pi * QPB);
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
-                if (!own_row[r]) continue;
-                const int j = gj[r + 1];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) pE[r][d][0] = pE[r][d][1] = T(0);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int s = 0; s < (NP > 0 ? NP : 1); ++s) Qv[r][c][s] = Pv[r][c][s] = T(0);
+                if (!(own_plane && own_row[r])) continue;
                 const uint32_t o = oR[r + 1];
-                T hxkm = T(0), hykm = T(0);
-                if (kml >= 0) {
-                    hxkm = buf[r * B + kml];
-                    hykm = buf[(TY + r) * B + kml];
-                } else if (ZP && kml == -2) {
-                    const T* zi = zh + (((pass - 1) & 1) * a.xc + (i - x0)) * 2 * TY;
-                    hxkm = zi[2 * r];
-                    hykm = zi[2 * r + 1];
+                if (xpml) {
+                    pE[r][0][0] = bld<T>(rpx, o, sxo);
+                    pE[r][0][1] = bld<T>(rpx, o, SB0 + sxo);
                 }
-                T dHz_y = h[r + 1][2] - h[r][2];
-                T dHx_y = h[r + 1][0] - h[r][0];
-                T dHy_z = h[r + 1][1] - hykm;
-                T dHx_z = h[r + 1][0] - hxkm;
-                T dHz_x = h[r + 1][2] - hp[r][1];
-                T dHy_x = h[r + 1][1] - hp[r][0];
                 if (ypml[r + 1]) {
-                    GLOBAL const T* cy = coef + a.coff[1];
-                    const T b = cy[j], c = cy[ny + j];
-                    dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
-                    dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
                     const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
-                    bst(rpy, oK, so, pE[r][1][0]);
-                    bst(rpy, oK, SB1 + so, pE[r][1][1]);
+                    pE[r][1][0] = bld<T>(rpy, oK, so);
+                    pE[r][1][1] = bld<T>(rpy, oK, SB1 + so);
                 }
                 if (zany) {
-                    dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
-                    dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
-                    const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    bst(rpz, oZ, so, pE[r][2][0]);
-                    bst(rpz, oZ, SB2 + so, pE[r][2][1]);
+                    const uint32_t so = (uint32_t)((pi * ny + gj[r + 1]) * Lz) * TS;
+                    pE[r][2][0] = bld<T>(rpz, oZ, so);
+                    pE[r][2][1] = bld<T>(rpz, oZ, SB2 + so);
                 }
-                if (xpml) {
-                    dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
-                    dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
-                    bst(rpx, o, sxo, pE[r][0][0]);
-                    bst(rpx, o, SB0 + sxo, pE[r][0][1]);
-                }
-                const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
-                if (uniform) {
-                    const T inv = slut[0].inv_eps;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
-                } else {
-                    int mm[3];
-                    bool des[3];
-                    T inv[3];
+                if (NP > 0 && !uniform) {
+                    int np[3], lm[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const int m = (mw[r] >> (8 * c)) & 0xff;
-                        des[c] = m == kDesignMat;
-                        mm[c] = des[c] ? 0 : m;
-                        inv[c] = slut[mm[c]].inv_eps;
+                        const int mc = m == kDesignMat ? 0 : m;  // LUT index (design voxels have no poles)
+                        np[c] = m == kDesignMat ? 0 : slut[mc].np;
+                        lm[c] = slut[mc].lorentz_mask;
                     }
-                    if (a.emap) {  // per-cell eps_inf (smoothed maps)
-                        const Rsrc re = rsrc((const char*)a.emap + (long)pi * 3 * PB);
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) inv[c] = bld<T>(re, o, c * PB);
-                    }
-                    if (wave_any(des[0] || des[1] || des[2])) {
-                        // design voxels: 1/eps from the design array [dx][dy][dz]
-                        const Rsrc rd = rsrc(a.dinv);
-                        const uint32_t dq = (uint32_t)(((pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
-                                                           (a.dhi[2] - a.dlo[2]) + (k - a.dlo[2])) * TS;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
-                            inv[c] = des[c] ? v : inv[c];
-                        }
-                    }
-                    T sq[3] = {T(0), T(0), T(0)};
 #pragma unroll
                     for (int s = 0; s < NP; ++s) {
-                        int npl[3];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) npl[c] = des[c] ? 0 : slut[mm[c]].np;
-                        if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+                        if (!wave_any(s < np[0] || s < np[1] || s < np[2])) continue;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
-                            const MatLut<T>& L = slut[mm[c]];
-                            const bool vq = s < npl[c];
-                            const bool vp = vq && ((L.lorentz_mask >> s) & 1);
-                            T qn;
-                            if constexpr (kPolePair<T>) {
-                                const T cur = podd ? Pv[r][c][s] : Qv[r][c][s];  // Lorentz: P^n
-                                const T old = podd ? Qv[r][c][s] : Pv[r][c][s];  // and P^{n-1}
-                                qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
-                                        : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
-                                qn = vq ? qn : T(0);
-                                bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
-                                bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
-                            } else {
-                                // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
-                                qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
-                                qn = vq ? qn : T(0);
-                                bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
-                                bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
-                            }
-                            sq[c] += qn;
+                            Qv[r][c][s] = bld<T>(rQ, s < np[c] ? o : kOOB, (s * 3 + c) * PB);
+                            Pv[r][c][s] = bld<T>(rP, (s < np[c] && ((lm[c] >> s) & 1)) ? o : kOOB, (s * 3 + c) * PB);
                         }
                     }
+                }
+            }
+
+            // ---------------------------------------------------------------- H^{n+1/2} -------
+            T h[TY + 1][3];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv[c] * (cdt * curl[c] - sq[c]));
+            for (int r = 0; r < TY + 1; ++r) {
+                const bool ok = rv[r];
+                const int j = gj[r];
+                const bool wr = own_plane && r >= 1 && own_row[r > 0 ? r - 1 : 0] && ok;
+                T dEz_y = ec[r + 1][2] - ec[r][2];
+                T dEx_y = ec[r + 1][0] - ec[r][0];
+                T dEy_z = ek1[r][1] - ec[r][1];
+                T dEx_z = ek1[r][0] - ec[r][0];
+                T dEz_x = en[r][2] - ec[r][2];
+                T dEy_x = en[r][1] - ec[r][1];
+                if (ypml[r]) {
+                    GLOBAL const T* cy = coef + a.coff[1];
+                    const T b = cy[2 * ny + j], c = cy[3 * ny + j];
+                    dEz_y = cpml(dEz_y, b, c, pH[r][1][0]);
+                    dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
+                    if (wr) {
+                        const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
+                        bst(rpy, oK, hw * SB1 + so, pH[r][1][0]);
+                        bst(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
+                    }
+                }
+                if (ok && zany) {  // lanes outside the layers: b = a = 0 leaves the term unchanged
+                    dEy_z = cpml(dEy_z, bHz, aHz, pH[r][2][0]);
+                    dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
+                    if (wr) {
+                        const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                        bst(rpz, oZ, hw * SB2 + so, pH[r][2][0]);
+                        bst(rpz, oZ, (hw + 1) * SB2 + so, pH[r][2][1]);
+                    }
+                }
+                if (ok && xpml) {
+                    dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
+                    dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
+                    if (wr) {
+                        bst(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
+                        bst(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
+                    }
+                }
+                // outside a metallic edge H is identically zero (ghost row of the halo)
+                h[r][0] = ok ? h0[r][0] - cdt * (dEz_y - dEy_z) : T(0);
+                h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
+                h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
+                if (wr) {
+                    bst(rW, oR[r], 3 * PB, h[r][0]);
+                    bst(rW, oR[r], 4 * PB, h[r][1]);
+                    bst(rW, oR[r], 5 * PB, h[r][2]);
+                }
+            }
+
+            if (own_plane) {
+                T* buf = sh + (i & 1) * 2 * TY * B;
+#pragma unroll
+                for (int r = 0; r < TY; ++r) {
+                    buf[r * B + kl] = h[r + 1][0];
+                    buf[(TY + r) * B + kl] = h[r + 1][1];
+                }
+                if (ZP && pass + 1 < npass && kl == Z - 1) {
+                    T* zo = zh + ((pass & 1) * a.xc + (i - x0)) * 2 * TY;
+#pragma unroll
+                    for (int r = 0; r < TY; ++r) {
+                        zo[2 * r] = h[r + 1][0];
+                        zo[2 * r + 1] = h[r + 1][1];
+                    }
+                }
+                lds_barrier();
+
+                // ------------------------------------------------------------ E^{n+1} ---------
+#pragma unroll
+                for (int r = 0; r < TY; ++r) {
+                    if (!own_row[r]) continue;
+                    const int j = gj[r + 1];
+                    const uint32_t o = oR[r + 1];
+                    // z neighbour below: the lane below in this pass, else the previous
+                    // pass's last lane (zh), else the periodic image (one pass), else a ghost
+                    T hxkm = T(0), hykm = T(0);
+                    if (kl > 0) {
+                        hxkm = buf[r * B + kl - 1];
+                        hykm = buf[(TY + r) * B + kl - 1];
+                    } else if (ZP && pass > 0) {
+                        const T* zi = zh + (((pass - 1) & 1) * a.xc + (i - x0)) * 2 * TY;
+                        hxkm = zi[2 * r];
+                        hykm = zi[2 * r + 1];
+                    } else if (a.pz) {  // periodic z runs one pass: the image of k = 0 is nz - 1
+                        hxkm = buf[r * B + nz - 1];
+                        hykm = buf[(TY + r) * B + nz - 1];
+                    }
+                    T dHz_y = h[r + 1][2] - h[r][2];
+                    T dHx_y = h[r + 1][0] - h[r][0];
+                    T dHy_z = h[r + 1][1] - hykm;
+                    T dHx_z = h[r + 1][0] - hxkm;
+                    T dHz_x = h[r + 1][2] - hp[r][1];
+                    T dHy_x = h[r + 1][1] - hp[r][0];
+                    if (ypml[r + 1]) {
+                        GLOBAL const T* cy = coef + a.coff[1];
+                        const T b = cy[j], c = cy[ny + j];
+                        dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
+                        dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
+                        const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
+                        bst(rpy, oK, so, pE[r][1][0]);
+                        bst(rpy, oK, SB1 + so, pE[r][1][1]);
+                    }
+                    if (zany) {
+                        dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
+                        dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
+                        const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
+                        bst(rpz, oZ, so, pE[r][2][0]);
+                        bst(rpz, oZ, SB2 + so, pE[r][2][1]);
+                    }
+                    if (xpml) {
+                        dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
+                        dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
+                        bst(rpx, o, sxo, pE[r][0][0]);
+                        bst(rpx, o, SB0 + sxo, pE[r][0][1]);
+                    }
+                    const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
+                    if (uniform) {
+                        const T inv = slut[0].inv_eps;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
+                    } else {
+                        int mm[3];
+                        bool des[3];
+                        T inv[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const int m = (mw[r] >> (8 * c)) & 0xff;
+                            des[c] = m == kDesignMat;
+                            mm[c] = des[c] ? 0 : m;
+                            inv[c] = slut[mm[c]].inv_eps;
+                        }
+                        if (a.emap) {  // per-cell eps_inf (smoothed maps)
+                            const Rsrc re = rsrc((const char*)a.emap + (long)pi * 3 * PB);
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) inv[c] = bld<T>(re, o, c * PB);
+                        }
+                        if (wave_any(des[0] || des[1] || des[2])) {
+                            // design voxels: 1/eps from the design array [dx][dy][dz]
+                            const Rsrc rd = rsrc(a.dinv);
+                            const uint32_t dq = (uint32_t)(((pi - a.dlo[0]) * (a.dhi[1] - a.dlo[1]) + (j - a.dlo[1])) *
+                                                               (a.dhi[2] - a.dlo[2]) + (kl + pass * Z - a.dlo[2])) * TS;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const T v = bld<T>(rd, des[c] ? dq : kOOB, 0);
+                                inv[c] = des[c] ? v : inv[c];
+                            }
+                        }
+                        T sq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+                        for (int s = 0; s < NP; ++s) {
+                            int npl[3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) npl[c] = des[c] ? 0 : slut[mm[c]].np;
+                            if (!wave_any(s < npl[0] || s < npl[1] || s < npl[2])) continue;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const MatLut<T>& L = slut[mm[c]];
+                                const bool vq = s < npl[c];
+                                const bool vp = vq && ((L.lorentz_mask >> s) & 1);
+                                T qn;
+                                if constexpr (kPolePair<T>) {
+                                    const T cur = podd ? Pv[r][c][s] : Qv[r][c][s];  // Lorentz: P^n
+                                    const T old = podd ? Qv[r][c][s] : Pv[r][c][s];  // and P^{n-1}
+                                    qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
+                                            : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
+                                    qn = vq ? qn : T(0);
+                                    bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
+                                    bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                                } else {
+                                    // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
+                                    qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
+                                    qn = vq ? qn : T(0);
+                                    bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
+                                    bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
+                                }
+                                sq[c] += qn;
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv[c] * (cdt * curl[c] - sq[c]));
+                    }
+                }
+            }
+
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                hp[r][0] = h[r + 1][1];
+                hp[r][1] = h[r + 1][2];
+                mw[r] = mwn[r];
+            }
+#pragma unroll
+            for (int r = 0; r < TY + 2; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ec[r][c] = en[r][c];
+                    en[r][c] = enn[r][c];
                 }
             }
         }
-
-#pragma unroll
-        for (int r = 0; r < TY; ++r) {
-            hp[r][0] = h[r + 1][1];
-            hp[r][1] = h[r + 1][2];
-            mw[r] = mwn[r];
-        }
-#pragma unroll
-        for (int r = 0; r < TY + 2; ++r) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ec[r][c] = en[r][c];
-                en[r][c] = enn[r][c];
-            }
-        }
-    }
     }  // pass
 }
 
