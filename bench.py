@@ -213,7 +213,7 @@ def adjoint_batched(a):
     steps = a.steps if a.steps is not None else scns[0].steps
     engs = [s.make(Engine) for s in scns]
     for e in engs:
-        set_tile(a, e)  # a register-rows tile also sets the reverse step's x-chunk
+        set_tile(a, e)  # each engine's own tile: a --batch-tile x-chunk of 0 takes its x-chunk
     b = Batch(engs)
     set_batch_tile(a, b)
     cells, _ = b.info()
@@ -256,7 +256,7 @@ def adjoint(a, variants=False):
     rev = None
     for s in scns:
         eng = s.make(Engine)
-        set_tile(a, eng)  # the forward steps and the reverse steps run the tuned x-chunk
+        set_tile(a, eng)  # the forward tile (--tile or the tuned cache); --adjoint-tile the reverse one
         if a.adjoint_tile:
             f, r, x = a.adjoint_tile.split(",")
             eng.set_adjoint_tile(f, int(r), int(x))

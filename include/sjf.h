@@ -245,9 +245,8 @@ int sjf_step_bytes(const sjf_engine* e, double* bytes);
 /* Launch configuration of the fused update kernel: form, y rows (per thread or per
  * workgroup), x-chunk length, threads per workgroup, workgroups. */
 int sjf_launch_info(const sjf_engine* e, int* form, int* rows, int* xchunk, int* threads, int* groups);
-/* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x). The
- * x-chunk of a REGISTER_ROWS tile also applies to the reverse step of sjf_adjoint_run (one y
- * row per workgroup); with a THREAD_ROWS tile the reverse step keeps its own default. */
+/* Override the default tile shape (xchunk >= 1: planes each workgroup marches along x) of the
+ * forward step.  The reverse step of sjf_adjoint_run has its own (sjf_set_adjoint_tile). */
 int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
 /* REGISTER_ROWS tile: a workgroup covers its z line in passes of `cells` cells (a multiple of
  * 64, at most 1024), the last lane of a pass handing its H to the first lane of the next through

@@ -911,14 +911,15 @@ int timed_set(LaunchSet* s, long n, double* ms_total, double* ms_kernel_mean) {
 
 void choose_tile(sjf_engine* e) {
     if (e->tile_set) return;
-    // Defaults measured on MI355X: for short z lines (nz <= 96) the row-of-threads form wins
-    // (up to ~1024 threads per workgroup, few halo rows); from 128 cells on one row per thread
-    // keeps more waves resident.  The autotuner refines per workload.
+    // Defaults measured on MI355X: for short z lines (nz < 96) the row-of-threads form wins
+    // (up to ~1024 threads per workgroup, few halo rows); from 96 cells on one row per thread
+    // keeps more waves resident (192x192x96 FP32: 0.120 ms against 0.126 ms for 4 thread rows;
+    // 96^3 FP64: 0.0467 against 0.0485 ms).  The autotuner refines per workload.
     long target_groups = 16384;
     e->form = SJF_TILE_REGISTER_ROWS;
     e->ty = 1;
     const int rows = e->g.nz > 96 ? default_thread_rows(e) : std::min(3, default_thread_rows(e));
-    if (e->g.nz <= 96 && thread_rows_fit(e, rows)) {
+    if (e->g.nz < 96 && thread_rows_fit(e, rows)) {
         e->form = SJF_TILE_THREAD_ROWS;
         e->ty = rows;
         target_groups = 2048;
@@ -1264,15 +1265,14 @@ int set_step(sjf_engine* e, long n) {
     return SJF_OK;
 }
 
-// x-chunk of the fused reverse step (one y row per workgroup): that of a register-rows tile set
-// on the engine (the same tiling), else enough chunks for about `target` workgroups over `rows`
-// y rows (the engine's own, or those of every engine of a batch).  Measured on MI355X: alone,
+// x-chunk of the fused reverse step (one y row per workgroup) unless sjf_set_adjoint_tile sets
+// one: enough chunks for about `target` workgroups over `rows` y rows (the engine's own, or those
+// of every engine of a batch), independent of the forward tile.  Measured on MI355X: alone,
 // 256x256x128 runs 1.019 ms per gradient step at 4 planes (16384 workgroups), 1.041 at 8 and
 // 1.13 at 16; sixteen 192x192x96 engines in one batch run the reverse launch in 1.96 ms at 3
 // planes (196608 workgroups), 1.82 ms at 6 and 1.79 ms at 12 on one box, and in 1.61 ms at 18
 // planes (33792 workgroups) against 1.64-1.68 ms at 12 and 1.69-1.71 ms at 32 on another.
 int adj_xchunk(const sjf_engine* e, long target = 16384, long rows = 0) {
-    if (e->tile_set && e->form == SJF_TILE_REGISTER_ROWS) return std::min(e->xc, e->g.nx);
     if (rows <= 0) rows = e->g.ny;
     long nxc = std::max(1L, (target + rows - 1) / rows);
     nxc = std::min<long>(nxc, e->g.nx);
@@ -2584,15 +2584,19 @@ int sjf_batch_create(sjf_engine* const* engines, int n, sjf_batch** out) {
     s->own_stream = true;
     s->prec = e0->prec;
     s->ty = 1;
-    // measured on MI355X for a sweep of 64 instances (64^3 to 192x192x128): the row-of-threads
-    // form with 6-plane x-chunks (3.13 ms per step) beats 3 to 12 planes (3.15-3.27 ms), one row
-    // per thread (3.46 ms) and the per-instance chunks (3.37 ms)
-    s->form = SJF_TILE_THREAD_ROWS;
-    s->xc = 6;
+    // measured on MI355X: with some z line shorter than 96 cells (the sweep of 64 instances,
+    // 64^3 to 192x192x128) the row-of-threads form with 6-plane x-chunks (2.78 ms per step)
+    // beats one row per thread (2.89-2.96 ms at 6-32 planes); with every line of 96 cells or more
+    // (the 16 design variants, 192x192x96) one row per thread with 16-plane chunks wins (3.38
+    // against 3.54 ms per gradient step)
+    bool short_z = false;
+    for (int i = 0; i < n; ++i) short_z = short_z || engines[i]->g.nz < 96;
+    s->form = short_z ? SJF_TILE_THREAD_ROWS : SJF_TILE_REGISTER_ROWS;
+    s->xc = short_z ? 6 : 16;
     for (int i = 0; i < n; ++i)
         if (!thread_rows_fit(engines[i], default_thread_rows(engines[i]))) {
             s->form = SJF_TILE_REGISTER_ROWS;
-            s->xc = 0;
+            s->xc = 16;
         }
     for (int i = 0; i < n; ++i) {
         s->eng.push_back(engines[i]);

@@ -182,10 +182,11 @@ def test_adjoint_independent_of_checkpoint_interval_and_leaves_final_state():
 
 def test_adjoint_kernel_info_reports_the_reverse_launch():
     """The reverse kernel's timing and launch shape: absent before an adjoint run, then one y row
-    per workgroup over x-chunks of the set tile's length."""
+    per workgroup over the x-chunks sjf_set_adjoint_tile gives, else its own default (about
+    16384 workgroups), whatever the forward tile."""
     scn, gap, eps = design_case("poles", precision="fp32", steps=30)
     eng = scn.make(Engine)
-    eng.set_tile("register_rows", 1, 4)
+    eng.set_adjoint_tile("register_rows", 1, 4)
     with pytest.raises(EngineError):
         eng.adjoint_kernel_info()
     eng.adjoint_run(scn.steps, 0, gap, eps.size)
@@ -195,14 +196,14 @@ def test_adjoint_kernel_info_reports_the_reverse_launch():
     assert threads == -(-nz // 64) * 64
     assert groups == ny * -(-nx // 4)
     eng.close()
-    # a row-of-threads tile leaves the reverse step its own x-chunks (about 16384 workgroups)
-    eng = scn.make(Engine)
-    eng.set_tile("thread_rows", 1, 4)
-    eng.adjoint_run(scn.steps, 0, gap, eps.size)
-    _, _, groups = eng.adjoint_kernel_info()
     nxc = min(-(-16384 // ny), nx)
-    assert groups == ny * -(-nx // -(-nx // nxc))
-    eng.close()
+    for tile in (("register_rows", 1, 4), ("thread_rows", 1, 4)):
+        eng = scn.make(Engine)
+        eng.set_tile(*tile)
+        eng.adjoint_run(scn.steps, 0, gap, eps.size)
+        _, _, groups = eng.adjoint_kernel_info()
+        assert groups == ny * -(-nx // -(-nx // nxc)), tile
+        eng.close()
 
 
 def test_adjoint_fp32_close_to_fp64():
@@ -319,9 +320,9 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     (the per-engine run is pinned to finite differences of the oracle above), and leaves each
     engine in its final forward state.  The batch's forward launch runs one tile form for all
     engines, and the forms round differently in the last bit, so each mode runs the same form
-    alone and in the batch: register rows with each engine's x-chunk (which the reverse step
-    then takes too), or the default row-of-threads form (the reverse step's default chunks),
-    also with the batch checkpointing every 13 steps and recomputing (the engines alone record)."""
+    alone and in the batch: register rows with each engine's x-chunk, or the default
+    row-of-threads form, also with the batch checkpointing every 13 steps and recomputing (the
+    engines alone record).  The reverse launch takes x-chunks common to all engines."""
     from sim_juncs_amd.engine import Batch
 
     steps = 60
@@ -357,11 +358,9 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     rows = sum(scn.shape[1] for scn, _, _ in cases)
     for (scn, _, _), m in zip(cases, BATCH_MEMBERS):
         nx, ny, _ = scn.shape
-        if mode == "register_rows":
-            xc = m[5]
-        else:  # x-chunks common to all engines, for about 32768 workgroups in all
-            nxc = min(-(-32768 // rows), nx)
-            xc = -(-nx // nxc)
+        # x-chunks common to all engines, for about 32768 workgroups in all
+        nxc = min(-(-32768 // rows), nx)
+        xc = -(-nx // nxc)
         tiles += ny * -(-nx // xc)
     assert groups == tiles
     for i, eng in enumerate(engs):
