@@ -443,3 +443,60 @@ def test_z_pass_arguments_are_checked():
     with pytest.raises(EngineError):  # a periodic z axis needs one pass
         eng.set_zsegment(64)
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_z_passes_with_design_region_and_eps_maps(precision):
+    """Design voxels and per-cell eps maps that straddle a pass boundary (z 200..300 across the
+    256-cell passes of a 512-cell line) step exactly as in one pass."""
+    shape = (8, 8, 512)
+    scn, rng = random_scenario(shape, seed=5, pml=[3] * 6, steps=30)
+    lo, hi = (2, 2, 200), (6, 7, 300)
+    scn.design = (lo, hi, rng.uniform(1.5, 4.0, size=tuple(h - l for l, h in zip(lo, hi))))
+    scn.eps_maps = rng.uniform(1.0, 3.0, size=(3,) + shape)
+    runs = []
+    for zseg in (0, 512, 64):
+        print(f"[case] z passes design+maps {precision} zseg={zseg}", flush=True)
+        eng = scn.make(Engine, precision=precision)
+        eng.set_tile("register_rows", 1, 4)
+        eng.set_zsegment(zseg)
+        assert eng.launch_info()["threads"] == (zseg or 256)
+        eng.step(30)
+        runs.append([eng.read_field(c) for c in range(6)])
+        eng.close()
+    assert np.abs(runs[0][0]).max() > 0
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_batch_of_pass_and_one_pass_lines_equals_each_engine():
+    """One fused launch over a 512-cell line stepped in passes, a 130-cell line and a periodic
+    96-cell line (both one pass) gives each engine the bits of its own run."""
+    from sim_juncs_amd.engine import Batch
+
+    specs = [((6, 5, 512), (0, 0, 0)), ((7, 6, 130), (0, 0, 0)), ((6, 6, 96), (0, 0, 1))]
+    scns = []
+    for i, (shape, per) in enumerate(specs):
+        scn, _ = random_scenario(shape, seed=30 + i, pml=None if per[2] else [2] * 6, periodic=per, steps=40)
+        scns.append(scn)
+    print("[case] batch of pass and one-pass lines", [s.shape for s in scns], flush=True)
+    own = []
+    for scn in scns:
+        eng = scn.make(Engine, precision="fp64")
+        eng.set_tile("register_rows", 1, 4)
+        eng.step(40)
+        own.append([eng.read_field(c) for c in range(6)])
+        eng.close()
+    engs = [scn.make(Engine, precision="fp64") for scn in scns]
+    b = Batch(engs)
+    b.set_tile("register_rows", 4)
+    b.step(40)
+    threads, _ = b.launch_info()
+    assert threads == 256
+    for i, eng in enumerate(engs):
+        for c in range(6):
+            assert np.array_equal(eng.read_field(c), own[i][c]), (i, c)
+    b.close()
+    for eng in engs:
+        eng.close()
