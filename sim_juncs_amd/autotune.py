@@ -24,17 +24,22 @@ XCHUNKS = (4, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64)
 
 
 def candidates(shape, xchunks=XCHUNKS):
-    """(form, rows, xchunk) triples worth timing for a grid shape."""
+    """(form, rows, xchunk, zseg) tuples worth timing for a grid shape; zseg 0 is the engine's
+    default z passes, other values a z pass length of one-row register tiles (lines of 512 cells
+    or more: 256-cell passes and one pass over the line)."""
     nx, ny, nz = shape
     out = []
     for rows in REGISTER_ROWS:
         if rows == 1 or rows <= ny:
-            out += [("register_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
+            out += [("register_rows", rows, xc, 0) for xc in xchunks if xc <= nx or xc == xchunks[0]]
+    if 512 <= nz <= 1024:
+        for zseg in (256, -(-nz // 64) * 64):
+            out += [("register_rows", 1, xc, zseg) for xc in xchunks if xc <= nx or xc == xchunks[0]]
     max_rows = 1024 // nz - 1
     for rows in sorted({r for r in (1, 2, 3, 4, 5, 7, 11, 15) if r <= max_rows} | ({max_rows} if max_rows >= 1 else set())):
         if rows > ny and rows != 1:
             continue
-        out += [("thread_rows", rows, xc) for xc in xchunks if xc <= nx or xc == xchunks[0]]
+        out += [("thread_rows", rows, xc, 0) for xc in xchunks if xc <= nx or xc == xchunks[0]]
     return out
 
 
@@ -67,20 +72,22 @@ def tune(eng, steps: int = 20, xchunks=XCHUNKS, verbose: bool = True):
     """Time every candidate tile on `eng`; returns (best dict, all results)."""
     results = []
     eng.step(2)
-    for form, rows, xc in candidates(eng.shape, xchunks):
+    for form, rows, xc, zseg in candidates(eng.shape, xchunks):
         try:
             eng.set_tile(form, rows, xc)
+            eng.set_zsegment(zseg)
             eng.timed_steps(2)
             _, ms = eng.timed_steps(steps)
         except Exception as exc:  # tile rejected (e.g. LDS budget)
             if verbose:
-                print(f"[tune] {form} rows={rows} xc={xc}: {exc}", file=sys.stderr, flush=True)
+                print(f"[tune] {form} rows={rows} xc={xc} zseg={zseg}: {exc}", file=sys.stderr, flush=True)
             continue
-        results.append(dict(form=form, rows=rows, xchunk=xc, ms=ms))
+        results.append(dict(form=form, rows=rows, xchunk=xc, zseg=zseg, ms=ms))
         if verbose:
-            print(f"[tune] {form} rows={rows} xc={xc}: {ms:.4f} ms", file=sys.stderr, flush=True)
+            print(f"[tune] {form} rows={rows} xc={xc} zseg={zseg}: {ms:.4f} ms", file=sys.stderr, flush=True)
     best = min(results, key=lambda r: r["ms"])
     eng.set_tile(best["form"], best["rows"], best["xchunk"])
+    eng.set_zsegment(best["zseg"])
     return best, results
 
 
@@ -88,6 +95,8 @@ def store(shape, precision, best, tag=""):
     c = load_cache()
     c[key(shape, precision, tag)] = dict(form=best["form"], rows=best["rows"], xchunk=best["xchunk"],
                                          ms=best["ms"])
+    if best.get("zseg"):
+        c[key(shape, precision, tag)]["zseg"] = best["zseg"]
     with open(CACHE, "w") as f:
         json.dump(c, f, indent=1, sort_keys=True)
 
