@@ -273,11 +273,13 @@ int sjf_set_zsegment(sjf_engine* e, int cells);
 int sjf_adjoint_run(sjf_engine* e, long nsteps, int checkpoint_every, const int gap[3], double* J,
                     double* grad, size_t cap);
 /* Tile of the fused reverse step of sjf_adjoint_run: SJF_TILE_REGISTER_ROWS (rows = 1: one y row
- * per workgroup, the default) or SJF_TILE_THREAD_ROWS (`rows` owned y rows of nz threads each
- * plus one halo row per workgroup, the y and z neighbours exchanged through LDS; needs
- * (rows + 1) * nz <= 1024 and the exchange planes within 64 KiB of LDS), marching over `xchunk`
- * x planes (0: enough chunks for about 16384 workgroups).  The gradient is the same either way
- * (the same arithmetic per cell).  New (tuning). */
+ * per workgroup) or SJF_TILE_THREAD_ROWS (`rows` owned y rows of nz threads each plus one halo
+ * row per workgroup, the y and z neighbours exchanged through LDS; needs (rows + 1) * nz <= 1024
+ * and the exchange planes within 64 KiB of LDS), marching over `xchunk` x planes (0: enough
+ * chunks for about 16384 workgroups).  The default is one row per workgroup.  The gradient is
+ * the same to rounding either way (the same arithmetic per cell).  A batched adjoint run runs one row per workgroup unless its first engine's tile was set
+ * to the row-of-threads form: then every engine runs that many rows (at most its ny) over the
+ * first engine's x-chunk (6 planes where it gave none).  New (tuning). */
 int sjf_set_adjoint_tile(sjf_engine* e, int form, int rows, int xchunk);
 /* The fused reverse-step kernel of the last sjf_adjoint_run: mean duration in ms (HIP events on
  * the engine's stream around the first min(nsteps, 64) reverse launches), threads per workgroup

@@ -122,6 +122,7 @@ struct sjf_engine {
     // its tile (sjf_set_adjoint_tile): form, owned rows of a row-of-threads tile, x-chunk (0: default)
     int adj_form = SJF_TILE_REGISTER_ROWS;
     int adj_rows = 0, adj_xc = 0;
+    bool adj_set = false;  // set by sjf_set_adjoint_tile (else the default of adj_tile_of)
 
     LaunchSet* own = nullptr;
     sjf_batch* batch = nullptr;
@@ -1282,6 +1283,22 @@ int adj_xchunk(const sjf_engine* e, long target = 16384, long rows = 0) {
 // workgroups the reverse launch of a batch aims at (x-chunks common to all engines)
 constexpr long kBatchAdjGroups = 32768;
 
+// Tile of the reverse step: the one sjf_set_adjoint_tile gave, else one row per workgroup (the
+// form a batch runs by default, so an engine alone and in a batch give the same bits).  Measured
+// on MI355X, one 192x192x96 FP32 engine: 0.139 ms per reverse step with row-of-threads tiles of
+// 9 rows and 6 planes, 0.155 ms one row per workgroup; sixteen such engines in one batch: 1.78 ms
+// per reverse launch with the 9-row tiles, 1.65 ms one row per workgroup; 256x256x128: 0.268 ms
+// for the best row-of-threads tile against 0.266 ms one row per workgroup.
+struct AdjTile {
+    bool rows;
+    int nrows, xc;  // xc 0: the default chunks
+};
+AdjTile adj_tile_of(const sjf_engine* e) {
+    if (e->adj_set && e->adj_form == SJF_TILE_THREAD_ROWS)
+        return {true, e->adj_rows, e->adj_xc > 0 ? std::min(e->adj_xc, e->g.nx) : 0};
+    return {false, 1, e->adj_set && e->adj_xc > 0 ? std::min(e->adj_xc, e->g.nx) : 0};
+}
+
 template <typename T, int NP>
 void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
     if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 256>), dim3(tiles), dim3(block), lds, e->stream, g);
@@ -1406,11 +1423,12 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     // ---- backward pass
     sjf::AdjArgs<T> g;
     std::memset(&g, 0, sizeof(g));
-    // register form: one y row per workgroup; row-of-threads form: adj_rows owned rows
-    const bool rows = e->adj_form == SJF_TILE_THREAD_ROWS;
-    const int arows = rows ? e->adj_rows : 1;
-    int axc = e->adj_xc > 0 ? std::min(e->adj_xc, e->g.nx) : adj_xchunk(e);
-    if (rows && e->adj_xc == 0) {  // about 16384 workgroups over the tile rows
+    // register form: one y row per workgroup; row-of-threads form: arows owned rows
+    const AdjTile at = adj_tile_of(e);
+    const bool rows = at.rows;
+    const int arows = at.nrows;
+    int axc = at.xc > 0 ? at.xc : adj_xchunk(e);
+    if (rows && at.xc == 0) {  // about 16384 workgroups over the tile rows
         const long trows = (e->g.ny + arows - 1) / arows;
         const long nxc = std::min<long>(std::max(1L, (16384 + trows - 1) / trows), e->g.nx);
         axc = (int)((e->g.nx + nxc - 1) / nxc);
@@ -1522,7 +1540,17 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
 
 template <typename T, int NP>
 void launch_adj_batch_np(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* first, int n, int block,
-                         unsigned tiles, size_t lds) {
+                         unsigned tiles, size_t lds, bool rows) {
+    if (rows) {
+        if (block <= 256)
+            hipLaunchKernelGGL((sjf::k_adj_rows_batch<T, NP, 256>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+        else if (block <= 512)
+            hipLaunchKernelGGL((sjf::k_adj_rows_batch<T, NP, 512>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+        else
+            hipLaunchKernelGGL((sjf::k_adj_rows_batch<T, NP, 1024>), dim3(tiles), dim3(block), lds, s->stream, args,
+                               first, n);
+        return;
+    }
     if (block <= 256)
         hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 256>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
     else if (block <= 512)
@@ -1583,6 +1611,12 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
     std::vector<int> first;
     int t = 0, nmat = 1;
     long maxv = 0, rows = 0;
+    // one row per workgroup with x-chunks common to all engines, or, when the first engine's tile
+    // was set to the row-of-threads form, its rows (at most each engine's ny) and its x-chunk
+    const AdjTile bt = adj_tile_of(e0);
+    const bool brows = bt.rows;
+    const int bxc = bt.xc > 0 ? bt.xc : 6;
+    int bblock = 64;
     for (const sjf_engine* e : s->eng) rows += e->g.ny;
     for (int i = 0; i < n; ++i) {
         sjf_engine* e = s->eng[i];
@@ -1625,15 +1659,22 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
         d.dt = e->dt;
         sjf::AdjArgs<T>& g = args[i];
         std::memset(&g, 0, sizeof(g));
-        g.f = step_args<T>(e, 1, adj_xchunk(e, kBatchAdjGroups, rows));  // one y row per workgroup
-        g.f.nty = e->g.ny;
+        // the first engine's tile form for all (they share nz and the precision)
+        if (brows) {
+            const int r = std::min(bt.nrows, e->g.ny);
+            g.f = step_args<T>(e, r, std::min(bxc, e->g.nx));
+            bblock = std::max(bblock, (((r + 1) * e->g.nz + 63) / 64) * 64);
+        } else {
+            g.f = step_args<T>(e, 1, adj_xchunk(e, kBatchAdjGroups, rows));  // one y row per workgroup
+            g.f.nty = e->g.ny;
+        }
         g.A[0] = d.A[0];
         g.A[1] = d.A[1];
         g.Aq = (T*)Aq;
         g.Ap = (T*)Ap;
         for (int q = 0; q < 3; ++q) g.Apsi[q] = (T*)Apsi[q];
         first.push_back(t);
-        t += (int)((long)e->g.ny * ((e->g.nx + g.f.xc - 1) / g.f.xc));
+        t += (int)((long)g.f.nty * ((e->g.nx + g.f.xc - 1) / g.f.xc));
         nmat = std::max(nmat, e->nmat);
         maxv = std::max(maxv, nd + 3);
     }
@@ -1661,8 +1702,9 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
         return bail(rc);
     const auto* tabp = (const sjf::AdjBatchDev<T>*)d_tab;
     const sjf::AdjArgs<T>* argd[2] = {(const sjf::AdjArgs<T>*)d_args0, (const sjf::AdjArgs<T>*)d_args1};
-    const int block = ((e0->g.nz + 63) / 64) * 64;
-    const size_t lds = sizeof(T) * 12 * block + sizeof(sjf::MatLut<T>) * nmat;
+    const int block = brows ? bblock : ((e0->g.nz + 63) / 64) * 64;
+    const size_t lds = (brows ? sizeof(T) * (8 * (size_t)block + 2 * (size_t)e0->g.nz) : sizeof(T) * 12 * (size_t)block) +
+                       sizeof(sjf::MatLut<T>) * nmat;
     if (lds > 65536) return bail(fail(e0, SJF_E_UNSUPPORTED, "adjoint tile needs more than 64 KiB of LDS"));
     const int npv = np_variant_of(e0->maxp, e0->uniform);
     const dim3 grid_v((unsigned)((maxv + 255) / 256), (unsigned)n);
@@ -1721,10 +1763,10 @@ int batch_adjoint_run_t(LaunchSet* s, long nsteps, long K, const long* gaps, con
         if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed], s->stream));
         const sjf::AdjArgs<T>* ad = argd[q & 1];
         switch (npv) {
-            case 0: launch_adj_batch_np<T, 0>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
-            case 1: launch_adj_batch_np<T, 1>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
-            case 3: launch_adj_batch_np<T, 3>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
-            default: launch_adj_batch_np<T, 4>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds); break;
+            case 0: launch_adj_batch_np<T, 0>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds, brows); break;
+            case 1: launch_adj_batch_np<T, 1>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds, brows); break;
+            case 3: launch_adj_batch_np<T, 3>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds, brows); break;
+            default: launch_adj_batch_np<T, 4>(s, ad, (const int*)d_first, n, block, (unsigned)t, lds, brows); break;
         }
         SETCHK(s, hipGetLastError());
         if (timed) SETCHK(s, hipEventRecord(evs[2 * ntimed++ + 1], s->stream));
@@ -1833,6 +1875,7 @@ extern "C" int sjf_set_adjoint_tile(sjf_engine* e, int form, int rows, int xchun
     e->adj_form = form;
     e->adj_rows = rows;
     e->adj_xc = xchunk;
+    e->adj_set = true;
     return SJF_OK;
 }
 

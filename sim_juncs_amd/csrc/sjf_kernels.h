@@ -2372,6 +2372,15 @@ k_adj_step_batch(const AdjArgs<T>* __restrict__ args, const int* __restrict__ ti
     adj_tile<T, NP>(args[b], t);
 }
 
+// The same, row-of-threads form
+template <typename T, int NP, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_adj_rows_batch(const AdjArgs<T>* __restrict__ args,
+                                                         const int* __restrict__ tile_first, int ninst) {
+    int t;
+    const int b = tile_instance(tile_first, ninst, t);
+    adj_rows<T, NP>(args[b], t);
+}
+
 // ---- plain streaming copy (achievable-bandwidth probe and counter calibration) -----------
 // one float4 per thread over the whole array (the fastest plain copy measured on MI355X)
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst,

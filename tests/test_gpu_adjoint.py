@@ -312,7 +312,7 @@ BATCH_MEMBERS = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["register_rows", "default", "checkpoints"])
+@pytest.mark.parametrize("mode", ["register_rows", "default", "checkpoints", "reverse_rows"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     """sjf_batch_adjoint_run over engines of different shapes, design boxes, objective cells and
@@ -322,7 +322,8 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
     engines, and the forms round differently in the last bit, so each mode runs the same form
     alone and in the batch: register rows with each engine's x-chunk, or the default
     row-of-threads form, also with the batch checkpointing every 13 steps and recomputing (the
-    engines alone record).  The reverse launch takes x-chunks common to all engines."""
+    engines alone record).  The reverse launch takes x-chunks common to all engines, or, in the
+    last mode, the row-of-threads reverse tile of 4 rows and 6-plane chunks set on every engine."""
     from sim_juncs_amd.engine import Batch
 
     steps = 60
@@ -333,6 +334,8 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
         eng = scn.make(Engine)
         if mode == "register_rows":
             eng.set_tile("register_rows", 1, m[5])
+        if mode == "reverse_rows":
+            eng.set_adjoint_tile("thread_rows", 4, 6)
         return eng
 
     own = []
@@ -353,11 +356,14 @@ def test_batched_adjoint_equals_each_engines_own_run(precision, mode):
                              13 if mode == "checkpoints" else 0)
     ms, threads, groups = b.adjoint_kernel_info()
     b.close()
-    assert 0 < ms < 100 and threads == 64
+    assert 0 < ms < 100 and threads == (128 if mode == "reverse_rows" else 64)
     tiles = 0
     rows = sum(scn.shape[1] for scn, _, _ in cases)
     for (scn, _, _), m in zip(cases, BATCH_MEMBERS):
         nx, ny, _ = scn.shape
+        if mode == "reverse_rows":  # 4-row tiles of (4 + 1) x 20 threads
+            tiles += -(-ny // 4) * -(-nx // 6)
+            continue
         # x-chunks common to all engines, for about 32768 workgroups in all
         nxc = min(-(-32768 // rows), nx)
         xc = -(-nx // nxc)
