@@ -1278,9 +1278,9 @@ __device__ __forceinline__ void dft_cell(const DftDev<T>* __restrict__ regs, int
 
 // One thread per (component, cell, frequency): blockIdx.y is the frequency index, and the
 // threads of a region run over its components' cells one component after another.  The phase
-// factors depend only on the step and the frequency, so one thread per block evaluates them
-// (FP64 sincos) and shares them through LDS; a block's threads all lie in one region when
-// regions start on block boundaries, otherwise each thread evaluates its own.
+// factors depend only on the step and the frequency; each thread evaluates them itself (two
+// FP64 sincos), which measured faster than one thread per block sharing them through LDS
+// behind a barrier.
 template <typename T>
 __device__ __forceinline__ void dft_block(const DftDev<T>* __restrict__ regs, const int nreg, const long total,
                                           const int bx, const int f) {
