@@ -253,6 +253,15 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
 // Row-of-threads form: (rows + 1) thread rows of nz threads per workgroup, 32-bit offsets.
 bool thread_rows_fit(const sjf_engine* e, int rows) { return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024; }
 
+// Field bytes one step of these engines moves (E and H read and written) above which the one-row
+// step kernels store with the non-temporal hint: more than the 256 MB Infinity Cache, so a step
+// cannot find the previous step's writes there.  Measured on MI355X (FP32): 1024x768x512 7.17 ->
+// 6.86 ms per step, 512x512x256 1.230 -> 1.205 ms, the 256x256x128 reverse step 0.282 -> 0.267
+// ms; the 96^3 FP64 box (85 MB of fields) 0.046 -> 0.059 ms with the hint, so it keeps plain
+// stores.
+constexpr double kStreamBytes = 256e6;
+double field_bytes(const sjf_engine* e) { return 12.0 * e->prec * (double)e->N; }
+
 // z cells per pass of a register-rows workgroup with `rows` rows per thread: the set value, else
 // 256-cell passes for one row per thread on a non-periodic z line that is a multiple of 256
 // cells above 256 or longer than 1024, else the whole line.  Measured on MI355X, 1024x768x512
@@ -343,6 +352,7 @@ struct LaunchSet {
     int form = SJF_TILE_REGISTER_ROWS;
     int xc = 0;  // x-chunk of every instance of a batch (0: each engine's own)
     bool zpass = false;  // register form with some z line stepped in passes (k_step_z)
+    bool nts = false;    // one-row tiles store with the non-temporal hint (field_bytes > kStreamBytes)
     size_t lds = 0;
     std::vector<long> versions;
     bool built = false;
@@ -455,6 +465,9 @@ int build_set_t(LaunchSet* s) {
             return fail(e0, SJF_E_UNSUPPORTED, "a z line longer than 1024 cells needs one row per thread (z passes)");
     }
     s->zpass = zh > 0;
+    double fb = 0;
+    for (sjf_engine* e : s->eng) fb += field_bytes(e);
+    s->nts = fb > kStreamBytes;
     first.push_back(t);
     s->ntiles = t;
     s->block = block;
@@ -640,12 +653,20 @@ int launch_step_np(LaunchSet* s) {
         return SJF_OK;
     }
     if (s->zpass) {
-        hipLaunchKernelGGL((sjf::k_step_z<T, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+        if (s->nts)
+            hipLaunchKernelGGL((sjf::k_step_z<T, NP, MAXT, true>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+        else
+            hipLaunchKernelGGL((sjf::k_step_z<T, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
         SETCHK(s, hipGetLastError());
         return SJF_OK;
     }
     switch (s->ty) {
-        case 1: hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
+        case 1:
+            if (s->nts)
+                hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT, true>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+            else
+                hipLaunchKernelGGL((sjf::k_step<T, 1, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n);
+            break;
         case 2: hipLaunchKernelGGL((sjf::k_step<T, 2, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
         case 4: hipLaunchKernelGGL((sjf::k_step<T, 4, NP, MAXT>), grid, block, s->lds, s->stream, args, s->d_tile_first, n); break;
         default: return fail(s->eng[0], SJF_E_ARG, "tile rows must be 1, 2 or 4");
@@ -1299,11 +1320,18 @@ AdjTile adj_tile_of(const sjf_engine* e) {
     return {false, 1, e->adj_set && e->adj_xc > 0 ? std::min(e->adj_xc, e->g.nx) : 0};
 }
 
+template <typename T, int NP, bool NTS>
+void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
+    if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 256, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else if (block <= 512) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 512, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
+    else hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 1024, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
+}
+
 template <typename T, int NP>
 void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
-    if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 256>), dim3(tiles), dim3(block), lds, e->stream, g);
-    else if (block <= 512) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 512>), dim3(tiles), dim3(block), lds, e->stream, g);
-    else hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 1024>), dim3(tiles), dim3(block), lds, e->stream, g);
+    // the reverse step streams the adjoint fields as the forward step streams E and H
+    if (field_bytes(e) > kStreamBytes) launch_adj_np<T, NP, true>(e, g, block, tiles, lds);
+    else launch_adj_np<T, NP, false>(e, g, block, tiles, lds);
 }
 
 template <typename T, int NP>
@@ -1538,6 +1566,18 @@ int adjoint_run_t(sjf_engine* e, long nsteps, long K, long gap, double* J_out, d
     return SJF_OK;
 }
 
+template <typename T, int NP, bool NTS>
+void launch_adj_batch_step(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* first, int n, int block,
+                           unsigned tiles, size_t lds) {
+    if (block <= 256)
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 256, NTS>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+    else if (block <= 512)
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 512, NTS>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
+    else
+        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 1024, NTS>), dim3(tiles), dim3(block), lds, s->stream, args,
+                           first, n);
+}
+
 template <typename T, int NP>
 void launch_adj_batch_np(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* first, int n, int block,
                          unsigned tiles, size_t lds, bool rows) {
@@ -1551,13 +1591,8 @@ void launch_adj_batch_np(LaunchSet* s, const sjf::AdjArgs<T>* args, const int* f
                                first, n);
         return;
     }
-    if (block <= 256)
-        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 256>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
-    else if (block <= 512)
-        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 512>), dim3(tiles), dim3(block), lds, s->stream, args, first, n);
-    else
-        hipLaunchKernelGGL((sjf::k_adj_step_batch<T, NP, 1024>), dim3(tiles), dim3(block), lds, s->stream, args, first,
-                           n);
+    if (s->nts) launch_adj_batch_step<T, NP, true>(s, args, first, n, block, tiles, lds);
+    else launch_adj_batch_step<T, NP, false>(s, args, first, n, block, tiles, lds);
 }
 
 // Adjoint gradients of every engine of a launch set in one pass: the forward steps are the set's

@@ -145,11 +145,17 @@ template <>
 __device__ __forceinline__ uint32_t bld<uint32_t>(Rsrc r, uint32_t v, uint32_t s) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0);
 }
+// NT: the non-temporal cache policy (a hint that the line will not be read again soon).  A step
+// over more data than the caches hold cannot find its writes there at the next step, and the
+// hint keeps the written lines from displacing the y-halo rows that neighbouring workgroups
+// read again.
+template <bool NT = false>
 __device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, float x) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, NT ? 2 : 0);
 }
+template <bool NT = false>
 __device__ __forceinline__ void bst(Rsrc r, uint32_t v, uint32_t s, double x) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, NT ? 2 : 0);
 }
 // true in every lane of the wavefront when p holds in at least one active lane (wave-uniform)
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
@@ -171,7 +177,7 @@ __host__ __device__ __forceinline__ long field_index(long q, int c, long nyz) { 
 // z PML layer, a pole slot or the design region carry the out-of-range offset instead of
 // branching; whole-wavefront conditions (row, plane, x/y PML, any lane in a z layer / using a
 // slot / in the design region) are uniform branches.
-template <typename T, int TY, int NP, bool ZP>
+template <typename T, int TY, int NP, bool ZP, bool NTS>
 __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
     constexpr uint32_t TS = sizeof(T);
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -418,8 +424,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                     dEx_y = cpml(dEx_y, b, c, pH[r][1][1]);
                     if (wr) {
                         const uint32_t so = (uint32_t)((pi * Ly + sy[r]) * nz) * TS;
-                        bst(rpy, oK, hw * SB1 + so, pH[r][1][0]);
-                        bst(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
+                        bst<NTS>(rpy, oK, hw * SB1 + so, pH[r][1][0]);
+                        bst<NTS>(rpy, oK, (hw + 1) * SB1 + so, pH[r][1][1]);
                     }
                 }
                 if (ok && zany) {  // lanes outside the layers: b = a = 0 leaves the term unchanged
@@ -427,16 +433,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                     dEx_z = cpml(dEx_z, bHz, aHz, pH[r][2][1]);
                     if (wr) {
                         const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                        bst(rpz, oZ, hw * SB2 + so, pH[r][2][0]);
-                        bst(rpz, oZ, (hw + 1) * SB2 + so, pH[r][2][1]);
+                        bst<NTS>(rpz, oZ, hw * SB2 + so, pH[r][2][0]);
+                        bst<NTS>(rpz, oZ, (hw + 1) * SB2 + so, pH[r][2][1]);
                     }
                 }
                 if (ok && xpml) {
                     dEz_x = cpml(dEz_x, bHx, aHx, pH[r][0][0]);
                     dEy_x = cpml(dEy_x, bHx, aHx, pH[r][0][1]);
                     if (wr) {
-                        bst(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
-                        bst(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
+                        bst<NTS>(rpx, oR[r], hw * SB0 + sxo, pH[r][0][0]);
+                        bst<NTS>(rpx, oR[r], (hw + 1) * SB0 + sxo, pH[r][0][1]);
                     }
                 }
                 // outside a metallic edge H is identically zero (ghost row of the halo)
@@ -444,9 +450,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                 h[r][1] = ok ? h0[r][1] - cdt * (dEx_z - dEz_x) : T(0);
                 h[r][2] = ok ? h0[r][2] - cdt * (dEy_x - dEx_y) : T(0);
                 if (wr) {
-                    bst(rW, oR[r], 3 * PB, h[r][0]);
-                    bst(rW, oR[r], 4 * PB, h[r][1]);
-                    bst(rW, oR[r], 5 * PB, h[r][2]);
+                    bst<NTS>(rW, oR[r], 3 * PB, h[r][0]);
+                    bst<NTS>(rW, oR[r], 4 * PB, h[r][1]);
+                    bst<NTS>(rW, oR[r], 5 * PB, h[r][2]);
                 }
             }
 
@@ -499,27 +505,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                         dHz_y = cpml(dHz_y, b, c, pE[r][1][0]);
                         dHx_y = cpml(dHx_y, b, c, pE[r][1][1]);
                         const uint32_t so = (uint32_t)((pi * Ly + sy[r + 1]) * nz) * TS;
-                        bst(rpy, oK, so, pE[r][1][0]);
-                        bst(rpy, oK, SB1 + so, pE[r][1][1]);
+                        bst<NTS>(rpy, oK, so, pE[r][1][0]);
+                        bst<NTS>(rpy, oK, SB1 + so, pE[r][1][1]);
                     }
                     if (zany) {
                         dHy_z = cpml(dHy_z, bEz, aEz, pE[r][2][0]);
                         dHx_z = cpml(dHx_z, bEz, aEz, pE[r][2][1]);
                         const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                        bst(rpz, oZ, so, pE[r][2][0]);
-                        bst(rpz, oZ, SB2 + so, pE[r][2][1]);
+                        bst<NTS>(rpz, oZ, so, pE[r][2][0]);
+                        bst<NTS>(rpz, oZ, SB2 + so, pE[r][2][1]);
                     }
                     if (xpml) {
                         dHz_x = cpml(dHz_x, bEx, aEx, pE[r][0][0]);
                         dHy_x = cpml(dHy_x, bEx, aEx, pE[r][0][1]);
-                        bst(rpx, o, sxo, pE[r][0][0]);
-                        bst(rpx, o, SB0 + sxo, pE[r][0][1]);
+                        bst<NTS>(rpx, o, sxo, pE[r][0][0]);
+                        bst<NTS>(rpx, o, SB0 + sxo, pE[r][0][1]);
                     }
                     const T curl[3] = {dHz_y - dHy_z, dHx_z - dHz_x, dHy_x - dHx_y};
                     if (uniform) {
                         const T inv = slut[0].inv_eps;
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
+                        for (int c = 0; c < 3; ++c) bst<NTS>(rW, o, c * PB, ec[r + 1][c] + inv * (cdt * curl[c]));
                     } else {
                         int mm[3];
                         bool des[3];
@@ -566,20 +572,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
                                     qn = vp ? L.alpha[s] * (cur - old) + L.beta[s] * cur + L.c3[s] * ec[r + 1][c]
                                             : L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c];
                                     qn = vq ? qn : T(0);
-                                    bst(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
-                                    bst(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
+                                    bst<NTS>(rP, (vp && !podd) ? o : kOOB, (s * 3 + c) * PB, cur + qn);
+                                    bst<NTS>(rQ, (vq && !(vp && !podd)) ? o : kOOB, (s * 3 + c) * PB, vp ? cur + qn : qn);
                                 } else {
                                     // Pv is zero unless vp (out-of-range load), so beta * Pv adds nothing then
                                     qn = L.alpha[s] * Qv[r][c][s] + L.c3[s] * ec[r + 1][c] + L.beta[s] * Pv[r][c][s];
                                     qn = vq ? qn : T(0);
-                                    bst(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
-                                    bst(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
+                                    bst<NTS>(rP, vp ? o : kOOB, (s * 3 + c) * PB, Pv[r][c][s] + qn);
+                                    bst<NTS>(rQ, vq ? o : kOOB, (s * 3 + c) * PB, qn);
                                 }
                                 sq[c] += qn;
                             }
                         }
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) bst(rW, o, c * PB, ec[r + 1][c] + inv[c] * (cdt * curl[c] - sq[c]));
+                        for (int c = 0; c < 3; ++c) bst<NTS>(rW, o, c * PB, ec[r + 1][c] + inv[c] * (cdt * curl[c] - sq[c]));
                     }
                 }
             }
@@ -1012,21 +1018,22 @@ __device__ __forceinline__ int tile_instance(const int* __restrict__ tile_first,
 template <typename T, int TY, int NP>
 constexpr int step_waves() { return (sizeof(T) == 4 && TY == 1 && NP <= 3) ? 5 : (sizeof(T) == 8 && TY == 1) ? 3 : 1; }
 
-template <typename T, int TY, int NP, int MAXT>
+// NTS: stores with the non-temporal hint (launch sets that stream more than the caches hold).
+template <typename T, int TY, int NP, int MAXT, bool NTS = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_waves<T, TY, NP>())))
 k_step(const StepArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
     int t;
     const int b = tile_instance(tile_first, ninst, t);
-    step_tile<T, TY, NP, false>(args[b], t);
+    step_tile<T, TY, NP, false, NTS>(args[b], t);
 }
 
 // The same with the z line in passes of StepArgs::zseg cells (one row per thread)
-template <typename T, int NP, int MAXT>
+template <typename T, int NP, int MAXT, bool NTS = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(step_waves<T, 1, NP>())))
 k_step_z(const StepArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
     int t;
     const int b = tile_instance(tile_first, ninst, t);
-    step_tile<T, 1, NP, true>(args[b], t);
+    step_tile<T, 1, NP, true, NTS>(args[b], t);
 }
 
 // Fused step over all instances of a launch set (row-of-threads form).
@@ -1439,7 +1446,7 @@ struct AdjArgs {
 // and predication as in step_tile: per-plane descriptors, lane offsets fixed for the march, the
 // out-of-range offset for ghost rows and for lanes past the z line, outside a z PML layer, a
 // pole slot or the design region; whole-wavefront conditions are uniform branches.
-template <typename T, int NP>
+template <typename T, int NP, bool NTS>
 __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
     constexpr uint32_t TS = sizeof(T);
     const StepArgs<T>& a = g.f;
@@ -1688,23 +1695,23 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
         // own row: Hbar at the old time and the new H auxiliary bars
         if (wr) {
             const int j = gj[1];
-            bst(rB, oR[1], 3 * PB, tot[1][0]);
-            bst(rB, oR[1], 4 * PB, tot[1][1]);
-            bst(rB, oR[1], 5 * PB, tot[1][2]);
+            bst<NTS>(rB, oR[1], 3 * PB, tot[1][0]);
+            bst<NTS>(rB, oR[1], 4 * PB, tot[1][1]);
+            bst<NTS>(rB, oR[1], 5 * PB, tot[1][2]);
             if (ypml[1]) {  // (Hx, y, +, slot 0), (Hz, y, -, slot 1)
                 const T b = cy[2 * ny + j];
                 const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                bst(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + phy[0]));
-                bst(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + phy[1]));
+                bst<NTS>(rpy, oK, hw * SB1 + so, b * (-cdt * tot[1][0] + phy[0]));
+                bst<NTS>(rpy, oK, (hw + 1) * SB1 + so, b * (cdt * tot[1][2] + phy[1]));
             }
             if (zany) {  // (Hx, z, -, slot 0), (Hy, z, +, slot 1)
                 const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                bst(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + phz[0]));
-                bst(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + phz[1]));
+                bst<NTS>(rpz, oZ, hw * SB2 + so, bHz * (cdt * tot[1][0] + phz[0]));
+                bst<NTS>(rpz, oZ, (hw + 1) * SB2 + so, bHz * (-cdt * tot[1][1] + phz[1]));
             }
             if (xpml) {  // (Hy, x, -, slot 0), (Hz, x, +, slot 1)
-                bst(rpx, oR[1], hw * SB0 + sxo, bHx * (cdt * tot[1][1] + phx[0]));
-                bst(rpx, oR[1], (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[1][2] + phx[1]));
+                bst<NTS>(rpx, oR[1], hw * SB0 + sxo, bHx * (cdt * tot[1][1] + phx[0]));
+                bst<NTS>(rpx, oR[1], (hw + 1) * SB0 + sxo, bHx * (-cdt * tot[1][2] + phx[1]));
             }
         }
 
@@ -1792,8 +1799,8 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                             const uint32_t so = (s * 3 + c) * PB;
                             const T pb = pv[s][c];
                             const T qb = vq ? qv[s][c] + pb - u0[c] : T(0);
-                            bst(rQ, vq ? o : kOOB, so, L.alpha[s] * qb);
-                            bst(rP, vp ? o : kOOB, so, pb + L.beta[s] * qb);
+                            bst<NTS>(rQ, vq ? o : kOOB, so, L.alpha[s] * qb);
+                            bst<NTS>(rP, vp ? o : kOOB, so, pb + L.beta[s] * qb);
                             out[c] += L.c3[s] * qb;
                         }
                     }
@@ -1802,22 +1809,22 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T>& g, const int t) {
                 if (ypml[1]) {  // (Ex, y, +, slot 0), (Ez, y, -, slot 1)
                     const T b = cy[j];
                     const uint32_t so = (uint32_t)((pi * Ly + sy[1]) * nz) * TS;
-                    bst(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + pey[0]));
-                    bst(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + pey[1]));
+                    bst<NTS>(rpy, oK, ew * SB1 + so, b * (cdt * u0[0] + pey[0]));
+                    bst<NTS>(rpy, oK, (ew + 1) * SB1 + so, b * (-cdt * u0[2] + pey[1]));
                 }
                 if (zany) {  // (Ex, z, -, slot 0), (Ey, z, +, slot 1)
                     const uint32_t so = (uint32_t)((pi * ny + j) * Lz) * TS;
-                    bst(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + pzE[1][0]));
-                    bst(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + pzE[1][1]));
+                    bst<NTS>(rpz, oZ, ew * SB2 + so, bEz * (-cdt * u0[0] + pzE[1][0]));
+                    bst<NTS>(rpz, oZ, (ew + 1) * SB2 + so, bEz * (cdt * u0[1] + pzE[1][1]));
                 }
                 if (xpml) {  // (Ey, x, -, slot 0), (Ez, x, +, slot 1)
-                    bst(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + pex[0]));
-                    bst(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + pex[1]));
+                    bst<NTS>(rpx, o, ew * SB0 + sxo, bEx * (-cdt * u0[1] + pex[0]));
+                    bst<NTS>(rpx, o, (ew + 1) * SB0 + sxo, bEx * (cdt * u0[2] + pex[1]));
                 }
                 // the state is Ubar = Ebar / eps
-                bst(rB, o, 0, inv[0] * out[0]);
-                bst(rB, o, PB, inv[1] * out[1]);
-                bst(rB, o, 2 * PB, inv[2] * out[2]);
+                bst<NTS>(rB, o, 0, inv[0] * out[0]);
+                bst<NTS>(rB, o, PB, inv[1] * out[1]);
+                bst<NTS>(rB, o, 2 * PB, inv[2] * out[2]);
             }
         } else {
             lds_barrier();  // keep the barrier count uniform across iterations
@@ -2196,12 +2203,12 @@ __global__ void __launch_bounds__(MAXT) k_adj_rows(const AdjArgs<T> g) {
 template <typename T>
 constexpr int adj_waves() { return sizeof(T) == 4 ? 4 : 1; }
 
-template <typename T, int NP, int MAXT>
+template <typename T, int NP, int MAXT, bool NTS = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(adj_waves<T>())))
 k_adj_step(const AdjArgs<T> g) {
     const int w = blockIdx.x, nw = gridDim.x;
     const int t = (nw % 8 == 0) ? ((w & 7) * (nw >> 3) + (w >> 3)) : w;
-    adj_tile<T, NP>(g, t);
+    adj_tile<T, NP, NTS>(g, t);
 }
 
 // record E of the design box and of the objective cell into slot `slot` of a ring buffer
@@ -2364,12 +2371,12 @@ __global__ void k_adj_inject_batch(const AdjBatchDev<T>* __restrict__ tab, int r
 }
 
 // Fused reverse step over every engine of a batch (tile t of engine b after the XCD-aware remap)
-template <typename T, int NP, int MAXT>
+template <typename T, int NP, int MAXT, bool NTS = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(adj_waves<T>())))
 k_adj_step_batch(const AdjArgs<T>* __restrict__ args, const int* __restrict__ tile_first, int ninst) {
     int t;
     const int b = tile_instance(tile_first, ninst, t);
-    adj_tile<T, NP>(args[b], t);
+    adj_tile<T, NP, NTS>(args[b], t);
 }
 
 // The same, row-of-threads form
