@@ -1321,7 +1321,7 @@ AdjTile adj_tile_of(const sjf_engine* e) {
 }
 
 template <typename T, int NP, bool NTS>
-void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
+void launch_adj_one_row(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
     if (block <= 256) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 256, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
     else if (block <= 512) hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 512, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
     else hipLaunchKernelGGL((sjf::k_adj_step<T, NP, 1024, NTS>), dim3(tiles), dim3(block), lds, e->stream, g);
@@ -1330,8 +1330,8 @@ void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned 
 template <typename T, int NP>
 void launch_adj_np(sjf_engine* e, const sjf::AdjArgs<T>& g, int block, unsigned tiles, size_t lds) {
     // the reverse step streams the adjoint fields as the forward step streams E and H
-    if (field_bytes(e) > kStreamBytes) launch_adj_np<T, NP, true>(e, g, block, tiles, lds);
-    else launch_adj_np<T, NP, false>(e, g, block, tiles, lds);
+    if (field_bytes(e) > kStreamBytes) launch_adj_one_row<T, NP, true>(e, g, block, tiles, lds);
+    else launch_adj_one_row<T, NP, false>(e, g, block, tiles, lds);
 }
 
 template <typename T, int NP>

@@ -217,7 +217,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<T>& a, const int t) {
         const int k = pass * Z + kl;
         const bool kin = kl < Z && k < nz;
         const int kp1 = wrap_or_ghost(k + 1, nz, a.pz);
-        const int km1 = wrap_or_ghost(k - 1, nz, a.pz);
         const uint32_t PB = (uint32_t)ny * (uint32_t)nz * TS;  // bytes of one component plane
         const long FPB = 6L * PB;                             // bytes of one x plane of a field set
         const long QPB = (long)a.nqp * PB;                    // and of the polarisation
