@@ -250,7 +250,7 @@ int sjf_launch_info(const sjf_engine* e, int* form, int* rows, int* xchunk, int*
 int sjf_set_tile(sjf_engine* e, int form, int rows, int xchunk);
 /* REGISTER_ROWS tile: a workgroup covers its z line in passes of `cells` cells (a multiple of
  * 64, at most 1024), the last lane of a pass handing its H to the first lane of the next through
- * LDS.  0 = the default: 256-cell passes with one row per thread on a non-periodic z line that is
+ * LDS.  0 = the default: 128-cell passes with one row per thread on a non-periodic z line that is
  * a multiple of 256 cells above 256 or longer than 1024, else one pass over the whole line.
  * Passes need one row per thread and a non-periodic z axis.  Results are the same for every
  * segment length.  New (tuning, and z lines longer than one workgroup). */

@@ -26,14 +26,14 @@ XCHUNKS = (4, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64)
 def candidates(shape, xchunks=XCHUNKS):
     """(form, rows, xchunk, zseg) tuples worth timing for a grid shape; zseg 0 is the engine's
     default z passes, other values a z pass length of one-row register tiles (lines of 512 cells
-    or more: 256-cell passes and one pass over the line)."""
+    or more: 128- and 256-cell passes and one pass over the line)."""
     nx, ny, nz = shape
     out = []
     for rows in REGISTER_ROWS:
         if rows == 1 or rows <= ny:
             out += [("register_rows", rows, xc, 0) for xc in xchunks if xc <= nx or xc == xchunks[0]]
     if 512 <= nz <= 1024:
-        for zseg in (256, -(-nz // 64) * 64):
+        for zseg in (128, 256, -(-nz // 64) * 64):
             out += [("register_rows", 1, xc, zseg) for xc in xchunks if xc <= nx or xc == xchunks[0]]
     max_rows = 1024 // nz - 1
     for rows in sorted({r for r in (1, 2, 3, 4, 5, 7, 11, 15) if r <= max_rows} | ({max_rows} if max_rows >= 1 else set())):

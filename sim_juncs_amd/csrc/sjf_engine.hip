@@ -263,15 +263,17 @@ constexpr double kStreamBytes = 256e6;
 double field_bytes(const sjf_engine* e) { return 12.0 * e->prec * (double)e->N; }
 
 // z cells per pass of a register-rows workgroup with `rows` rows per thread: the set value, else
-// 256-cell passes for one row per thread on a non-periodic z line that is a multiple of 256
+// 128-cell passes for one row per thread on a non-periodic z line that is a multiple of 256
 // cells above 256 or longer than 1024, else the whole line.  Measured on MI355X, 1024x768x512
 // FP32: one pass of 512 threads leaves two 8-wave workgroups per CU (4 waves per SIMD of the 5
 // the registers allow) and runs 8.07 ms per step; two passes of 256 threads (5 workgroups per
-// CU) run 7.31 ms.  On 512x512x256, 128-cell passes lose (1.370 against 1.275 ms).
+// CU) run 7.31 ms.  With the non-temporal stores, two boxes: 256-cell passes 6.45 and 6.87 ms,
+// 128-cell passes (ten 2-wave workgroups per CU) 6.20 and 6.69 ms, 64-cell passes 6.80 ms.
+// On 512x512x256 one pass wins: 128-cell passes 1.295 against 1.183 ms.
 int zseg_of(const sjf_engine* e, int rows) {
     const int nz = e->g.nz;
     if (e->zseg > 0) return std::min(e->zseg, nz);
-    if (rows == 1 && !e->g.periodic[2] && nz > 256 && (nz % 256 == 0 || nz > 1024)) return 256;
+    if (rows == 1 && !e->g.periodic[2] && nz > 256 && (nz % 256 == 0 || nz > 1024)) return 128;
     return nz;
 }
 

@@ -1010,7 +1010,7 @@ __device__ __forceinline__ int tile_instance(const int* __restrict__ tile_first,
 
 // Fused step over all instances of a launch set (rows-per-thread form).
 // Waves per SIMD the register allocator must leave room for: the FP32 one-row tile with up to
-// three pole slots fits 96 registers (5 waves; the two-pass variant keeps 12 bytes per lane in
+// three pole slots fits 96 registers (5 waves; the z-pass variant keeps 12 bytes per lane in
 // scratch); the FP64 one-row tile is held to 168 registers (3 waves; with three pole slots it
 // spills about 30 bytes per lane and still runs 3% faster than at 2 waves on the 96^3 box);
 // other variants are unconstrained.

@@ -386,7 +386,7 @@ def test_read_field_box_is_the_slice_of_the_whole_component(precision):
     ("fp64", 64, (12, 10, 150)),   # three passes, the last 22 cells
     ("fp64", 128, (10, 9, 200)),
     ("fp32", 256, (8, 8, 512)),
-    ("fp64", 0, (8, 7, 1100)),     # longer than a workgroup: default 256-cell passes
+    ("fp64", 0, (8, 7, 1100)),     # longer than a workgroup: default 128-cell passes
 ])
 def test_z_passes_match_one_pass_and_oracle(precision, zseg, shape):
     """A workgroup that covers its z line in passes (sjf_set_zsegment; the last lane of a pass
@@ -399,7 +399,7 @@ def test_z_passes_match_one_pass_and_oracle(precision, zseg, shape):
     eng.set_zsegment(zseg)
     eng.step(40)
     info = eng.launch_info()
-    assert info["threads"] == (zseg or 256)
+    assert info["threads"] == (zseg or 128)
     if shape[2] <= 1024:
         one = scn.make(Engine, precision=precision)
         one.set_tile("register_rows", 1, 4)
@@ -429,11 +429,11 @@ def test_z_pass_arguments_are_checked():
     with pytest.raises(EngineError):  # passes run one row per thread
         eng.step(1)
     eng.close()
-    # the default: 256-cell passes for a 512-cell line with one row per thread, one pass with two
+    # the default: 128-cell passes for a 512-cell line with one row per thread, one pass with two
     scn, _ = random_scenario((6, 5, 512), seed=3, pml=[2] * 6, steps=4)
     eng = scn.make(Engine, precision="fp32")
     eng.set_tile("register_rows", 1, 4)
-    assert eng.launch_info()["threads"] == 256
+    assert eng.launch_info()["threads"] == 128
     eng.set_tile("register_rows", 2, 4)
     assert eng.launch_info()["threads"] == 512
     eng.step(2)
@@ -447,8 +447,8 @@ def test_z_pass_arguments_are_checked():
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_z_passes_with_design_region_and_eps_maps(precision):
-    """Design voxels and per-cell eps maps that straddle a pass boundary (z 200..300 across the
-    256-cell passes of a 512-cell line) step exactly as in one pass."""
+    """Design voxels and per-cell eps maps that straddle pass boundaries (z 200..300 across the
+    128-cell passes of a 512-cell line, and the 64-cell ones) step exactly as in one pass."""
     shape = (8, 8, 512)
     scn, rng = random_scenario(shape, seed=5, pml=[3] * 6, steps=30)
     lo, hi = (2, 2, 200), (6, 7, 300)
@@ -460,7 +460,7 @@ def test_z_passes_with_design_region_and_eps_maps(precision):
         eng = scn.make(Engine, precision=precision)
         eng.set_tile("register_rows", 1, 4)
         eng.set_zsegment(zseg)
-        assert eng.launch_info()["threads"] == (zseg or 256)
+        assert eng.launch_info()["threads"] == (zseg or 128)
         eng.step(30)
         runs.append([eng.read_field(c) for c in range(6)])
         eng.close()
@@ -493,7 +493,7 @@ def test_batch_of_pass_and_one_pass_lines_equals_each_engine():
     b.set_tile("register_rows", 4)
     b.step(40)
     threads, _ = b.launch_info()
-    assert threads == 256
+    assert threads == 192  # the 130-cell line in one pass; the 512-cell one in 128-cell passes
     for i, eng in enumerate(engs):
         for c in range(6):
             assert np.array_equal(eng.read_field(c), own[i][c]), (i, c)
