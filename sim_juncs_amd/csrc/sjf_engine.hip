@@ -254,12 +254,12 @@ double inv_eps_at(const sjf_engine* e, long q, int c) {
 bool thread_rows_fit(const sjf_engine* e, int rows) { return rows >= 1 && (long)(rows + 1) * e->g.nz <= 1024; }
 
 // Field bytes one step of these engines moves (E and H read and written) above which the one-row
-// step kernels store with the non-temporal hint: more than the 256 MB Infinity Cache, so a step
-// cannot find the previous step's writes there.  Measured on MI355X (FP32): 1024x768x512 7.17 ->
+// step kernels store with the non-temporal hint.  Measured on MI355X (FP32): 1024x768x512 7.17 ->
 // 6.86 ms per step, 512x512x256 1.230 -> 1.205 ms, the 256x256x128 reverse step 0.282 -> 0.267
-// ms; the 96^3 FP64 box (85 MB of fields) 0.046 -> 0.059 ms with the hint, so it keeps plain
-// stores.
-constexpr double kStreamBytes = 256e6;
+// ms, one 192x192x96 design variant (170 MB of fields) 12.46 -> 12.75 G cell-updates/s per
+// gradient; the 96^3 FP64 box (85 MB, which the 256 MB Infinity Cache holds from one step to the
+// next) 0.046 -> 0.059 ms with the hint, so it keeps plain stores.
+constexpr double kStreamBytes = 128e6;
 double field_bytes(const sjf_engine* e) { return 12.0 * e->prec * (double)e->N; }
 
 // z cells per pass of a register-rows workgroup with `rows` rows per thread: the set value, else
