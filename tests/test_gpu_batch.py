@@ -74,6 +74,34 @@ def test_batch_matches_individual_engines(prec):
         e.close()
 
 
+def test_nontemporal_stores_change_no_bits():
+    """The fused step stores with the non-temporal cache hint when a launch set's fields (E and
+    H read and written, 48 bytes per FP32 cell) exceed 128 MB.  Three FP32 boxes below that
+    alone (43-54 MB each) and above it together (139 MB) step to the same bits alone and as one
+    batch, both in the one-row register tile."""
+    scns = early([scenario.plain_box(n=96, steps=0, precision="fp32", seed=1),
+                  scenario.plain_box(n=96, steps=0, precision="fp32", seed=2),
+                  scenario.plain_box(n=104, steps=0, precision="fp32", seed=3)])
+    fields = [48 * s.cells for s in scns]
+    assert max(fields) < 128e6 < sum(fields)
+    print("[case] non-temporal stores", [s.shape for s in scns], flush=True)
+    single = [s.make(Engine) for s in scns]
+    for e in single:
+        e.set_tile("register_rows", 1, 8)
+        e.step(60)
+    grouped = [s.make(Engine) for s in scns]
+    b = Batch(grouped)
+    b.set_tile("register_rows", 8)
+    b.step(60)
+    assert b.launch_info()[0] == 128  # one row per thread over the 96- and 104-cell lines
+    for e1, e2 in zip(single, grouped):
+        same(e1, e2)
+        assert np.abs(e1.read_field("Ex")).max() > 0
+    b.close()
+    for e in single + grouped:
+        e.close()
+
+
 @pytest.mark.parametrize("tile", [None, ("register_rows", 0), ("register_rows", 3), ("thread_rows", 2)])
 def test_batch_matches_oracle_fp64(tile):
     scns = early(scenario.batched_sweep(n=3, seed=11, scale=0.25, precision="fp64"))
