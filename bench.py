@@ -189,6 +189,8 @@ def sweep(a):
     wall = time.perf_counter() - t
     _, ms_kernel = b.timed_steps(a.kernel_steps)
     threads, groups = b.launch_info()
+    # the batch default for z lines under 96 cells is the row-of-threads form
+    kernel = "k_step" if a.batch_tile and a.batch_tile.startswith("register_rows") else "k_step_rows"
     b.close()
     for e in engs:
         e.close()
@@ -198,7 +200,7 @@ def sweep(a):
     return dict(value=cells * steps / wall, steps=steps, wall=wall,
                 config={"workload": f"sweep_{len(scns)}_instances", "cells": cells, "precision": "fp32",
                         "launch": "one fused launch per step over all instances, hipGraph replay"},
-                roofline=roofline(bytes_step, ms_kernel, "k_step_rows", threads, groups,
+                roofline=roofline(bytes_step, ms_kernel, kernel, threads, groups,
                                   f"sweep_{len(scns)}_instances"), cpu=cpu, dtype="f32")
 
 
